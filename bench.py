@@ -1,0 +1,211 @@
+"""Benchmark of the MI355X CTU+CT step on the 3D ideal-MHD turbulence box.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+
+Workload: 256^3 periodic box, gamma = 5/3, PLM + Roe + CTU + CT, CFL 0.4, FP32
+bulk with FP64 islands; rho = p = 1, uniform B_x = 0.5, random-phase solenoidal
+velocity with per-mode power |k|^-2 over integer modes 1 <= |k| <= 8 (seeded),
+rescaled to rms Mach 0.5.  A step is one full cycle (CFL reduction included).
+Inputs are resident in HBM before the timed region.  The same K steps are run in
+FP64 for the FP64 rate and the FP32-vs-FP64 density error.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s (FP32 vs FP64 GPU) at 3D 256^3 MHD; L1 err vs FP64; adjoint/fwd time"
+GAMMA = 5.0 / 3.0
+HBM_PEAK = 8.0e12
+
+# Algorithmic (compulsory) words per element of each launch: distinct inputs read
+# once plus outputs written once, in units of the storage word (4 B FP32, 8 B FP64).
+ALG_WORDS = {
+    'slopes': 8 + 7,        # rotated prim + normal B in, limited slope out (per extended cell)
+    'edges': 8 + 7 + 14,    # prim + slope in, two interface states out (per cell)
+    'copy': 7 + 14,
+    'predict': 20 + 21,     # two states + face/cell B terms in; U_L, U_R, flux out (per face)
+    'cell_e': 7 + 3,
+    'emf': 12 + 3,          # three face-flux components per face array + cell E in (per edge point)
+    'ctu': 34 + 14,         # states, transverse fluxes, EMF, faces, cell B/v in; states out
+    'ct_half': 3 + 1,
+    'ct_full': 3 + 1,
+    'half_e': 19 + 3,
+    'correct': 15 + 7,
+    'update': 31 + 16,      # U, fluxes, new faces in; cons + prim out (per cell)
+    'cfl': 11,
+}
+
+
+def turbulence_ic(n, seed=2024, kmax=8, mach=0.5):
+    """(prim, cons, faces) of the turbulence box in the reference layout (y, x, z)."""
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n, 1.0 / n)
+    ky, kx, kz = np.meshgrid(k1, k1, k1, indexing='ij')
+    kk = np.sqrt(kx * kx + ky * ky + kz * kz)
+    sel = (kk >= 1.0) & (kk <= kmax)
+    amp = np.zeros_like(kk)
+    amp[sel] = kk[sel] ** -1.0          # per-mode power |k|^-2
+    vh = []
+    for _ in range(3):
+        ph = rng.uniform(0, 2 * np.pi, kk.shape)
+        g = rng.standard_normal(kk.shape)
+        vh.append(amp * g * np.exp(1j * ph))
+    kd = np.where(kk > 0, kk * kk, 1.0)
+    kdotv = (kx * vh[0] + ky * vh[1] + kz * vh[2]) / kd
+    vx = np.real(np.fft.ifftn(vh[0] - kx * kdotv))
+    vy = np.real(np.fft.ifftn(vh[1] - ky * kdotv))
+    vz = np.real(np.fft.ifftn(vh[2] - kz * kdotv))
+    del vh, kdotv, kx, ky, kz, kk, amp
+    rms = np.sqrt(np.mean(vx * vx + vy * vy + vz * vz))
+    s = mach * np.sqrt(GAMMA) / rms
+    prim = np.zeros((8, n, n, n))
+    prim[0] = 1.0
+    prim[1], prim[2], prim[3] = vx * s, vy * s, vz * s
+    prim[4] = 0.5
+    prim[7] = 1.0
+    cons = np.empty_like(prim)
+    cons[0] = prim[0]
+    cons[1:4] = prim[1:4] * prim[0]
+    cons[4:7] = prim[4:7]
+    cons[7] = prim[7] / (GAMMA - 1.0) + 0.5 * prim[0] * (prim[1] ** 2 + prim[2] ** 2 + prim[3] ** 2) \
+        + 0.5 * (prim[4] ** 2 + prim[5] ** 2 + prim[6] ** 2)
+    faces = (np.full((n, n + 1, n), 0.5), np.zeros((n + 1, n, n)), np.zeros((n, n, n + 1)))
+    return prim, cons, faces
+
+
+def div_b(faces, h):
+    fbx, fby, fbz = faces
+    d = (fbx[:, 1:, :] - fbx[:, :-1, :]) + (fby[1:, :, :] - fby[:-1, :, :]) + (fbz[:, :, 1:] - fbz[:, :, :-1])
+    bmax = max(np.max(np.abs(fbx)), np.max(np.abs(fby)), np.max(np.abs(fbz)))
+    return float(np.max(np.abs(d)) / bmax)
+
+
+def run_gpu(prim, cons, faces, n, precision, steps, warmup, profile_steps=3):
+    from pythonmhd_amd._lib import Solver
+    h = 1.0 / n
+    s = Solver((n, n, n), (h, h, h), (1, 1, 1), GAMMA, 0.4, order=1, entropy_fix=False,
+               precision=precision)
+    s.set_state(prim, cons, faces)
+    t, cyc, _ = s.advance(0.0, 1e30, 0, warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    t, cyc, _ = s.advance(t, 1e30, cyc, cyc + steps)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    prof = None
+    if profile_steps:
+        s.set_profiling(True)
+        s.advance(t, 1e30, cyc, cyc + profile_steps)
+        s.synchronize()
+        prof = s.profile()
+        s.set_profiling(False)
+    return s, el, prof
+
+
+def cpu_baseline(n=64, cycles=2):
+    """The FP64 NumPy oracle (single thread) on a bounded sample of the workload."""
+    from oracle import mhd as M
+    prim, cons, faces = turbulence_ic(n)
+    P = M.Params(GAMMA, 0.4, order=1, entropy_fix=False)
+    h = 1.0 / n
+    dt = M.calc_dt(cons, faces, GAMMA, 0.4, (h, h, h))
+    M.step_3d(prim, cons, *faces, dt, h, h, h, 1, 1, 1, P)      # warm-up
+    t0 = time.perf_counter()
+    for _ in range(cycles):
+        dt = M.calc_dt(cons, faces, GAMMA, 0.4, (h, h, h))
+        prim, cons, f0, f1, f2 = M.step_3d(prim, cons, *faces, dt, h, h, h, 1, 1, 1, P)
+        faces = (f0, f1, f2)
+    el = time.perf_counter() - t0
+    return {"value": n ** 3 * cycles / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "FP64 NumPy oracle, same workload at %d^3, %d cycles after 1 warm-up cycle, "
+                      "1 thread (host: %s, os.cpu_count()=%d)" % (n, cycles, _cpu_model(), os.cpu_count())}
+
+
+def _cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def roofline(prof, word_bytes):
+    if not prof:
+        return None
+    best = max(prof.items(), key=lambda kv: kv[1][0])
+    name, (ms, nl, ne) = best
+    base = name.rsplit('_', 1)[0] if name.endswith(('_x', '_y', '_z')) else name
+    words = ALG_WORDS.get(base, ALG_WORDS.get(name))
+    per_launch_s = ms / nl / 1e3
+    bytes_per_launch = words * word_bytes * ne / nl
+    ach = bytes_per_launch / per_launch_s / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": round(ach / (HBM_PEAK / 1e9), 4), "traffic": None, "kernel": name,
+            "avg_launch_ms": round(ms / nl, 4), "alg_bytes_per_launch": int(bytes_per_launch)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--n', type=int, default=256)
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-fp64', action='store_true')
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print('bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d refused)' % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    n, K, W = a.n, a.steps, a.warmup
+    prim, cons, faces = turbulence_ic(n)
+    print('bench: %d^3 turbulence IC ready' % n, file=sys.stderr, flush=True)
+    s32, el32, prof32 = run_gpu(prim, cons, faces, n, 32, K, W)
+    cells = n ** 3
+    v32 = cells * K / el32
+    print('bench: fp32 %.3f ms/step, %.3e cell-updates/s' % (el32 / K * 1e3, v32), file=sys.stderr, flush=True)
+    out = {"metric": METRIC, "value": v32, "unit": "cell-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+           "ms_per_step": el32 / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "3D ideal-MHD turbulence box %d^3 periodic, gamma=5/3, PLM+Roe+CTU+CT, "
+                                  "CFL 0.4, FP32 bulk with FP64 islands" % n,
+                      "cells": cells, "reconstruct_order": 1, "parallelism": "single GPU"},
+           "roofline": roofline(prof32, 4),
+           "hbm_roof_fraction_step": v32 * 64 / HBM_PEAK,
+           "kernel_ms_per_step": {k: round(v[0] / 3, 4) for k, v in sorted(prof32.items())}}
+    if not a.no_fp64:
+        p32, c32, f32 = s32.get_state(0)
+        s32.close()
+        s64, el64, prof64 = run_gpu(prim, cons, faces, n, 64, K, W, profile_steps=3)
+        v64 = cells * K / el64
+        print('bench: fp64 %.3f ms/step, %.3e cell-updates/s' % (el64 / K * 1e3, v64), file=sys.stderr,
+              flush=True)
+        p64, c64, f64 = s64.get_state(0)
+        s64.close()
+        out["fp64"] = {"value": v64, "ms_per_step": el64 / K * 1e3, "roofline": roofline(prof64, 8)}
+        out["l1_rho_fp32_vs_fp64"] = float(np.mean(np.abs(c32[0] - c64[0])))
+        out["divb_fp32"] = div_b(f32, 1.0 / n)
+        out["divb_fp64"] = div_b(f64, 1.0 / n)
+        out["fp32_vs_fp64_steps"] = W + K + 3
+    else:
+        s32.close()
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+        print('bench: cpu baseline %.3e cell-updates/s' % out["cpu_baseline"]["value"], file=sys.stderr,
+              flush=True)
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

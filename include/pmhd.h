@@ -1,0 +1,99 @@
+/* C ABI of the MI355X CTU+CT step (libpmhd.so).
+ *
+ * The boundary replaces PythonMHD's evolve functions
+ *   evolve_1D_mhd  (Source/EvolveFunctions_MHD.py:195)
+ *   evolve_2D_mhd  (Source/EvolveFunctions_MHD.py:393)
+ *   evolve_3D_mhd  (Source/EvolveFunctions_MHD.py:750)
+ * which Simulation.__run_sim dispatches (Source/Simulation.py:312-377).  The
+ * reference has no C interface; the Python front-end pythonmhd_amd.evolve keeps
+ * the reference's signatures and return tuples and calls these entry points
+ * through ctypes.  Arrays crossing this ABI use the reference layout:
+ *   cells  (8, ny, nx, nz)  z contiguous   [2D: (8, ny, nx); 1D: (8, nx)]
+ *   faceBx (ny, nx+1, nz), faceBy (ny+1, nx, nz), faceBz (ny, nx, nz+1)
+ * Host arrays are float64.  Status: 0 ok, negative on error (see
+ * pmhd_last_error); no C++ exception crosses the ABI.  One handle owns one
+ * device state; calls on one handle are not re-entrant.
+ */
+#ifndef PMHD_H
+#define PMHD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PMHD_OK 0
+#define PMHD_EINVAL -1   /* bad shape, boundary condition or parameter */
+#define PMHD_EHIP -2     /* a HIP runtime call failed */
+#define PMHD_ENOMEM -3   /* device allocation failed */
+
+typedef struct pmhd_grid {
+  int32_t ndim;            /* 1, 2 or 3 */
+  int32_t pad0;
+  int64_t nx, ny, nz;      /* cells; unused extents are 1 */
+  double dx, dy, dz;       /* SimulationGrid.dx/dy/dz (Source/SimulationGrid.py:121-126) */
+  int32_t bcx, bcy, bcz;   /* 0 outflow, 1 periodic (Source/PythonMHD_Constants.py:10-11) */
+  int32_t pad1;
+} pmhd_grid;
+
+typedef struct pmhd_params {
+  double gamma, cfl;                     /* simPar gamma, cfl */
+  double min_dens, min_pres, min_energy; /* floors (Source/PrimCons_HelperModule.py:499) */
+  double epsilon;                        /* entropy-fix width */
+  int32_t reconstruct_order;             /* 0 none, 1 PLM, 2 PPM */
+  int32_t entropy_fix;                   /* 0/1 */
+  int32_t precision;                     /* 64: FP64 throughout; 32: FP32 bulk, FP64 islands */
+  int32_t flags;                         /* reserved, 0 */
+} pmhd_params;
+
+typedef struct pmhd_handle pmhd_handle;
+
+/* Allocate the device state and work buffers of one grid. */
+int pmhd_create(const pmhd_grid* grid, const pmhd_params* params, pmhd_handle** out);
+int pmhd_destroy(pmhd_handle* h);
+
+/* Load the state the evolve functions read from the SimulationGrid
+ * (primVars, consVars, faceBx/By/Bz; Source/EvolveFunctions_MHD.py:752-773).
+ * Face pointers that a grid of lower dimension lacks may be NULL. */
+int pmhd_set_state(pmhd_handle* h, const double* prim, const double* cons,
+                   const double* fbx, const double* fby, const double* fbz);
+
+/* Copy the current (which = 0) or second-last (which = 1) state back.  The
+ * second-last state is the one before the last completed cycle, as returned by
+ * the reference evolve functions (Source/EvolveFunctions_MHD.py:1156-1157). */
+int pmhd_get_state(pmhd_handle* h, int32_t which, double* prim, double* cons,
+                   double* fbx, double* fby, double* fbz);
+
+/* CFL timestep of the current state (calcDt_{1,2,3}D_mhd,
+ * Source/TimestepCalculations_HelperModule.py:183-361). */
+int pmhd_compute_dt(pmhd_handle* h, double* dt);
+
+/* One cycle with the given dt (the body of the evolve loop). */
+int pmhd_step(pmhd_handle* h, double dt);
+
+/* The evolve loop: cycles until t_end or max_cycles with the reference's
+ * clipping of the last dt.  Returns the final time and cycle and the time of
+ * the second-last state. */
+int pmhd_advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int64_t max_cycles,
+                 double* t_out, int64_t* cycle_out, double* t_second_last);
+
+/* Wait for all queued work of the handle. */
+int pmhd_synchronize(pmhd_handle* h);
+
+/* Per-launch timing with HIP events on the handle's stream (measurement only).
+ * enable = 1 clears the totals and starts recording; 0 stops. */
+int pmhd_set_profiling(pmhd_handle* h, int32_t enable);
+
+/* Totals recorded since profiling was enabled, one entry per kernel: name k at
+ * names + 32*k (NUL-terminated), summed milliseconds, number of launches and
+ * summed elements (cells, faces or edge points) of those launches. */
+int pmhd_get_profile(pmhd_handle* h, int32_t max_entries, char* names, double* total_ms,
+                     int64_t* launches, int64_t* elements, int32_t* n_entries);
+
+/* Last error message of the handle (or of the last failed pmhd_create when h is NULL). */
+const char* pmhd_last_error(const pmhd_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,204 @@
+"""ctypes binding of the C ABI in include/pmhd.h.
+
+``load()`` opens the in-tree HIP library ``pythonmhd_amd/libpmhd.so`` (built by
+``__graft_entry__.build()``).  There is no fallback: if the library is missing or
+cannot be loaded, ``load()`` raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_LIB = os.path.join(_HERE, 'libpmhd.so')
+
+EXPORTS = ('pmhd_create', 'pmhd_destroy', 'pmhd_set_state', 'pmhd_get_state', 'pmhd_compute_dt',
+           'pmhd_step', 'pmhd_advance', 'pmhd_synchronize', 'pmhd_set_profiling', 'pmhd_get_profile',
+           'pmhd_last_error')
+
+
+class PmhdGrid(ctypes.Structure):
+    _fields_ = [('ndim', ctypes.c_int32), ('pad0', ctypes.c_int32),
+                ('nx', ctypes.c_int64), ('ny', ctypes.c_int64), ('nz', ctypes.c_int64),
+                ('dx', ctypes.c_double), ('dy', ctypes.c_double), ('dz', ctypes.c_double),
+                ('bcx', ctypes.c_int32), ('bcy', ctypes.c_int32), ('bcz', ctypes.c_int32),
+                ('pad1', ctypes.c_int32)]
+
+
+class PmhdParams(ctypes.Structure):
+    _fields_ = [('gamma', ctypes.c_double), ('cfl', ctypes.c_double),
+                ('min_dens', ctypes.c_double), ('min_pres', ctypes.c_double),
+                ('min_energy', ctypes.c_double), ('epsilon', ctypes.c_double),
+                ('reconstruct_order', ctypes.c_int32), ('entropy_fix', ctypes.c_int32),
+                ('precision', ctypes.c_int32), ('flags', ctypes.c_int32)]
+
+
+class PmhdError(RuntimeError):
+    pass
+
+
+_cache = {}
+
+
+def _declare(lib):
+    H = ctypes.c_void_p
+    P = ctypes.c_void_p
+    lib.pmhd_create.argtypes = [ctypes.POINTER(PmhdGrid), ctypes.POINTER(PmhdParams), ctypes.POINTER(H)]
+    lib.pmhd_destroy.argtypes = [H]
+    lib.pmhd_set_state.argtypes = [H, P, P, P, P, P]
+    lib.pmhd_get_state.argtypes = [H, ctypes.c_int32, P, P, P, P, P]
+    lib.pmhd_compute_dt.argtypes = [H, ctypes.POINTER(ctypes.c_double)]
+    lib.pmhd_step.argtypes = [H, ctypes.c_double]
+    lib.pmhd_advance.argtypes = [H, ctypes.c_double, ctypes.c_double, ctypes.c_int64, ctypes.c_int64,
+                                 ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+                                 ctypes.POINTER(ctypes.c_double)]
+    lib.pmhd_synchronize.argtypes = [H]
+    lib.pmhd_set_profiling.argtypes = [H, ctypes.c_int32]
+    lib.pmhd_get_profile.argtypes = [H, ctypes.c_int32, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                     ctypes.POINTER(ctypes.c_int32)]
+    lib.pmhd_last_error.argtypes = [H]
+    lib.pmhd_last_error.restype = ctypes.c_char_p
+    for name in EXPORTS:
+        if name != 'pmhd_last_error':
+            getattr(lib, name).restype = ctypes.c_int
+    return lib
+
+
+def load(path=None):
+    """Open the step library (default: the in-tree HIP build).  Raises OSError /
+    PmhdError when it is absent -- the product has no CPU fallback."""
+    path = os.path.abspath(path or os.environ.get('PMHD_LIBRARY', DEFAULT_LIB))
+    if path in _cache:
+        return _cache[path]
+    if not os.path.exists(path):
+        raise PmhdError('libpmhd not built: %s is missing (run __graft_entry__.build())' % path)
+    lib = _declare(ctypes.CDLL(path))
+    _cache[path] = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Solver:
+    """One device-resident grid state driven through the C ABI."""
+
+    def __init__(self, shape, widths, bcs, gamma, cfl, min_dens=1e-12, min_pres=1e-12,
+                 min_energy=1e-12, order=2, entropy_fix=False, epsilon=0.5, precision=64,
+                 lib=None):
+        self.lib = lib if lib is not None else load()
+        self.ndim = len(shape)
+        self.shape = tuple(int(s) for s in shape)   # (nx,) / (ny, nx) / (ny, nx, nz)
+        g = PmhdGrid()
+        g.ndim = self.ndim
+        if self.ndim == 1:
+            g.nx, g.ny, g.nz = self.shape[0], 1, 1
+        elif self.ndim == 2:
+            g.ny, g.nx, g.nz = self.shape[0], self.shape[1], 1
+        else:
+            g.ny, g.nx, g.nz = self.shape
+        w = list(widths) + [1.0] * (3 - len(widths))
+        b = list(bcs) + [bcs[0]] * (3 - len(bcs))
+        g.dx, g.dy, g.dz = float(w[0]), float(w[1]), float(w[2])
+        g.bcx, g.bcy, g.bcz = int(b[0]), int(b[1]), int(b[2])
+        p = PmhdParams()
+        p.gamma, p.cfl = float(gamma), float(cfl)
+        p.min_dens, p.min_pres, p.min_energy = float(min_dens), float(min_pres), float(min_energy)
+        p.epsilon = float(epsilon)
+        p.reconstruct_order = int(order)
+        p.entropy_fix = 1 if entropy_fix else 0
+        p.precision = int(precision)
+        p.flags = 0
+        self.grid, self.params = g, p
+        h = ctypes.c_void_p()
+        rc = self.lib.pmhd_create(ctypes.byref(g), ctypes.byref(p), ctypes.byref(h))
+        if rc != 0:
+            raise PmhdError('pmhd_create failed (%d): %s' % (rc, self.lib.pmhd_last_error(None).decode()))
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise PmhdError('%s failed (%d): %s' % (what, rc, self.lib.pmhd_last_error(self.h).decode()))
+
+    def face_shapes(self):
+        s = self.shape
+        if self.ndim == 1:
+            return []
+        if self.ndim == 2:
+            ny, nx = s
+            return [(ny, nx + 1), (ny + 1, nx)]
+        ny, nx, nz = s
+        return [(ny, nx + 1, nz), (ny + 1, nx, nz), (ny, nx, nz + 1)]
+
+    def set_state(self, prim, cons, faces=()):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        cons = np.ascontiguousarray(cons, dtype=np.float64)
+        if prim.shape != (8,) + self.shape or cons.shape != (8,) + self.shape:
+            raise ValueError('state shape %s does not match the grid %s' % (prim.shape, self.shape))
+        fs = [np.ascontiguousarray(f, dtype=np.float64) for f in faces]
+        for f, s in zip(fs, self.face_shapes()):
+            if f.shape != s:
+                raise ValueError('face shape %s, expected %s' % (f.shape, s))
+        if len(fs) != len(self.face_shapes()):
+            raise ValueError('expected %d face arrays' % len(self.face_shapes()))
+        fp = [_ptr(f) for f in fs] + [None] * (3 - len(fs))
+        self._check(self.lib.pmhd_set_state(self.h, _ptr(prim), _ptr(cons), *fp), 'pmhd_set_state')
+
+    def get_state(self, which=0):
+        prim = np.empty((8,) + self.shape)
+        cons = np.empty((8,) + self.shape)
+        fs = [np.empty(s) for s in self.face_shapes()]
+        fp = [_ptr(f) for f in fs] + [None] * (3 - len(fs))
+        self._check(self.lib.pmhd_get_state(self.h, which, _ptr(prim), _ptr(cons), *fp), 'pmhd_get_state')
+        return prim, cons, fs
+
+    def compute_dt(self):
+        dt = ctypes.c_double()
+        self._check(self.lib.pmhd_compute_dt(self.h, ctypes.byref(dt)), 'pmhd_compute_dt')
+        return dt.value
+
+    def step(self, dt):
+        self._check(self.lib.pmhd_step(self.h, float(dt)), 'pmhd_step')
+
+    def advance(self, t0, t_end, cycle0, max_cycles):
+        t = ctypes.c_double()
+        cyc = ctypes.c_int64()
+        t2 = ctypes.c_double()
+        self._check(self.lib.pmhd_advance(self.h, float(t0), float(t_end), int(cycle0), int(max_cycles),
+                                          ctypes.byref(t), ctypes.byref(cyc), ctypes.byref(t2)),
+                    'pmhd_advance')
+        return t.value, cyc.value, t2.value
+
+    def set_profiling(self, on):
+        self._check(self.lib.pmhd_set_profiling(self.h, 1 if on else 0), 'pmhd_set_profiling')
+
+    def profile(self):
+        """{kernel name: (total ms, launches, elements)} recorded with HIP events."""
+        m = 64
+        names = ctypes.create_string_buffer(32 * m)
+        ms = (ctypes.c_double * m)()
+        nl = (ctypes.c_int64 * m)()
+        ne = (ctypes.c_int64 * m)()
+        n = ctypes.c_int32()
+        self._check(self.lib.pmhd_get_profile(self.h, m, names, ms, nl, ne, ctypes.byref(n)), 'pmhd_get_profile')
+        out = {}
+        for k in range(n.value):
+            nm = names.raw[32 * k:32 * k + 32].split(b'\0', 1)[0].decode()
+            out[nm] = (ms[k], nl[k], ne[k])
+        return out
+
+    def synchronize(self):
+        self._check(self.lib.pmhd_synchronize(self.h), 'pmhd_synchronize')
+
+    def close(self):
+        if getattr(self, 'h', None) is not None and self.h.value:
+            self.lib.pmhd_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,815 @@
+// Kernel bodies of one CTU+CT cycle: one function per launch, called once per
+// element (cell, face or edge point).  The same bodies run under the HIP
+// executor (one thread per element) and under the host test executor.
+//
+// Layouts are the reference's (Source/SimulationGrid.py:47-169,
+// Source/CT_HelperModule.py:29-103): cells (v, ny, nx, nz) with z contiguous,
+// faces (ny, nx+1, nz) / (ny+1, nx, nz) / (ny, nx, nz+1), edges (ny+1, nx+1, nz+1).
+// 2D uses nz = 1, 1D uses ny = nz = 1.  Interface states and fluxes are stored in
+// the reference's directional 7-vector order (Source/EvolveFunctions_MHD.py:775-807).
+#pragma once
+#include "pmhd_math.h"
+
+namespace pmhd {
+
+enum { OUTFLOW = 0, PERIODIC = 1 };
+
+// launch identifiers (for per-kernel timing); "+ d" for the per-direction ones
+enum KernelId {
+  K_SLOPES = 0, K_EDGES = 3, K_COPY = 6, K_PREDICT = 9, K_CELL_E = 12, K_EMF = 13, K_CTU = 14,
+  K_CT_HALF = 17, K_HALF_E = 20, K_CORRECT = 21, K_CT_FULL = 24, K_UPDATE = 27, K_CFL = 28,
+  K_COUNT = 29
+};
+inline const char* kernel_name(int id) {
+  static const char* names[K_COUNT] = {
+      "slopes_x", "slopes_y", "slopes_z", "edges_x", "edges_y", "edges_z", "copy_x", "copy_y", "copy_z",
+      "predict_x", "predict_y", "predict_z", "cell_e", "emf", "ctu_x", "ctu_y", "ctu_z",
+      "ct_half_x", "ct_half_y", "ct_half_z", "half_e", "correct_x", "correct_y", "correct_z",
+      "ct_full_x", "ct_full_y", "ct_full_z", "update", "cfl"};
+  return (id >= 0 && id < K_COUNT) ? names[id] : "?";
+}
+
+template <class T>
+struct Ctx {
+  int ndim;
+  int64_t ny, nx, nz;
+  int bc[3];         // boundary condition per direction (x, y, z)
+  int bc_zrec;       // boundary condition the z reconstruction/corrections use
+  int order;         // 0 copy, 1 PLM, 2 PPM
+  int efix;
+  T gamma, eps, min_dens, min_pres, min_energy;
+  // per-cycle scalars (all formed on the host in double)
+  T dt, hdt;
+  T dtd[3];          // dt/dx, dt/dy, dt/dz
+  T hdtd[3];         // 0.5*(dt/dx), ...
+  T invd[3];         // 1/dx, ...
+  T width[3];        // dx, dy, dz
+  T emfq[3];         // 0.5*(0.5*(dt/dx)), ...
+  T bx1d;            // uniform B_x of the 1D step
+  double ldh[2];     // 0.5*dt/dy as the reference's 80-bit value, hi + lo (FP64 only)
+  // state
+  const T* U;
+  const T* W;
+  T* U2;
+  T* W2;
+  const T* fb[3];
+  T* fb2[3];
+  T* nfb[3];
+  // work
+  T* WL;
+  T* WR;
+  T* S;
+  T* UL[3];
+  T* UR[3];
+  T* F[3];
+  T* Ec;
+  T* E[3];
+  // element counts
+  PMHD_FI int64_t ncell() const { return ny * nx * nz; }
+  PMHD_FI int64_t n_along(int d) const { return d == 0 ? nx : (d == 1 ? ny : nz); }
+  PMHD_FI int64_t fny(int d) const { return ny + (d == 1); }
+  PMHD_FI int64_t fnx(int d) const { return nx + (d == 0); }
+  PMHD_FI int64_t fnz(int d) const { return nz + (d == 2); }
+  PMHD_FI int64_t nface(int d) const { return fny(d) * fnx(d) * fnz(d); }
+  PMHD_FI int64_t cid(int64_t j, int64_t i, int64_t k) const { return (j * nx + i) * nz + k; }
+  PMHD_FI int64_t fid(int d, int64_t j, int64_t i, int64_t k) const {
+    return (j * fnx(d) + i) * fnz(d) + k;
+  }
+  PMHD_FI int64_t eny() const { return ny + 1; }
+  PMHD_FI int64_t enx() const { return nx + 1; }
+  PMHD_FI int64_t enz() const { return ndim == 3 ? nz + 1 : 1; }
+  PMHD_FI int64_t nedge() const { return eny() * enx() * enz(); }
+  PMHD_FI int64_t eid(int64_t j, int64_t i, int64_t k) const { return (j * enx() + i) * enz() + k; }
+  // extended-cell array for slopes: one ghost each side along d
+  PMHD_FI int64_t sny(int d) const { return ny + 2 * (d == 1); }
+  PMHD_FI int64_t snx(int d) const { return nx + 2 * (d == 0); }
+  PMHD_FI int64_t snz(int d) const { return nz + 2 * (d == 2); }
+  PMHD_FI int64_t nslope(int d) const { return sny(d) * snx(d) * snz(d); }
+};
+
+struct I3 {
+  int64_t j, i, k;
+};
+PMHD_FI int64_t co(const I3& p, int d) { return d == 0 ? p.i : (d == 1 ? p.j : p.k); }
+PMHD_FI I3 with(I3 p, int d, int64_t v) {
+  if (d == 0) p.i = v; else if (d == 1) p.j = v; else p.k = v;
+  return p;
+}
+PMHD_FI I3 unflat(int64_t idx, int64_t ny, int64_t nx, int64_t nz) {
+  (void)ny;
+  I3 p;
+  p.k = idx % nz;
+  int64_t t = idx / nz;
+  p.i = t % nx;
+  p.j = t / nx;
+  return p;
+}
+
+// ghost maps along one axis of n cells
+PMHD_FI int64_t map2(int64_t e, int64_t n, int bc) {  // reconstruction (2 ghosts)
+  if (bc == OUTFLOW) return e < 0 ? 0 : (e >= n ? n - 1 : e);
+  int64_t m = e % n;
+  return m < 0 ? m + n : m;
+}
+PMHD_FI int64_t map1(int64_t e, int64_t n, int bc) {  // one ghost each side
+  if (e < 0) return bc == OUTFLOW ? 0 : n - 1;
+  if (e >= n) return bc == OUTFLOW ? n - 1 : 0;
+  return e;
+}
+PMHD_FI int64_t edge_lo(int64_t e, int64_t n, int bc) { return e == 0 ? (bc == PERIODIC ? n - 1 : 0) : e - 1; }
+PMHD_FI int64_t edge_hi(int64_t e, int64_t n, int bc) { return e == n ? (bc == PERIODIC ? 0 : n - 1) : e; }
+
+// rotated slot r <- 8-variable index, direction d (0 x, 1 y, 2 z)
+PMHD_FI int rot8(int d, int r) {
+  switch (r) {
+    case 0: return 0;
+    case 1: return 1 + d;
+    case 2: return 1 + (d + 1) % 3;
+    case 3: return 1 + (d + 2) % 3;
+    case 4: return 4 + (d + 1) % 3;
+    case 5: return 4 + (d + 2) % 3;
+    default: return 7;
+  }
+}
+// rotated slot r -> directional 7-vector slot
+PMHD_FI int rot_to_dir(int d, int r) {
+  switch (r) {
+    case 1: return 1 + d;
+    case 2: return 1 + (d + 1) % 3;
+    case 3: return 1 + (d + 2) % 3;
+    default: return r;
+  }
+}
+
+template <class T>
+PMHD_FI void load_rot8(const T* A, int64_t N, int64_t c, int d, T w[7]) {
+#pragma unroll
+  for (int r = 0; r < 7; ++r) w[r] = A[rot8(d, r) * N + c];
+}
+template <class T>
+PMHD_FI void to_rot(const T dir[7], int d, T rot[7]) {
+#pragma unroll
+  for (int r = 0; r < 7; ++r) rot[r] = dir[rot_to_dir(d, r)];
+}
+template <class T>
+PMHD_FI void from_rot(const T rot[7], int d, T dir[7]) {
+#pragma unroll
+  for (int r = 0; r < 7; ++r) dir[rot_to_dir(d, r)] = rot[r];
+}
+template <class T>
+PMHD_FI void load7(const T* A, int64_t NF, int64_t f, T v[7]) {
+#pragma unroll
+  for (int r = 0; r < 7; ++r) v[r] = A[r * NF + f];
+}
+template <class T>
+PMHD_FI void store7(T* A, int64_t NF, int64_t f, const T v[7]) {
+#pragma unroll
+  for (int r = 0; r < 7; ++r) A[r * NF + f] = v[r];
+}
+
+// ===========================================================================
+// CFL speeds of one cell
+// ===========================================================================
+template <class T>
+PMHD_FI void k_cfl(const Ctx<T>& c, int64_t idx, T out[3]) {
+  const int64_t N = c.ncell();
+  I3 p = unflat(idx, c.ny, c.nx, c.nz);
+  T u[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) u[v] = c.U[v * N + idx];
+  T bf[3] = {T(0), T(0), T(0)};
+  if (c.ndim >= 2) {
+    bf[0] = c.fb[0][c.fid(0, p.j, p.i, p.k)];
+    bf[1] = c.fb[1][c.fid(1, p.j, p.i, p.k)];
+  }
+  if (c.ndim == 3) bf[2] = c.fb[2][c.fid(2, p.j, p.i, p.k)];
+  cell_speeds(u, bf, c.ndim, c.gamma, out);
+}
+
+// ===========================================================================
+// reconstruction: characteristic-limited slopes on the cells -1..n along d
+// ===========================================================================
+template <class T>
+PMHD_FI void k_slopes(const Ctx<T>& c, int d, int bc, int64_t idx) {
+  const int64_t N = c.ncell();
+  I3 q = unflat(idx, c.sny(d), c.snx(d), c.snz(d));
+  const int64_t n = c.n_along(d);
+  const int64_t e = co(q, d) - 1;
+  I3 pc = with(q, d, map2(e, n, bc));
+  I3 pl = with(q, d, map2(e - 1, n, bc));
+  I3 pr = with(q, d, map2(e + 1, n, bc));
+  T wl[7], wc[7], wr[7];
+  load_rot8(c.W, N, c.cid(pl.j, pl.i, pl.k), d, wl);
+  load_rot8(c.W, N, c.cid(pc.j, pc.i, pc.k), d, wc);
+  load_rot8(c.W, N, c.cid(pr.j, pr.i, pr.k), d, wr);
+  T bn = c.W[(4 + d) * N + c.cid(pc.j, pc.i, pc.k)];
+  PrimEig<T> E;
+  eigen_prim(wc, bn, c.gamma, E);
+  T ps[7];
+  limited_slope(E, wl, wc, wr, ps);
+  store7(c.S, c.nslope(d), idx, ps);
+}
+
+// edges and characteristic tracing of each cell; writes the two faces it owns
+template <class T>
+PMHD_FI void k_edges(const Ctx<T>& c, int d, int bc, int64_t idx) {
+  const int64_t N = c.ncell();
+  I3 p = unflat(idx, c.ny, c.nx, c.nz);
+  const int64_t n = c.n_along(d);
+  const int64_t e = co(p, d);
+  I3 pl = with(p, d, map2(e - 1, n, bc));
+  I3 pr = with(p, d, map2(e + 1, n, bc));
+  T wl[7], wc[7], wr[7];
+  load_rot8(c.W, N, c.cid(pl.j, pl.i, pl.k), d, wl);
+  load_rot8(c.W, N, idx, d, wc);
+  load_rot8(c.W, N, c.cid(pr.j, pr.i, pr.k), d, wr);
+  T bn = c.W[(4 + d) * N + idx];
+  PrimEig<T> E;
+  eigen_prim(wc, bn, c.gamma, E);
+  const int64_t NS = c.nslope(d);
+  auto sidx = [&](int64_t ee) {
+    I3 s = with(p, d, ee + 1);
+    return (s.j * c.snx(d) + s.i) * c.snz(d) + s.k;
+  };
+  T sl[7], sc[7], sr[7];
+  load7(c.S, NS, sidx(e - 1), sl);
+  load7(c.S, NS, sidx(e), sc);
+  load7(c.S, NS, sidx(e + 1), sr);
+  T la[7], ra[7];
+  trace_cell(E, wl, wc, wr, sl, sc, sr, c.dtd[d], c.order, la, ra);
+  T dla[7], dra[7];
+  from_rot(la, d, dla);
+  from_rot(ra, d, dra);
+  const int64_t NF = c.nface(d);
+  I3 f0 = with(p, d, e);
+  I3 f1 = with(p, d, e + 1);
+  store7(c.WR, NF, c.fid(d, f0.j, f0.i, f0.k), dla);
+  store7(c.WL, NF, c.fid(d, f1.j, f1.i, f1.k), dra);
+  if (e == 0) {
+    if (bc == OUTFLOW) {
+      store7(c.WL, NF, c.fid(d, f0.j, f0.i, f0.k), dla);
+    } else {
+      I3 fn = with(p, d, n);
+      store7(c.WR, NF, c.fid(d, fn.j, fn.i, fn.k), dla);
+    }
+  }
+  if (e == n - 1) {
+    if (bc == OUTFLOW) {
+      store7(c.WR, NF, c.fid(d, f1.j, f1.i, f1.k), dra);
+    } else {
+      I3 fz = with(p, d, 0);
+      store7(c.WL, NF, c.fid(d, fz.j, fz.i, fz.k), dra);
+    }
+  }
+}
+
+// first-order interface states (reconstructOrder 0)
+template <class T>
+PMHD_FI void k_copy_states(const Ctx<T>& c, int d, int bc, int64_t idx) {
+  const int64_t N = c.ncell();
+  I3 f = unflat(idx, c.fny(d), c.fnx(d), c.fnz(d));
+  const int64_t n = c.n_along(d);
+  const int64_t e = co(f, d);
+  I3 pl = with(f, d, map1(e - 1, n, bc));
+  I3 pr = with(f, d, map1(e, n, bc));
+  T wl[7], wr[7], dl[7], dr[7];
+  load_rot8(c.W, N, c.cid(pl.j, pl.i, pl.k), d, wl);
+  load_rot8(c.W, N, c.cid(pr.j, pr.i, pr.k), d, wr);
+  from_rot(wl, d, dl);
+  from_rot(wr, d, dr);
+  const int64_t NF = c.nface(d);
+  store7(c.WL, NF, idx, dl);
+  store7(c.WR, NF, idx, dr);
+}
+
+// ===========================================================================
+// predictor: floors, MHD source terms on the reconstructed transverse B,
+// prim -> cons, Roe flux
+// ===========================================================================
+template <class T>
+PMHD_FI T face_val(const Ctx<T>& c, const T* fb, int a, I3 p, int64_t fa) {
+  // value of face array of axis a at face coordinate fa along a, other coords of p
+  I3 q = with(p, a, fa);
+  return fb[c.fid(a, q.j, q.i, q.k)];
+}
+
+// source term of the 3D reconstruction on the (ghost-mapped) cell e along d
+template <class T>
+PMHD_FI void recon_b_term_3d(const Ctx<T>& c, int d, int bc, I3 p, int64_t e, T out[2]) {
+  const int64_t N = c.ncell();
+  const int64_t n = c.n_along(d);
+  const int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
+  I3 pm = with(p, d, map1(e, n, bc));
+  // normal derivative with ghost faces
+  int64_t flo, fhi;
+  if (e < 0) flo = bc == OUTFLOW ? 0 : n - 1; else flo = e;
+  if (e + 1 > n) fhi = bc == OUTFLOW ? n : 1; else fhi = e + 1;
+  T dbn = (face_val(c, c.fb[d], d, p, fhi) - face_val(c, c.fb[d], d, p, flo)) * c.invd[d];
+  T db1 = (face_val(c, c.fb[t1], t1, pm, co(pm, t1) + 1) - face_val(c, c.fb[t1], t1, pm, co(pm, t1))) * c.invd[t1];
+  T db2 = (face_val(c, c.fb[t2], t2, pm, co(pm, t2) + 1) - face_val(c, c.fb[t2], t2, pm, co(pm, t2))) * c.invd[t2];
+  T term1 = minmod_recon(dbn, db1, d == 1);
+  T term2 = minmod_recon(dbn, db2, false);
+  int64_t cm = c.cid(pm.j, pm.i, pm.k);
+  T rho = c.W[cm];
+  T v1 = c.U[(1 + t1) * N + cm] / rho;
+  T v2 = c.U[(1 + t2) * N + cm] / rho;
+  out[0] = c.hdt * (v1 * term1);
+  out[1] = c.hdt * (v2 * term2);
+}
+
+template <class T>
+PMHD_FI void k_predict(const Ctx<T>& c, int d, int64_t idx) {
+  const int64_t N = c.ncell();
+  const int64_t NF = c.nface(d);
+  I3 f = unflat(idx, c.fny(d), c.fnx(d), c.fnz(d));
+  const int64_t e = co(f, d);
+  T wl[7], wr[7];
+  load7(c.WL, NF, idx, wl);
+  load7(c.WR, NF, idx, wr);
+  apply_floors(wl, c.min_dens, c.min_pres);
+  apply_floors(wr, c.min_dens, c.min_pres);
+  T bn;
+  int ke = 0;
+  if (c.ndim == 3) {
+    int bcs = d == 2 ? c.bc_zrec : c.bc[d];
+    T cl[2], cr[2];
+    recon_b_term_3d(c, d, bcs, f, e - 1, cl);
+    recon_b_term_3d(c, d, bcs, f, e, cr);
+    wl[4] = wl[4] + cl[0];
+    wl[5] = wl[5] + cl[1];
+    wr[4] = wr[4] + cr[0];
+    wr[5] = wr[5] + cr[1];
+    bn = c.fb[d][idx];
+    ke = d;
+  } else if (c.ndim == 2) {
+    // getByReconSourceTerms_2D / getBxReconSourceTerms_2D
+    const int64_t n = c.n_along(d);
+    const int bc = c.bc[d];
+    I3 pl = with(f, d, map1(e - 1, n, bc));
+    I3 pr = with(f, d, map1(e, n, bc));
+    int vslot = d == 0 ? 2 : 1;  // x faces use v_y, y faces use v_x
+    T lv = c.W[vslot * N + c.cid(pl.j, pl.i, pl.k)];
+    T rv = c.W[vslot * N + c.cid(pr.j, pr.i, pr.k)];
+    int64_t flo = e - 1 < 0 ? (bc == OUTFLOW ? 0 : n - 1) : e - 1;
+    int64_t fhi = e + 1 > n ? (bc == OUTFLOW ? n : 1) : e + 1;
+    T fm = c.fb[d][idx];
+    T lf = face_val(c, c.fb[d], d, f, flo);
+    T rf = face_val(c, c.fb[d], d, f, fhi);
+    T c0 = c.hdt * lv * ((fm - lf) * c.invd[d]);
+    T c1 = c.hdt * rv * ((rf - fm) * c.invd[d]);
+    int slot = d == 0 ? 4 : 5;
+    wl[slot] = wl[slot] + c0;
+    wr[slot] = wr[slot] + c1;
+    bn = fm;
+  } else {
+    bn = c.bx1d;
+  }
+  T ul[7], ur[7];
+  prim_to_cons(wl, bn, c.gamma, ke, ul);
+  prim_to_cons(wr, bn, c.gamma, ke, ur);
+  store7(c.UL[d], NF, idx, ul);
+  store7(c.UR[d], NF, idx, ur);
+  T rwl[7], rwr[7], rul[7], rur[7], rf[7], df[7];
+  to_rot(wl, d, rwl);
+  to_rot(wr, d, rwr);
+  to_rot(ul, d, rul);
+  to_rot(ur, d, rur);
+  roe_flux(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf);
+  from_rot(rf, d, df);
+  store7(c.F[d], NF, idx, df);
+}
+
+// ===========================================================================
+// cell-centred electric field
+// ===========================================================================
+template <class T>
+PMHD_FI void k_cell_e(const Ctx<T>& c, int64_t idx) {
+  const int64_t N = c.ncell();
+  if (c.ndim == 3) {
+    T rho = c.U[idx], mx = c.U[N + idx], my = c.U[2 * N + idx], mz = c.U[3 * N + idx];
+    T bx = c.W[4 * N + idx], by = c.W[5 * N + idx], bz = c.W[6 * N + idx];
+    c.Ec[idx] = (by * mz - bz * my) / rho;
+    c.Ec[N + idx] = (bz * mx - bx * mz) / rho;
+    c.Ec[2 * N + idx] = (bx * my - by * mx) / rho;
+  } else {
+    c.Ec[idx] = c.W[2 * N + idx] * c.W[4 * N + idx] - c.W[N + idx] * c.W[5 * N + idx];
+  }
+}
+
+// half-step cell electric field from the predictor fluxes and half-step faces
+template <class T>
+PMHD_FI void k_half_e(const Ctx<T>& c, int64_t idx) {
+  const int64_t N = c.ncell();
+  I3 p = unflat(idx, c.ny, c.nx, c.nz);
+  const int64_t NX = c.nface(0), NY = c.nface(1);
+  int64_t fx0 = c.fid(0, p.j, p.i, p.k), fx1 = c.fid(0, p.j, p.i + 1, p.k);
+  int64_t fy0 = c.fid(1, p.j, p.i, p.k), fy1 = c.fid(1, p.j + 1, p.i, p.k);
+  if (c.ndim == 3) {
+    const int64_t NZ = c.nface(2);
+    int64_t fz0 = c.fid(2, p.j, p.i, p.k), fz1 = c.fid(2, p.j, p.i, p.k + 1);
+    T h[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      h[v] = c.U[v * N + idx] - c.hdtd[0] * (c.F[0][v * NX + fx1] - c.F[0][v * NX + fx0])
+             - c.hdtd[1] * (c.F[1][v * NY + fy1] - c.F[1][v * NY + fy0])
+             - c.hdtd[2] * (c.F[2][v * NZ + fz1] - c.F[2][v * NZ + fz0]);
+    }
+    T bx = T(0.5) * (c.nfb[0][fx0] + c.nfb[0][fx1]);
+    T by = T(0.5) * (c.nfb[1][fy0] + c.nfb[1][fy1]);
+    T bz = T(0.5) * (c.nfb[2][fz0] + c.nfb[2][fz1]);
+    c.Ec[idx] = (by * h[3] - bz * h[2]) / h[0];
+    c.Ec[N + idx] = (bz * h[1] - bx * h[3]) / h[0];
+    c.Ec[2 * N + idx] = (bx * h[2] - by * h[1]) / h[0];
+  } else {
+    T h[3];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      T a = c.U[v * N + idx] - c.hdtd[0] * (c.F[0][v * NX + fx1] - c.F[0][v * NX + fx0]);
+      h[v] = a - c.hdtd[1] * (c.F[1][v * NY + fy1] - c.F[1][v * NY + fy0]);
+    }
+    T bx = (c.nfb[0][fx0] + c.nfb[0][fx1]) / T(2);
+    T by = (c.nfb[1][fy0] + c.nfb[1][fy1]) / T(2);
+    T vx = h[1] / h[0];
+    T vy = h[2] / h[0];
+    c.Ec[idx] = vy * bx - vx * by;
+  }
+}
+
+// ===========================================================================
+// corner EMFs (Source/CornerEMF_HelperModule.py:440-952)
+// ===========================================================================
+template <class T>
+PMHD_FI T fluxv(const Ctx<T>& c, int d, int v, int64_t j, int64_t i, int64_t k) {
+  return c.F[d][v * c.nface(d) + c.fid(d, j, i, k)];
+}
+
+template <class T>
+PMHD_FI void k_emf(const Ctx<T>& c, int64_t idx) {
+  const int64_t N = c.ncell();
+  I3 p = unflat(idx, c.eny(), c.enx(), c.enz());
+  const int bx = c.bc[0], by = c.bc[1], bz = c.bc[2];
+  const int64_t xl = edge_lo(p.i, c.nx, bx), xr = edge_hi(p.i, c.nx, bx);
+  const int64_t yl = edge_lo(p.j, c.ny, by), yr = edge_hi(p.j, c.ny, by);
+  if (c.ndim == 2) {
+    T t0 = fluxv(c, 1, 0, p.j, xr, 0), t5 = fluxv(c, 1, 5, p.j, xr, 0);
+    T lt0 = fluxv(c, 1, 0, p.j, xl, 0), lt5 = fluxv(c, 1, 5, p.j, xl, 0);
+    T lf0 = fluxv(c, 0, 0, yr, p.i, 0), lf4 = fluxv(c, 0, 4, yr, p.i, 0);
+    T ul0 = fluxv(c, 0, 0, yl, p.i, 0), ul4 = fluxv(c, 0, 4, yl, p.i, 0);
+    T cE = c.Ec[c.cid(yr, xr, 0)], lE = c.Ec[c.cid(yr, xl, 0)];
+    T uE = c.Ec[c.cid(yl, xr, 0)], ulE = c.Ec[c.cid(yl, xl, 0)];
+    T x1 = upwind(lt0, lt5 - ul4 - ulE, lt5 - lf4 - lE, lt5 - T(0.5) * (lf4 + ul4 + lE + ulE));
+    T x2 = upwind(t0, t5 - ul4 - uE, t5 - lf4 - cE, t5 - T(0.5) * (lf4 + ul4 + cE + uE));
+    T y1 = upwind(ul0, -ul4 + lt5 - ulE, -ul4 + t5 - uE, -ul4 + T(0.5) * (lt5 + t5 - ulE - uE));
+    T y2 = upwind(lf0, -lf4 + lt5 - lE, -lf4 + t5 - cE, -lf4 + T(0.5) * (lt5 + t5 - lE - cE));
+    c.E[2][idx] = T(0.25) * (x1 + x2 + y1 + y2);
+    return;
+  }
+  const int64_t zl = edge_lo(p.k, c.nz, bz), zr = edge_hi(p.k, c.nz, bz);
+  const T* Exc = c.Ec;
+  const T* Eyc = c.Ec + N;
+  const T* Ezc = c.Ec + 2 * N;
+  {  // E_x (getXCornerEMF_3D): faces y (G) and z (H)
+    T b0 = fluxv(c, 2, 0, yr, xr, p.k), b5 = fluxv(c, 2, 5, yr, xr, p.k);
+    T ub0 = fluxv(c, 2, 0, yl, xr, p.k), ub5 = fluxv(c, 2, 5, yl, xr, p.k);
+    T t0 = fluxv(c, 1, 0, p.j, xr, zr), t4 = fluxv(c, 1, 4, p.j, xr, zr);
+    T bt0 = fluxv(c, 1, 0, p.j, xr, zl), bt4 = fluxv(c, 1, 4, p.j, xr, zl);
+    T cE = Exc[c.cid(yr, xr, zr)], bE = Exc[c.cid(yr, xr, zl)];
+    T uE = Exc[c.cid(yl, xr, zr)], ubE = Exc[c.cid(yl, xr, zl)];
+    T z1 = upwind(bt0, ub5 - ubE, b5 - bE, T(0.5) * (ub5 + b5 - ubE - bE));
+    T z2 = upwind(t0, ub5 - uE, b5 - cE, T(0.5) * (ub5 + b5 - uE - cE));
+    T y1 = upwind(ub0, -bt4 - ubE, -t4 - uE, T(0.5) * (-bt4 - t4 - ubE - uE));
+    T y2 = upwind(b0, -bt4 - bE, -t4 - cE, T(0.5) * (-bt4 - t4 - bE - cE));
+    c.E[0][idx] = T(0.25) * (b5 + ub5 - t4 - bt4 + y1 + y2 + z1 + z2);
+  }
+  {  // E_y (getYCornerEMF_3D): faces x (F) and z (H)
+    T b0 = fluxv(c, 2, 0, yr, xr, p.k), b4 = fluxv(c, 2, 4, yr, xr, p.k);
+    T lb0 = fluxv(c, 2, 0, yr, xl, p.k), lb4 = fluxv(c, 2, 4, yr, xl, p.k);
+    T lf0 = fluxv(c, 0, 0, yr, p.i, zr), lf5 = fluxv(c, 0, 5, yr, p.i, zr);
+    T blf0 = fluxv(c, 0, 0, yr, p.i, zl), blf5 = fluxv(c, 0, 5, yr, p.i, zl);
+    T cE = Eyc[c.cid(yr, xr, zr)], bE = Eyc[c.cid(yr, xr, zl)];
+    T lE = Eyc[c.cid(yr, xl, zr)], blE = Eyc[c.cid(yr, xl, zl)];
+    T z1 = upwind(blf0, -lb4 - blE, -b4 - bE, T(0.5) * (-lb4 - b4 - blE - bE));
+    T z2 = upwind(lf0, -lb4 - lE, -b4 - cE, T(0.5) * (-lb4 - b4 - lE - cE));
+    T x1 = upwind(lb0, blf5 - blE, lf5 - lE, T(0.5) * (blf5 + lf5 - blE - lE));
+    T x2 = upwind(b0, blf5 - bE, lf5 - cE, T(0.5) * (blf5 + lf5 - bE - cE));
+    c.E[1][idx] = T(0.25) * (lf5 + blf5 - b4 - lb4 + x1 + x2 + z1 + z2);
+  }
+  {  // E_z (getZCornerEMF_3D): faces x (F) and y (G)
+    T t0 = fluxv(c, 1, 0, p.j, xr, zr), t5 = fluxv(c, 1, 5, p.j, xr, zr);
+    T lt0 = fluxv(c, 1, 0, p.j, xl, zr), lt5 = fluxv(c, 1, 5, p.j, xl, zr);
+    T lf0 = fluxv(c, 0, 0, yr, p.i, zr), lf4 = fluxv(c, 0, 4, yr, p.i, zr);
+    T ul0 = fluxv(c, 0, 0, yl, p.i, zr), ul4 = fluxv(c, 0, 4, yl, p.i, zr);
+    T cE = Ezc[c.cid(yr, xr, zr)], lE = Ezc[c.cid(yr, xl, zr)];
+    T uE = Ezc[c.cid(yl, xr, zr)], ulE = Ezc[c.cid(yl, xl, zr)];
+    T x1 = upwind(lt0, -ul4 - ulE, -lf4 - lE, T(-0.5) * (lf4 + ul4 + lE + ulE));
+    T x2 = upwind(t0, -ul4 - uE, -lf4 - cE, T(-0.5) * (lf4 + ul4 + cE + uE));
+    T y1 = upwind(ul0, lt5 - ulE, t5 - uE, T(0.5) * (lt5 + t5 - ulE - uE));
+    T y2 = upwind(lf0, lt5 - lE, t5 - cE, T(0.5) * (lt5 + t5 - lE - cE));
+    c.E[2][idx] = T(0.25) * (lt5 + t5 - ul4 - lf4 + x1 + x2 + y1 + y2);
+  }
+}
+
+// ===========================================================================
+// CTU transverse corrections of the interface states
+// ===========================================================================
+template <class T>
+PMHD_FI void ctu_source_3d(const Ctx<T>& c, int d, int bc, I3 f, int64_t e, T src[7]) {
+  const int64_t N = c.ncell();
+  const int64_t n = c.n_along(d);
+  const int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
+  I3 pm = with(f, d, map1(e, n, bc));
+  int64_t flo, fhi;
+  if (e < 0) flo = bc == OUTFLOW ? 0 : n - 1; else flo = e;
+  if (e + 1 > n) fhi = bc == OUTFLOW ? (d == 2 ? n - 1 : n) : 1; else fhi = e + 1;
+  T dd = face_val(c, c.fb[d], d, f, fhi) - face_val(c, c.fb[d], d, f, flo);
+  T d1 = face_val(c, c.fb[t1], t1, pm, co(pm, t1) + 1) - face_val(c, c.fb[t1], t1, pm, co(pm, t1));
+  T d2 = face_val(c, c.fb[t2], t2, pm, co(pm, t2) + 1) - face_val(c, c.fb[t2], t2, pm, co(pm, t2));
+  T dbn, db1, db2;
+  if (d == 0) {
+    dbn = dd * c.invd[d]; db1 = d1 * c.invd[t1]; db2 = d2 * c.invd[t2];
+  } else {
+    dbn = dd / c.width[d]; db1 = d1 / c.width[t1]; db2 = d2 / c.width[t2];
+  }
+  T mm1 = minmod_ctu(db1, dbn);
+  T mm2 = minmod_ctu(db2, dbn);
+  int64_t cm = c.cid(pm.j, pm.i, pm.k);
+  T cb[3] = {c.W[4 * N + cm], c.W[5 * N + cm], c.W[6 * N + cm]};
+  T cv[3] = {c.W[1 * N + cm], c.W[2 * N + cm], c.W[3 * N + cm]};
+  T v1 = cv[t1], v2 = cv[t2], b1 = cb[t1], b2 = cb[t2];
+  src[0] = T(0);
+  src[1] = c.hdt * cb[0] * dbn;
+  src[2] = c.hdt * cb[1] * dbn;
+  src[3] = c.hdt * cb[2] * dbn;
+  src[4] = c.hdt * v1 * (-mm2);
+  src[5] = c.hdt * v2 * (-mm1);
+  src[6] = c.hdt * (b1 * v1 * (-mm2) + b2 * v2 * (-mm1));
+}
+
+// one transverse direction tc of the 3D CTU update of a d-face state sitting in
+// cell column cell_e (already ghost-mapped) -- flux difference plus EMF term
+template <class T>
+PMHD_FI void ctu_apply_3d(const Ctx<T>& c, int d, int tc, I3 pcell, int64_t hi_override,
+                          I3 fedge, int64_t layer, T u[7]) {
+  const int t1 = (d + 1) % 3;
+  const int64_t NF = c.nface(tc);
+  I3 lo = pcell;
+  I3 hi = with(pcell, tc, co(pcell, tc) + 1);
+  if (hi_override >= 0) hi = with(hi, 1, hi_override);
+  int64_t flo = c.fid(tc, lo.j, lo.i, lo.k);
+  int64_t fhi = c.fid(tc, hi.j, hi.i, hi.k);
+  T h = c.hdtd[tc];
+  const int vs[5] = {0, 1, 2, 3, 6};
+  // x faces, y fluxes: the reference's 80-bit factor (FP64 instantiation only)
+  const bool ld = sizeof(T) == sizeof(double) && d == 0 && tc == 1;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    int v = vs[q];
+    T x = c.F[tc][v * NF + fhi] - c.F[tc][v * NF + flo];
+    if (ld)
+      u[v] = T(ld_fms(double(u[v]), c.ldh[0], c.ldh[1], double(x), false));
+    else
+      u[v] = u[v] - h * x;
+  }
+  // EMF stencil E(+t1, +t2) around the face in the layer of this state
+  const int t2 = (d + 2) % 3;
+  I3 b = with(fedge, d, layer);
+  I3 b10 = with(b, t1, co(b, t1) + 1);
+  I3 b01 = with(b, t2, co(b, t2) + 1);
+  I3 b11 = with(b10, t2, co(b, t2) + 1);
+  const T* E = c.E[d];
+  T e00 = E[c.eid(b.j, b.i, b.k)], e10 = E[c.eid(b10.j, b10.i, b10.k)];
+  T e01 = E[c.eid(b01.j, b01.i, b01.k)], e11 = E[c.eid(b11.j, b11.i, b11.k)];
+  T q = c.emfq[tc];
+  if (tc == t1) {
+    T y = e10 - e00 + e11 - e01;
+    if (ld)
+      u[5] = T(ld_fms(double(u[5]), 0.5 * c.ldh[0], 0.5 * c.ldh[1], double(y), true));
+    else
+      u[5] = u[5] + q * y;
+  } else {
+    u[4] = u[4] - q * (e01 - e00 + e11 - e10);
+  }
+}
+
+template <class T>
+PMHD_FI void k_ctu(const Ctx<T>& c, int d, int64_t idx) {
+  const int64_t N = c.ncell();
+  const int64_t NF = c.nface(d);
+  I3 f = unflat(idx, c.fny(d), c.fnx(d), c.fnz(d));
+  const int64_t n = c.n_along(d);
+  const int bc = c.bc[d];
+  const int64_t e = co(f, d);
+  T ul[7], ur[7];
+  load7(c.UL[d], NF, idx, ul);
+  load7(c.UR[d], NF, idx, ur);
+  I3 pl = with(f, d, map1(e - 1, n, bc));
+  I3 pr = with(f, d, map1(e, n, bc));
+  if (c.ndim == 3) {
+    const int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
+    const int first = d == 1 ? t2 : t1;
+    const int second = d == 1 ? t1 : t2;
+    int64_t layer_l = e == 0 ? (bc == PERIODIC ? n - 1 : 0) : e - 1;
+    int64_t layer_r = e;
+    // y-outflow: the bottom ghost of the high x-face flux sits in row nx-1
+    int64_t hov = (d == 1 && bc == OUTFLOW && e == n) ? c.nx - 1 : -1;
+    for (int s = 0; s < 2; ++s) {
+      int tc = s == 0 ? first : second;
+      ctu_apply_3d(c, d, tc, pl, -1, f, layer_l, ul);
+      ctu_apply_3d(c, d, tc, pr, tc == 0 ? hov : -1, f, layer_r, ur);
+    }
+    T sl[7], sr[7];
+    ctu_source_3d(c, d, bc, f, e - 1, sl);
+    ctu_source_3d(c, d, bc, f, e, sr);
+#pragma unroll
+    for (int v = 1; v < 7; ++v) {
+      ul[v] = ul[v] + sl[v];
+      ur[v] = ur[v] + sr[v];
+    }
+  } else {
+    // 2D (getXTransverseUpdates_2D / getYTransverseUpdates_2D)
+    const int tc = 1 - d;
+    const int64_t NT = c.nface(tc);
+    const T h = c.hdtd[tc];
+    int64_t cl = map1(e - 1, n, bc), cr = map1(e, n, bc);
+    int64_t el = e - 1, er = e;  // ext cells for the source term
+    auto src2 = [&](int64_t ee, T s[7]) {
+      int64_t cm;
+      int64_t flo, fhi;
+      if (d == 0) {
+        cm = map1(ee, n, bc);
+        flo = ee < 0 ? (bc == OUTFLOW ? 0 : n - 1) : ee;
+        fhi = ee + 1 > n ? (bc == OUTFLOW ? n : 1) : ee + 1;
+      } else if (bc == OUTFLOW) {
+        cm = ee < 0 ? 1 : (ee >= n ? n - 1 : ee);
+        flo = ee < 0 ? 1 : ee;
+        fhi = ee + 1 > n ? n - 1 : ee + 1;
+      } else {
+        cm = map1(ee, n, bc);
+        flo = ee < 0 ? n - 1 : ee;
+        fhi = ee + 1 > n ? 1 : ee + 1;
+      }
+      I3 pc = with(f, d, cm);
+      int64_t ci = c.cid(pc.j, pc.i, pc.k);
+      T db = face_val(c, c.fb[d], d, f, fhi) - face_val(c, c.fb[d], d, f, flo);
+      T cbx = c.W[4 * N + ci], cby = c.W[5 * N + ci], cbz = c.W[6 * N + ci], cvz = c.W[3 * N + ci];
+      T inv = c.invd[d];
+      s[0] = T(0);
+      s[1] = cbx * db * inv;
+      s[2] = cby * db * inv;
+      s[3] = cbz * db * inv;
+      s[6] = (cbz * cvz) * db * inv;
+      if (d == 0) { s[4] = T(0); s[5] = cvz * db * inv; }
+      else { s[4] = cvz * db * inv; s[5] = T(0); }
+    };
+    T sl[7], sr[7];
+    src2(el, sl);
+    src2(er, sr);
+    for (int side = 0; side < 2; ++side) {
+      I3 pc = with(f, d, side == 0 ? cl : cr);
+      I3 hi = with(pc, tc, co(pc, tc) + 1);
+      int64_t flo = c.fid(tc, pc.j, pc.i, pc.k), fhi = c.fid(tc, hi.j, hi.i, hi.k);
+      T g[7];
+#pragma unroll
+      for (int v = 0; v < 7; ++v) g[v] = c.F[tc][v * NT + fhi] - c.F[tc][v * NT + flo];
+      // slot remap: the transverse flux's in-plane B component moves to the
+      // out-of-plane B_z slot of this direction, its own B slot is zeroed
+      T dr[7] = {g[0], g[1], g[2], g[3], T(0), T(0), g[6]};
+      if (d == 0) dr[5] = g[4]; else dr[4] = g[5];
+      T* u = side == 0 ? ul : ur;
+      const T* s = side == 0 ? sl : sr;
+#pragma unroll
+      for (int v = 0; v < 7; ++v) u[v] = u[v] + ((-h) * dr[v] + c.hdt * s[v]);
+      apply_floors(u, c.min_dens, c.min_energy);
+    }
+  }
+  store7(c.UL[d], NF, idx, ul);
+  store7(c.UR[d], NF, idx, ur);
+}
+
+// ===========================================================================
+// constrained transport of the face field (Source/EvolveFunctions_MHD.py:1003-1028,
+// :1082-1104 for 3D; :579-586, :643-649 for 2D)
+// ===========================================================================
+template <class T>
+PMHD_FI void k_ct(const Ctx<T>& c, int d, const T* fin, T* fout, const T h[3], int64_t idx) {
+  I3 f = unflat(idx, c.fny(d), c.fnx(d), c.fnz(d));
+  if (c.ndim == 2) {
+    const T* E = c.E[2];
+    if (d == 0) {
+      fout[idx] = fin[idx] - h[1] * (E[c.eid(f.j + 1, f.i, 0)] - E[c.eid(f.j, f.i, 0)]);
+    } else {
+      fout[idx] = fin[idx] + h[0] * (E[c.eid(f.j, f.i + 1, 0)] - E[c.eid(f.j, f.i, 0)]);
+    }
+    return;
+  }
+  const T* Ex = c.E[0];
+  const T* Ey = c.E[1];
+  const T* Ez = c.E[2];
+  T a, b;
+  if (d == 0) {
+    a = h[2] * (Ey[c.eid(f.j, f.i, f.k + 1)] - Ey[c.eid(f.j, f.i, f.k)]);
+    b = h[1] * (Ez[c.eid(f.j + 1, f.i, f.k)] - Ez[c.eid(f.j, f.i, f.k)]);
+  } else if (d == 1) {
+    a = h[0] * (Ez[c.eid(f.j, f.i + 1, f.k)] - Ez[c.eid(f.j, f.i, f.k)]);
+    b = h[2] * (Ex[c.eid(f.j, f.i, f.k + 1)] - Ex[c.eid(f.j, f.i, f.k)]);
+  } else {
+    a = h[1] * (Ex[c.eid(f.j + 1, f.i, f.k)] - Ex[c.eid(f.j, f.i, f.k)]);
+    b = h[0] * (Ey[c.eid(f.j, f.i + 1, f.k)] - Ey[c.eid(f.j, f.i, f.k)]);
+  }
+  fout[idx] = fin[idx] + (a - b);
+}
+
+// ===========================================================================
+// corrector: cons -> prim with the half-step normal field, Roe flux
+// ===========================================================================
+template <class T>
+PMHD_FI void k_correct(const Ctx<T>& c, int d, int64_t idx) {
+  const int64_t NF = c.nface(d);
+  T ul[7], ur[7], wl[7], wr[7];
+  load7(c.UL[d], NF, idx, ul);
+  load7(c.UR[d], NF, idx, ur);
+  T bn = c.ndim == 1 ? c.bx1d : c.nfb[d][idx];
+  int ke = c.ndim == 3 ? d : 0;
+  cons_to_prim(ul, bn, c.gamma, ke, wl);
+  cons_to_prim(ur, bn, c.gamma, ke, wr);
+  T rwl[7], rwr[7], rul[7], rur[7], rf[7], df[7];
+  to_rot(wl, d, rwl);
+  to_rot(wr, d, rwr);
+  to_rot(ul, d, rul);
+  to_rot(ur, d, rur);
+  roe_flux(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf);
+  from_rot(rf, d, df);
+  store7(c.F[d], NF, idx, df);
+}
+
+// ===========================================================================
+// conservative update of the cells and the new primitive state
+// ===========================================================================
+template <class T>
+PMHD_FI void k_update(const Ctx<T>& c, int64_t idx) {
+  const int64_t N = c.ncell();
+  I3 p = unflat(idx, c.ny, c.nx, c.nz);
+  // X-directional conserved vector (rho, Mx, My, Mz, By, Bz, E)
+  const int dv[7] = {0, 1, 2, 3, 5, 6, 7};
+  T u[7];
+#pragma unroll
+  for (int v = 0; v < 7; ++v) u[v] = c.U[dv[v] * N + idx];
+  const int64_t NX = c.nface(0);
+  int64_t fx0 = c.fid(0, p.j, p.i, p.k), fx1 = c.fid(0, p.j, p.i + 1, p.k);
+  T bx, w[7];
+  if (c.ndim == 1) {
+#pragma unroll
+    for (int v = 0; v < 7; ++v) u[v] = u[v] - c.dtd[0] * (c.F[0][v * NX + fx1] - c.F[0][v * NX + fx0]);
+    bx = c.bx1d;
+    cons_to_prim(u, bx, c.gamma, 0, w);
+  } else if (c.ndim == 2) {
+    const int64_t NY = c.nface(1);
+    int64_t fy0 = c.fid(1, p.j, p.i, p.k), fy1 = c.fid(1, p.j + 1, p.i, p.k);
+    T nbz = u[5] - c.dtd[0] * (c.F[0][5 * NX + fx1] - c.F[0][5 * NX + fx0])
+            - c.dtd[1] * (c.F[1][4 * NY + fy1] - c.F[1][4 * NY + fy0]);
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      u[v] = u[v] - c.dtd[0] * (c.F[0][v * NX + fx1] - c.F[0][v * NX + fx0]);
+      u[v] = u[v] - c.dtd[1] * (c.F[1][v * NY + fy1] - c.F[1][v * NY + fy0]);
+    }
+    bx = (c.fb2[0][fx0] + c.fb2[0][fx1]) / T(2);
+    T by = (c.fb2[1][fy0] + c.fb2[1][fy1]) / T(2);
+    u[5] = nbz;
+    u[4] = by;
+    cons_to_prim(u, bx, c.gamma, 0, w);
+  } else {
+    const int64_t NY = c.nface(1), NZ = c.nface(2);
+    int64_t fy0 = c.fid(1, p.j, p.i, p.k), fy1 = c.fid(1, p.j + 1, p.i, p.k);
+    int64_t fz0 = c.fid(2, p.j, p.i, p.k), fz1 = c.fid(2, p.j, p.i, p.k + 1);
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      u[v] = u[v] - c.dtd[0] * (c.F[0][v * NX + fx1] - c.F[0][v * NX + fx0]);
+      u[v] = u[v] - c.dtd[1] * (c.F[1][v * NY + fy1] - c.F[1][v * NY + fy0]);
+      u[v] = u[v] - c.dtd[2] * (c.F[2][v * NZ + fz1] - c.F[2][v * NZ + fz0]);
+    }
+    bx = T(0.5) * (c.fb2[0][fx0] + c.fb2[0][fx1]);
+    T by = T(0.5) * (c.fb2[1][fy0] + c.fb2[1][fy1]);
+    T bz = T(0.5) * (c.fb2[2][fz0] + c.fb2[2][fz1]);
+    u[4] = by;
+    u[5] = bz;
+    cons_to_prim(u, bx, c.gamma, 0, w);
+  }
+  c.U2[idx] = u[0];
+  c.U2[N + idx] = u[1];
+  c.U2[2 * N + idx] = u[2];
+  c.U2[3 * N + idx] = u[3];
+  c.U2[4 * N + idx] = bx;
+  c.U2[5 * N + idx] = u[4];
+  c.U2[6 * N + idx] = u[5];
+  c.U2[7 * N + idx] = u[6];
+  c.W2[idx] = w[0];
+  c.W2[N + idx] = w[1];
+  c.W2[2 * N + idx] = w[2];
+  c.W2[3 * N + idx] = w[3];
+  c.W2[4 * N + idx] = bx;
+  c.W2[5 * N + idx] = w[4];
+  c.W2[6 * N + idx] = w[5];
+  c.W2[7 * N + idx] = w[6];
+}
+
+}  // namespace pmhd

@@ -1,0 +1,114 @@
+"""Drop-in evolve functions with PythonMHD's signatures and return tuples
+(Source/EvolveFunctions_MHD.py:195, :393, :750), running the step on the MI355X
+through libpmhd (include/pmhd.h).
+
+Each function reads the duck-typed grid attributes the reference reads
+(primVars, consVars, faceBx/faceBy/faceBz, dx/dy/dz, BcX/BcY/BcZ), advances from
+tStart until tEnd or maxCycles with the reference's clipping of the last dt, and
+returns the final state, the time and cycle reached, and the second-last state.
+
+Extension: ``reconstructOrder=1`` selects PLM (absent from the reference) and the
+keyword ``precision=32`` runs the FP32-bulk step.
+"""
+import numpy as np
+
+from . import constants as C
+from ._lib import Solver
+
+_solver_cache = {}
+
+
+def _solver(key, make):
+    s = _solver_cache.get(key)
+    if s is None:
+        _solver_cache.clear()   # one resident grid at a time
+        s = make()
+        _solver_cache[key] = s
+    return s
+
+
+def release_device_state():
+    """Free the device buffers of the cached solver."""
+    for s in _solver_cache.values():
+        s.close()
+    _solver_cache.clear()
+
+
+def _check_common(tStart, tEnd, startCycle):
+    assert tStart <= tEnd
+    assert startCycle >= 0
+
+
+def _run(shape, widths, bcs, prim, cons, faces, tStart, tEnd, startCycle, maxCycles, gamma, cfl,
+         minDens, minPres, minEnergy, reconstructOrder, entropyFix, epsilon, precision):
+    key = (shape, tuple(widths), tuple(bcs), float(gamma), float(cfl), float(minDens), float(minPres),
+           float(minEnergy), int(reconstructOrder), bool(entropyFix), float(epsilon), int(precision))
+    s = _solver(key, lambda: Solver(shape, widths, bcs, gamma, cfl, minDens, minPres, minEnergy,
+                                    reconstructOrder, entropyFix, epsilon, precision))
+    s.set_state(prim, cons, faces)
+    t, cyc, t2 = s.advance(tStart, tEnd, startCycle, maxCycles)
+    p, c, f = s.get_state(0)
+    p2, c2, f2 = s.get_state(1)
+    return p, c, f, t, cyc, p2, c2, f2, t2
+
+
+def evolve_1D_mhd(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
+                  minEnergy, reconstructOrder=0, entropyFix=False, epsilon=0.5,
+                  precision=C.DEFAULT_PRECISION):
+    prim = np.asarray(simulationGrid.primVars, dtype=np.float64)
+    assert prim.ndim == 2 and prim.shape[0] == 8 and prim.shape[1] > 0
+    cons = np.asarray(simulationGrid.consVars, dtype=np.float64)
+    assert prim.shape == cons.shape
+    dx = simulationGrid.dx
+    assert dx > 0
+    bcx = simulationGrid.BcX
+    assert bcx == C.OUTFLOW or bcx == C.PERIODIC
+    _check_common(tStart, tEnd, startCycle)
+    p, c, _, t, cyc, p2, c2, _, t2 = _run(
+        (prim.shape[1],), (dx,), (bcx,), prim, cons, (), tStart, tEnd, startCycle, maxCycles, gamma, cfl,
+        minDens, minPres, minEnergy, reconstructOrder, entropyFix, epsilon, precision)
+    return p, c, t, cyc, p2, c2, t2
+
+
+def evolve_2D_mhd(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
+                  minEnergy, reconstructOrder=0, entropyFix=False, epsilon=0.5,
+                  precision=C.DEFAULT_PRECISION):
+    prim = np.asarray(simulationGrid.primVars, dtype=np.float64)
+    bcx, bcy = simulationGrid.BcX, simulationGrid.BcY
+    assert bcx == C.OUTFLOW or bcx == C.PERIODIC
+    assert bcy == C.OUTFLOW or bcy == C.PERIODIC
+    assert prim.ndim == 3 and prim.shape[0] == 8 and prim.shape[1] > 0
+    cons = np.asarray(simulationGrid.consVars, dtype=np.float64)
+    assert prim.shape == cons.shape
+    dx, dy = simulationGrid.dx, simulationGrid.dy
+    assert dx > 0
+    assert dy > 0
+    _check_common(tStart, tEnd, startCycle)
+    faces = (simulationGrid.faceBx, simulationGrid.faceBy)
+    p, c, f, t, cyc, p2, c2, f2, t2 = _run(
+        prim.shape[1:], (dx, dy), (bcx, bcy), prim, cons, faces, tStart, tEnd, startCycle, maxCycles,
+        gamma, cfl, minDens, minPres, minEnergy, reconstructOrder, entropyFix, epsilon, precision)
+    return p, c, f[0], f[1], t, cyc, p2, c2, f2[0], f2[1], t2
+
+
+def evolve_3D_mhd(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
+                  minEnergy, reconstructOrder=0, entropyFix=True, epsilon=0.5,
+                  precision=C.DEFAULT_PRECISION):
+    prim = np.asarray(simulationGrid.primVars, dtype=np.float64)
+    bcx, bcy, bcz = simulationGrid.BcX, simulationGrid.BcY, simulationGrid.BcZ
+    for b in (bcx, bcy, bcz):
+        assert b == C.OUTFLOW or b == C.PERIODIC
+    assert prim.ndim == 4 and prim.shape[0] == 8 and prim.shape[1] > 0
+    cons = np.asarray(simulationGrid.consVars, dtype=np.float64)
+    assert prim.shape == cons.shape
+    dx, dy, dz = simulationGrid.dx, simulationGrid.dy, simulationGrid.dz
+    assert dx > 0
+    assert dy > 0
+    assert dz > 0
+    _check_common(tStart, tEnd, startCycle)
+    faces = (simulationGrid.faceBx, simulationGrid.faceBy, simulationGrid.faceBz)
+    p, c, f, t, cyc, p2, c2, f2, t2 = _run(
+        prim.shape[1:], (dx, dy, dz), (bcx, bcy, bcz), prim, cons, faces, tStart, tEnd, startCycle,
+        maxCycles, gamma, cfl, minDens, minPres, minEnergy, reconstructOrder, entropyFix, epsilon,
+        precision)
+    return p, c, f[0], f[1], f[2], t, cyc, p2, c2, f2[0], f2[1], f2[2], t2

@@ -1,0 +1,50 @@
+// Host (CPU) build of the product's kernel bodies and C ABI -- TEST
+// INFRASTRUCTURE ONLY.  It compiles pythonmhd_amd/csrc/*.h with g++ so the
+// FP64 arithmetic of the HIP kernels can be checked against the oracle in a
+// container without a GPU.  The product package never loads this library.
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+
+#include "pmhd_api.h"
+
+namespace pmhd {
+
+struct HostExec {
+  template <class F>
+  void run(int, int64_t n, F f) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) f(i);
+  }
+  template <class F>
+  void max3(int, int64_t n, F f, double out[3]) {
+    uint64_t m[3] = {0, 0, 0};
+    for (int64_t i = 0; i < n; ++i) {
+      double o[3];
+      f(i, o);
+      for (int d = 0; d < 3; ++d) {
+        uint64_t b;
+        std::memcpy(&b, &o[d], 8);
+        if (b > m[d]) m[d] = b;
+      }
+    }
+    for (int d = 0; d < 3; ++d) std::memcpy(&out[d], &m[d], 8);
+  }
+};
+
+struct HostBackend {
+  HostExec ex;
+  HostExec& exec() { return ex; }
+  void* alloc(size_t bytes) { return std::calloc(bytes, 1); }
+  void release(void* p) { std::free(p); }
+  void h2d(void* d, const void* s, size_t b) { std::memcpy(d, s, b); }
+  void d2h(void* d, const void* s, size_t b) { std::memcpy(d, s, b); }
+  void sync() {}
+  const char* error() { return nullptr; }
+  void set_profiling(bool) {}
+  int profile(int, char*, double*, int64_t*, int64_t*) { return 0; }
+};
+
+}  // namespace pmhd
+
+PMHD_DEFINE_C_ABI(pmhd::HostBackend)

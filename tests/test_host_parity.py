@@ -1,0 +1,18 @@
+"""The parity suite on the CPU through the host build of the same kernel bodies
+(tests/hostbuild/libpmhd_host.so) -- checks the kernels' arithmetic without a GPU."""
+import os
+
+import pytest
+
+from tests.parity_suite import *  # noqa: F401,F403
+
+HOST_LIB = os.path.join(os.path.dirname(__file__), 'hostbuild', 'libpmhd_host.so')
+
+
+@pytest.fixture(scope='module')
+def lib():
+    if not os.path.exists(HOST_LIB):
+        from pythonmhd_amd import build
+        build.build_host()
+    from pythonmhd_amd._lib import load
+    return load(HOST_LIB)
