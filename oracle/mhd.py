@@ -481,6 +481,23 @@ def interface_states(w_dir, bn_cell, gamma, dt, dn, bc, d, axis, order):
 # ---------------------------------------------------------------------------
 def roe_flux(WL, WR, UL, UR, bn, gamma, entropy_fix, eps, d):
     """Directional prim/cons 7-vectors at faces -> directional flux 7-vector."""
+    lam, Lm, R, FL, FR, ul, ur = roe_parts(WL, WR, UL, UR, bn, gamma, entropy_fix, eps, d)
+    dU = ur - ul
+    proj = _project(Lm, dU)
+    amps = [0.5 * np.abs(lam[k]) * proj[k] for k in range(7)]
+    F = 0.5 * (FL + FR)
+    for i in range(7):
+        for k in range(7):
+            F[i] = F[i] - amps[k] * R[i, k]
+    up = lam[0] >= 0.0
+    F = np.where(up, FL, F)
+    dn = lam[6] <= 0.0
+    F = np.where(dn, FR, F)
+    return _unrot(F, d)
+
+
+def roe_parts(WL, WR, UL, UR, bn, gamma, entropy_fix, eps, d):
+    """Roe eigen-system in the rotated frame: (lam[7], L[7,7], R[7,7], F_L, F_R, U_L, U_R)."""
     wl = _rot(WL, d)
     wr = _rot(WR, d)
     ul = _rot(UL, d)
@@ -667,18 +684,7 @@ def roe_flux(WL, WR, UL, UR, bn, gamma, entropy_fix, eps, d):
     Lm[6, 5] = Lm[0, 5]
     FL = _cell_flux(wl, ul, bn, hbl)
     FR = _cell_flux(wr, ur, bn, hbr)
-    dU = ur - ul
-    proj = _project(Lm, dU)
-    amps = [0.5 * np.abs(lam[k]) * proj[k] for k in range(7)]
-    F = 0.5 * (FL + FR)
-    for i in range(7):
-        for k in range(7):
-            F[i] = F[i] - amps[k] * R[i, k]
-    up = lam[0] >= 0.0
-    F = np.where(up, FL, F)
-    dn = lam[6] <= 0.0
-    F = np.where(dn, FR, F)
-    return _unrot(F, d)
+    return lam, Lm, R, FL, FR, ul, ur
 
 
 def _cell_flux(w, u, bn, hb):

@@ -68,7 +68,7 @@ def _declare(lib):
 def load(path=None):
     """Open the step library (default: the in-tree HIP build).  Raises OSError /
     PmhdError when it is absent -- the product has no CPU fallback."""
-    path = os.path.abspath(path or os.environ.get('PMHD_LIBRARY', DEFAULT_LIB))
+    path = os.path.abspath(path or DEFAULT_LIB)
     if path in _cache:
         return _cache[path]
     if not os.path.exists(path):
