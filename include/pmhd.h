@@ -80,6 +80,15 @@ int pmhd_advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int64_
 /* Wait for all queued work of the handle. */
 int pmhd_synchronize(pmhd_handle* h);
 
+/* Diagnostics of the current state, all accumulated in double with a fixed
+ * (deterministic) summation order: totals[8] = sums of the 8 conserved variables
+ * over the cells; max_divb = max_cells |div B| * min(dx, dy, dz) / max |B_face|
+ * (dimensionless; 0 in 1D); max_b = max |B_face|; nonfinite = number of
+ * non-finite conserved values.  Any output pointer may be NULL.  The reference has
+ * no counterpart (its divergence check is a TODO, Source/SimulationGrid_HelperModule.py:406). */
+int pmhd_diagnostics(pmhd_handle* h, double* totals, double* max_divb, double* max_b,
+                     int64_t* nonfinite);
+
 /* Per-launch timing with HIP events on the handle's stream (measurement only).
  * enable = 1 clears the totals and starts recording; 0 stops. */
 int pmhd_set_profiling(pmhd_handle* h, int32_t enable);

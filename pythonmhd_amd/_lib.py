@@ -13,7 +13,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_LIB = os.path.join(_HERE, 'libpmhd.so')
 
 EXPORTS = ('pmhd_create', 'pmhd_destroy', 'pmhd_set_state', 'pmhd_get_state', 'pmhd_compute_dt',
-           'pmhd_step', 'pmhd_advance', 'pmhd_synchronize', 'pmhd_set_profiling', 'pmhd_get_profile',
+           'pmhd_step', 'pmhd_advance', 'pmhd_synchronize', 'pmhd_diagnostics', 'pmhd_set_profiling',
+           'pmhd_get_profile',
            'pmhd_last_error')
 
 
@@ -53,6 +54,8 @@ def _declare(lib):
                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                                  ctypes.POINTER(ctypes.c_double)]
     lib.pmhd_synchronize.argtypes = [H]
+    lib.pmhd_diagnostics.argtypes = [H, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
     lib.pmhd_set_profiling.argtypes = [H, ctypes.c_int32]
     lib.pmhd_get_profile.argtypes = [H, ctypes.c_int32, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
@@ -170,6 +173,17 @@ class Solver:
                                           ctypes.byref(t), ctypes.byref(cyc), ctypes.byref(t2)),
                     'pmhd_advance')
         return t.value, cyc.value, t2.value
+
+    def diagnostics(self):
+        """FP64 conserved totals (8), dimensionless max |div B|, max |B_face|, non-finite count."""
+        tot = (ctypes.c_double * 8)()
+        dvb = ctypes.c_double()
+        bmx = ctypes.c_double()
+        nnf = ctypes.c_int64()
+        self._check(self.lib.pmhd_diagnostics(self.h, tot, ctypes.byref(dvb), ctypes.byref(bmx),
+                                              ctypes.byref(nnf)), 'pmhd_diagnostics')
+        return {'totals': np.array(tot[:]), 'max_divb': dvb.value, 'max_b': bmx.value,
+                'nonfinite': nnf.value}
 
     def set_profiling(self, on):
         self._check(self.lib.pmhd_set_profiling(self.h, 1 if on else 0), 'pmhd_set_profiling')

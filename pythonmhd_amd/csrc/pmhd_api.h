@@ -23,6 +23,7 @@ struct pmhd_handle {
   virtual int compute_dt(double*) = 0;
   virtual int step(double) = 0;
   virtual int synchronize() = 0;
+  virtual int diagnostics(double* totals, double* max_divb, double* max_b, int64_t* nonfinite) = 0;
   virtual int set_profiling(int on) = 0;
   virtual int get_profile(int maxn, char* names, double* ms, int64_t* launches, int64_t* elems,
                           int32_t* n) = 0;
@@ -85,6 +86,7 @@ struct Handle : public pmhd_handle {
     const double w[3] = {g.dx, g.dy, g.dz};
     for (int d = 0; d < 3; ++d) {
       c.width[d] = T(w[d]);
+      c.wd[d] = w[d];
       c.invd[d] = T(1.0 / w[d]);
     }
     try {
@@ -236,6 +238,19 @@ struct Handle : public pmhd_handle {
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
   }
 
+  int diagnostics(double* totals, double* max_divb, double* max_b, int64_t* nonfinite) override {
+    double sums[9], maxes[2];
+    pmhd::diagnostics(be.exec(), c, sums, maxes);
+    const char* e = be.error();
+    if (e) return fail(PMHD_EHIP, e);
+    if (totals)
+      for (int q = 0; q < 8; ++q) totals[q] = sums[q];
+    if (nonfinite) *nonfinite = int64_t(sums[8]);
+    if (max_divb) *max_divb = g.ndim >= 2 && maxes[1] > 0 ? maxes[0] / maxes[1] : 0.0;
+    if (max_b) *max_b = maxes[1];
+    return PMHD_OK;
+  }
+
   int set_profiling(int on) override {
     be.set_profiling(on != 0);
     const char* e = be.error();
@@ -367,6 +382,11 @@ inline int advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int6
   extern "C" int pmhd_synchronize(pmhd_handle* h) {                                         \
     if (!h) return PMHD_EINVAL;                                                             \
     return h->synchronize();                                                                \
+  }                                                                                         \
+  extern "C" int pmhd_diagnostics(pmhd_handle* h, double* totals, double* max_divb,          \
+                                  double* max_b, int64_t* nonfinite) {                      \
+    if (!h) return PMHD_EINVAL;                                                             \
+    return h->diagnostics(totals, max_divb, max_b, nonfinite);                              \
   }                                                                                         \
   extern "C" int pmhd_set_profiling(pmhd_handle* h, int32_t enable) {                       \
     if (!h) return PMHD_EINVAL;                                                             \

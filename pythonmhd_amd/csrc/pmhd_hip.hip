@@ -55,6 +55,45 @@ __global__ void __launch_bounds__(256) k_max3(int64_t n, F f, unsigned long long
   }
 }
 
+// Deterministic two-level reduction of 9 sums and 2 maxima in double: each
+// thread accumulates a fixed grid-stride subset, the block combines in a fixed
+// tree, and the host adds the per-block partials in block order.
+constexpr int kRedThreads = 256;
+constexpr int kRedBlocks = 1024;
+template <class F>
+__global__ void __launch_bounds__(kRedThreads) k_reduce(int64_t n, F f, double* part) {
+  double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double m[2] = {0, 0};
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double a[9], b[2];
+    f(i, a, b);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) s[q] += a[q];
+    m[0] = npmax(m[0], b[0]);
+    m[1] = npmax(m[1], b[1]);
+  }
+  __shared__ double sh[11][kRedThreads];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) sh[q][threadIdx.x] = s[q];
+  sh[9][threadIdx.x] = m[0];
+  sh[10][threadIdx.x] = m[1];
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+      sh[9][threadIdx.x] = npmax(sh[9][threadIdx.x], sh[9][threadIdx.x + w]);
+      sh[10][threadIdx.x] = npmax(sh[10][threadIdx.x], sh[10][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 11; ++q) part[blockIdx.x * 11 + q] = sh[q][0];
+  }
+}
+
 struct HipExec {
   hipStream_t s = nullptr;
   unsigned long long* red = nullptr;
@@ -136,6 +175,27 @@ struct HipExec {
     note(hipGetLastError());
   }
 
+  double* part = nullptr;
+  template <class F>
+  void reduce(int id, int64_t n, F f, double sums[9], double maxes[2]) {
+    if (!part) note(hipMalloc(reinterpret_cast<void**>(&part), sizeof(double) * 11 * kRedBlocks));
+    int64_t nb = (n + kRedThreads - 1) / kRedThreads;
+    if (nb > kRedBlocks) nb = kRedBlocks;
+    if (nb < 1) nb = 1;
+    timed(id, n, [&] { hipLaunchKernelGGL(k_reduce<F>, dim3(unsigned(nb)), dim3(kRedThreads), 0, s, n, f, part); });
+    note(hipGetLastError());
+    std::vector<double> h(size_t(11 * nb));
+    note(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+    note(hipStreamSynchronize(s));
+    for (int q = 0; q < 9; ++q) sums[q] = 0;
+    maxes[0] = maxes[1] = 0;
+    for (int64_t b = 0; b < nb; ++b) {
+      for (int q = 0; q < 9; ++q) sums[q] += h[size_t(b * 11 + q)];
+      maxes[0] = npmax(maxes[0], h[size_t(b * 11 + 9)]);
+      maxes[1] = npmax(maxes[1], h[size_t(b * 11 + 10)]);
+    }
+  }
+
   template <class F>
   void max3(int id, int64_t n, F f, double out[3]) {
     note(hipMemsetAsync(red, 0, 3 * sizeof(unsigned long long), s));
@@ -164,6 +224,7 @@ struct HipBackend {
     ex.destroy_events();
     if (ex.s) hipStreamSynchronize(ex.s);
     if (ex.red) hipFree(ex.red);
+    if (ex.part) hipFree(ex.part);
     if (ex.s) hipStreamDestroy(ex.s);
   }
   HipExec& exec() { return ex; }

@@ -18,14 +18,14 @@ enum { OUTFLOW = 0, PERIODIC = 1 };
 enum KernelId {
   K_SLOPES = 0, K_EDGES = 3, K_COPY = 6, K_PREDICT = 9, K_CELL_E = 12, K_EMF = 13, K_CTU = 14,
   K_CT_HALF = 17, K_HALF_E = 20, K_CORRECT = 21, K_CT_FULL = 24, K_UPDATE = 27, K_CFL = 28,
-  K_COUNT = 29
+  K_DIAG = 29, K_COUNT = 30
 };
 inline const char* kernel_name(int id) {
   static const char* names[K_COUNT] = {
       "slopes_x", "slopes_y", "slopes_z", "edges_x", "edges_y", "edges_z", "copy_x", "copy_y", "copy_z",
       "predict_x", "predict_y", "predict_z", "cell_e", "emf", "ctu_x", "ctu_y", "ctu_z",
       "ct_half_x", "ct_half_y", "ct_half_z", "half_e", "correct_x", "correct_y", "correct_z",
-      "ct_full_x", "ct_full_y", "ct_full_z", "update", "cfl"};
+      "ct_full_x", "ct_full_y", "ct_full_z", "update", "cfl", "diag"};
   return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 
@@ -44,6 +44,7 @@ struct Ctx {
   T hdtd[3];         // 0.5*(dt/dx), ...
   T invd[3];         // 1/dx, ...
   T width[3];        // dx, dy, dz
+  double wd[3];      // dx, dy, dz in double (diagnostics)
   T emfq[3];         // 0.5*(0.5*(dt/dx)), ...
   T bx1d;            // uniform B_x of the 1D step
   double ldh[2];     // 0.5*dt/dy as the reference's 80-bit value, hi + lo (FP64 only)
@@ -810,6 +811,40 @@ PMHD_FI void k_update(const Ctx<T>& c, int64_t idx) {
   c.W2[5 * N + idx] = w[4];
   c.W2[6 * N + idx] = w[5];
   c.W2[7 * N + idx] = w[6];
+}
+
+// ===========================================================================
+// diagnostics of one cell, in double: the 8 conserved values (summed over the
+// grid), |div B| of the face field times the smallest width, |B| of the faces
+// (both maximised) and the number of non-finite conserved values (summed)
+// ===========================================================================
+template <class T>
+PMHD_FI void k_diag(const Ctx<T>& c, int64_t idx, double sums[9], double maxes[2]) {
+  const int64_t N = c.ncell();
+  I3 p = unflat(idx, c.ny, c.nx, c.nz);
+  double bad = 0.0;
+  for (int v = 0; v < 8; ++v) {
+    double u = double(c.U[v * N + idx]);
+    sums[v] = u;
+    if (!(u == u) || u - u != 0.0) bad += 1.0;
+  }
+  sums[8] = bad;
+  double div = 0.0, bmax = 0.0, wmin = c.wd[0];
+  if (c.ndim >= 2) {
+    double fx0 = c.fb[0][c.fid(0, p.j, p.i, p.k)], fx1 = c.fb[0][c.fid(0, p.j, p.i + 1, p.k)];
+    double fy0 = c.fb[1][c.fid(1, p.j, p.i, p.k)], fy1 = c.fb[1][c.fid(1, p.j + 1, p.i, p.k)];
+    div = (fx1 - fx0) / c.wd[0] + (fy1 - fy0) / c.wd[1];
+    bmax = npmax(npmax(fabs(fx0), fabs(fx1)), npmax(fabs(fy0), fabs(fy1)));
+    wmin = npmin(wmin, c.wd[1]);
+    if (c.ndim == 3) {
+      double fz0 = c.fb[2][c.fid(2, p.j, p.i, p.k)], fz1 = c.fb[2][c.fid(2, p.j, p.i, p.k + 1)];
+      div += (fz1 - fz0) / c.wd[2];
+      bmax = npmax(bmax, npmax(fabs(fz0), fabs(fz1)));
+      wmin = npmin(wmin, c.wd[2]);
+    }
+  }
+  maxes[0] = fabs(div) * wmin;
+  maxes[1] = bmax;
 }
 
 }  // namespace pmhd

@@ -18,6 +18,14 @@ void cfl_maxima(X& ex, const Ctx<T>& c, double out[3]) {
   }, out);
 }
 
+// FP64 diagnostics: sums[0..7] conserved totals, sums[8] non-finite count,
+// maxes[0] max |div B| * min width, maxes[1] max |B| on faces
+template <class T, class X>
+void diagnostics(X& ex, const Ctx<T>& c, double sums[9], double maxes[2]) {
+  ex.reduce(K_DIAG, c.ncell(), [=] PMHD_HD(int64_t i, double* s, double* m) { k_diag(c, i, s, m); },
+            sums, maxes);
+}
+
 template <class T, class X>
 void cycle(X& ex, const Ctx<T>& c) {
   const int nd = c.ndim;

@@ -17,6 +17,18 @@ struct HostExec {
     for (int64_t i = 0; i < n; ++i) f(i);
   }
   template <class F>
+  void reduce(int, int64_t n, F f, double sums[9], double maxes[2]) {
+    for (int q = 0; q < 9; ++q) sums[q] = 0;
+    maxes[0] = maxes[1] = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      double a[9], b[2];
+      f(i, a, b);
+      for (int q = 0; q < 9; ++q) sums[q] += a[q];
+      maxes[0] = npmax(maxes[0], b[0]);
+      maxes[1] = npmax(maxes[1], b[1]);
+    }
+  }
+  template <class F>
   void max3(int, int64_t n, F f, double out[3]) {
     uint64_t m[3] = {0, 0, 0};
     for (int64_t i = 0; i < n; ++i) {

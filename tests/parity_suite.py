@@ -128,3 +128,26 @@ def test_conservation_and_divb_fp64(lib):
         tot1 = np.sum(got[1][v])
         assert abs(tot1 - tot0) <= 1e-12 * max(1.0, np.sum(np.abs(cons[v]))), (v, tot0, tot1)
     assert div_b(got[2], widths) < 1e-13 + d0
+
+
+@pytest.mark.parametrize('precision', [64, 32])
+def test_diagnostics(lib, precision):
+    """pmhd_diagnostics: FP64 conserved totals and the dimensionless max |div B|
+    against NumPy on the downloaded state."""
+    from pythonmhd_amd._lib import Solver as S
+    rng = np.random.default_rng(11)
+    shape = (12, 14, 10)
+    prim, faces = smooth_state(rng, shape)
+    cons = M.prim_to_cons_all(prim, GAMMA)
+    w = widths_of(shape)
+    s = S(shape, w, (1, 1, 1), GAMMA, 0.4, order=1, precision=precision, lib=lib)
+    s.set_state(prim, cons, faces)
+    s.advance(0.0, 10.0, 0, 3)
+    d = s.diagnostics()
+    p, c, f = s.get_state()
+    s.close()
+    tot = np.array([np.sum(c[v]) for v in range(8)])
+    assert np.allclose(d['totals'], tot, rtol=1e-13, atol=1e-12)
+    assert d['nonfinite'] == 0
+    assert abs(d['max_divb'] - div_b(f, w)) <= 1e-12 * max(1.0, div_b(f, w))
+    assert d['max_b'] == max(np.max(np.abs(a)) for a in f)
