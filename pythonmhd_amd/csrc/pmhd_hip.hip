@@ -11,10 +11,14 @@
 
 namespace pmhd {
 
+// one thread per element of an (n0, n1, n2) box: x covers the contiguous axis
 template <class F>
-__global__ void __launch_bounds__(256) k_each(int64_t n, F f) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) f(i);
+__global__ void __launch_bounds__(256) k_each3(int n0, int n1, int n2, F f) {
+  const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+  const int i = int(blockIdx.y * blockDim.y + threadIdx.y);
+  const int j = int(blockIdx.z);
+  if (k < n2 && i < n1) f(I3{j, i, k});
+  (void)n0;
 }
 
 __device__ inline unsigned long long umax64(unsigned long long a, unsigned long long b) {
@@ -167,11 +171,15 @@ struct HipExec {
   }
 
   template <class F>
-  void run(int id, int64_t n, F f) {
-    if (n <= 0) return;
-    const int tb = 256;
-    const int64_t nb = (n + tb - 1) / tb;
-    timed(id, n, [&] { hipLaunchKernelGGL(k_each<F>, dim3(unsigned(nb)), dim3(tb), 0, s, n, f); });
+  void run3(int id, int64_t n0, int64_t n1, int64_t n2, F f) {
+    if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
+    unsigned bx = 1;
+    while (bx < 64 && int64_t(bx) < n2) bx <<= 1;
+    const unsigned by = 256 / bx;
+    dim3 grid(unsigned((n2 + bx - 1) / bx), unsigned((n1 + by - 1) / by), unsigned(n0));
+    timed(id, n0 * n1 * n2, [&] {
+      hipLaunchKernelGGL(k_each3<F>, grid, dim3(bx, by, 1), 0, s, int(n0), int(n1), int(n2), f);
+    });
     note(hipGetLastError());
   }
 

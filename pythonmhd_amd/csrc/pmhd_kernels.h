@@ -191,9 +191,9 @@ PMHD_FI void k_cfl(const Ctx<T>& c, int64_t idx, T out[3]) {
 // reconstruction: characteristic-limited slopes on the cells -1..n along d
 // ===========================================================================
 template <class T>
-PMHD_FI void k_slopes(const Ctx<T>& c, int d, int bc, int64_t idx) {
+PMHD_FI void k_slopes(const Ctx<T>& c, int d, int bc, I3 q) {
   const int64_t N = c.ncell();
-  I3 q = unflat(idx, c.sny(d), c.snx(d), c.snz(d));
+  const int64_t idx = (q.j * c.snx(d) + q.i) * c.snz(d) + q.k;
   const int64_t n = c.n_along(d);
   const int64_t e = co(q, d) - 1;
   I3 pc = with(q, d, map2(e, n, bc));
@@ -213,9 +213,9 @@ PMHD_FI void k_slopes(const Ctx<T>& c, int d, int bc, int64_t idx) {
 
 // edges and characteristic tracing of each cell; writes the two faces it owns
 template <class T>
-PMHD_FI void k_edges(const Ctx<T>& c, int d, int bc, int64_t idx) {
+PMHD_FI void k_edges(const Ctx<T>& c, int d, int bc, I3 p) {
   const int64_t N = c.ncell();
-  I3 p = unflat(idx, c.ny, c.nx, c.nz);
+  const int64_t idx = c.cid(p.j, p.i, p.k);
   const int64_t n = c.n_along(d);
   const int64_t e = co(p, d);
   I3 pl = with(p, d, map2(e - 1, n, bc));
@@ -266,9 +266,9 @@ PMHD_FI void k_edges(const Ctx<T>& c, int d, int bc, int64_t idx) {
 
 // first-order interface states (reconstructOrder 0)
 template <class T>
-PMHD_FI void k_copy_states(const Ctx<T>& c, int d, int bc, int64_t idx) {
+PMHD_FI void k_copy_states(const Ctx<T>& c, int d, int bc, I3 f) {
   const int64_t N = c.ncell();
-  I3 f = unflat(idx, c.fny(d), c.fnx(d), c.fnz(d));
+  const int64_t idx = c.fid(d, f.j, f.i, f.k);
   const int64_t n = c.n_along(d);
   const int64_t e = co(f, d);
   I3 pl = with(f, d, map1(e - 1, n, bc));
@@ -319,10 +319,10 @@ PMHD_FI void recon_b_term_3d(const Ctx<T>& c, int d, int bc, I3 p, int64_t e, T 
 }
 
 template <class T>
-PMHD_FI void k_predict(const Ctx<T>& c, int d, int64_t idx) {
+PMHD_FI void k_predict(const Ctx<T>& c, int d, I3 f) {
   const int64_t N = c.ncell();
   const int64_t NF = c.nface(d);
-  I3 f = unflat(idx, c.fny(d), c.fnx(d), c.fnz(d));
+  const int64_t idx = c.fid(d, f.j, f.i, f.k);
   const int64_t e = co(f, d);
   T wl[7], wr[7];
   load7(c.WL, NF, idx, wl);
@@ -384,8 +384,9 @@ PMHD_FI void k_predict(const Ctx<T>& c, int d, int64_t idx) {
 // cell-centred electric field
 // ===========================================================================
 template <class T>
-PMHD_FI void k_cell_e(const Ctx<T>& c, int64_t idx) {
+PMHD_FI void k_cell_e(const Ctx<T>& c, I3 p) {
   const int64_t N = c.ncell();
+  const int64_t idx = c.cid(p.j, p.i, p.k);
   if (c.ndim == 3) {
     T rho = c.U[idx], mx = c.U[N + idx], my = c.U[2 * N + idx], mz = c.U[3 * N + idx];
     T bx = c.W[4 * N + idx], by = c.W[5 * N + idx], bz = c.W[6 * N + idx];
@@ -399,9 +400,9 @@ PMHD_FI void k_cell_e(const Ctx<T>& c, int64_t idx) {
 
 // half-step cell electric field from the predictor fluxes and half-step faces
 template <class T>
-PMHD_FI void k_half_e(const Ctx<T>& c, int64_t idx) {
+PMHD_FI void k_half_e(const Ctx<T>& c, I3 p) {
   const int64_t N = c.ncell();
-  I3 p = unflat(idx, c.ny, c.nx, c.nz);
+  const int64_t idx = c.cid(p.j, p.i, p.k);
   const int64_t NX = c.nface(0), NY = c.nface(1);
   int64_t fx0 = c.fid(0, p.j, p.i, p.k), fx1 = c.fid(0, p.j, p.i + 1, p.k);
   int64_t fy0 = c.fid(1, p.j, p.i, p.k), fy1 = c.fid(1, p.j + 1, p.i, p.k);
@@ -445,9 +446,9 @@ PMHD_FI T fluxv(const Ctx<T>& c, int d, int v, int64_t j, int64_t i, int64_t k) 
 }
 
 template <class T>
-PMHD_FI void k_emf(const Ctx<T>& c, int64_t idx) {
+PMHD_FI void k_emf(const Ctx<T>& c, I3 p) {
   const int64_t N = c.ncell();
-  I3 p = unflat(idx, c.eny(), c.enx(), c.enz());
+  const int64_t idx = c.eid(p.j, p.i, p.k);
   const int bx = c.bc[0], by = c.bc[1], bz = c.bc[2];
   const int64_t xl = edge_lo(p.i, c.nx, bx), xr = edge_hi(p.i, c.nx, bx);
   const int64_t yl = edge_lo(p.j, c.ny, by), yr = edge_hi(p.j, c.ny, by);
@@ -593,10 +594,10 @@ PMHD_FI void ctu_apply_3d(const Ctx<T>& c, int d, int tc, I3 pcell, int64_t hi_o
 }
 
 template <class T>
-PMHD_FI void k_ctu(const Ctx<T>& c, int d, int64_t idx) {
+PMHD_FI void k_ctu(const Ctx<T>& c, int d, I3 f) {
   const int64_t N = c.ncell();
   const int64_t NF = c.nface(d);
-  I3 f = unflat(idx, c.fny(d), c.fnx(d), c.fnz(d));
+  const int64_t idx = c.fid(d, f.j, f.i, f.k);
   const int64_t n = c.n_along(d);
   const int bc = c.bc[d];
   const int64_t e = co(f, d);
@@ -692,8 +693,8 @@ PMHD_FI void k_ctu(const Ctx<T>& c, int d, int64_t idx) {
 // :1082-1104 for 3D; :579-586, :643-649 for 2D)
 // ===========================================================================
 template <class T>
-PMHD_FI void k_ct(const Ctx<T>& c, int d, const T* fin, T* fout, const T h[3], int64_t idx) {
-  I3 f = unflat(idx, c.fny(d), c.fnx(d), c.fnz(d));
+PMHD_FI void k_ct(const Ctx<T>& c, int d, const T* fin, T* fout, const T h[3], I3 f) {
+  const int64_t idx = c.fid(d, f.j, f.i, f.k);
   if (c.ndim == 2) {
     const T* E = c.E[2];
     if (d == 0) {
@@ -724,8 +725,9 @@ PMHD_FI void k_ct(const Ctx<T>& c, int d, const T* fin, T* fout, const T h[3], i
 // corrector: cons -> prim with the half-step normal field, Roe flux
 // ===========================================================================
 template <class T>
-PMHD_FI void k_correct(const Ctx<T>& c, int d, int64_t idx) {
+PMHD_FI void k_correct(const Ctx<T>& c, int d, I3 f) {
   const int64_t NF = c.nface(d);
+  const int64_t idx = c.fid(d, f.j, f.i, f.k);
   T ul[7], ur[7], wl[7], wr[7];
   load7(c.UL[d], NF, idx, ul);
   load7(c.UR[d], NF, idx, ur);
@@ -747,9 +749,9 @@ PMHD_FI void k_correct(const Ctx<T>& c, int d, int64_t idx) {
 // conservative update of the cells and the new primitive state
 // ===========================================================================
 template <class T>
-PMHD_FI void k_update(const Ctx<T>& c, int64_t idx) {
+PMHD_FI void k_update(const Ctx<T>& c, I3 p) {
   const int64_t N = c.ncell();
-  I3 p = unflat(idx, c.ny, c.nx, c.nz);
+  const int64_t idx = c.cid(p.j, p.i, p.k);
   // X-directional conserved vector (rho, Mx, My, Mz, By, Bz, E)
   const int dv[7] = {0, 1, 2, 3, 5, 6, 7};
   T u[7];

@@ -1,6 +1,9 @@
 // One CTU+CT cycle as a sequence of launches over an executor X that provides
-//   X::run(id, n, f)        -- f(i) for i in [0, n); id names the launch
-//   X::max3(id, n, f, out)  -- out[d] = max over i of f(i)[d]
+//   X::run3(id, n0, n1, n2, f)    -- f(I3{j, i, k}) over a (n0, n1, n2) index box
+//   X::max3(id, n, f, out)        -- out[d] = max over i in [0, n) of f(i)[d]
+//   X::reduce(id, n, f, sums, maxes)
+// The direction of each sweep is a template parameter so the rotations and index
+// arithmetic of the kernel bodies fold at compile time.
 // (Source/EvolveFunctions_MHD.py:252-317 for 1D, :464-692 for 2D, :840-1152 for 3D)
 #pragma once
 #include "pmhd_kernels.h"
@@ -26,44 +29,67 @@ void diagnostics(X& ex, const Ctx<T>& c, double sums[9], double maxes[2]) {
             sums, maxes);
 }
 
+// interface states and predictor fluxes of direction D
+template <int D, class T, class X>
+void predictor_sweep(X& ex, const Ctx<T>& c) {
+  const int bc = c.bc[D];
+  if (c.order > 0) {
+    ex.run3(K_SLOPES + D, c.sny(D), c.snx(D), c.snz(D), [=] PMHD_HD(I3 p) { k_slopes(c, D, bc, p); });
+    ex.run3(K_EDGES + D, c.ny, c.nx, c.nz, [=] PMHD_HD(I3 p) { k_edges(c, D, bc, p); });
+  } else {
+    // the reference's 3D first-order z states use the x boundary condition
+    const int bcq = D == 2 ? c.bc_zrec : bc;
+    ex.run3(K_COPY + D, c.fny(D), c.fnx(D), c.fnz(D), [=] PMHD_HD(I3 p) { k_copy_states(c, D, bcq, p); });
+  }
+  ex.run3(K_PREDICT + D, c.fny(D), c.fnx(D), c.fnz(D), [=] PMHD_HD(I3 p) { k_predict(c, D, p); });
+}
+
+template <int D, class T, class X>
+void face_pass(X& ex, const Ctx<T>& c, int which) {
+  const int64_t n0 = c.fny(D), n1 = c.fnx(D), n2 = c.fnz(D);
+  if (which == 0) {
+    ex.run3(K_CTU + D, n0, n1, n2, [=] PMHD_HD(I3 p) { k_ctu(c, D, p); });
+  } else if (which == 1) {
+    ex.run3(K_CT_HALF + D, n0, n1, n2, [=] PMHD_HD(I3 p) {
+      T h[3] = {c.hdtd[0], c.hdtd[1], c.hdtd[2]};
+      k_ct(c, D, c.fb[D], c.nfb[D], h, p);
+    });
+  } else if (which == 2) {
+    ex.run3(K_CORRECT + D, n0, n1, n2, [=] PMHD_HD(I3 p) { k_correct(c, D, p); });
+  } else {
+    ex.run3(K_CT_FULL + D, n0, n1, n2, [=] PMHD_HD(I3 p) {
+      T h[3] = {c.dtd[0], c.dtd[1], c.dtd[2]};
+      k_ct(c, D, c.fb[D], c.fb2[D], h, p);
+    });
+  }
+}
+
+template <class T, class X>
+void face_passes(X& ex, const Ctx<T>& c, int which) {
+  face_pass<0>(ex, c, which);
+  face_pass<1>(ex, c, which);
+  if (c.ndim == 3) face_pass<2>(ex, c, which);
+}
+
 template <class T, class X>
 void cycle(X& ex, const Ctx<T>& c) {
   const int nd = c.ndim;
-  for (int d = 0; d < nd; ++d) {
-    const int bc = c.bc[d];
-    if (c.order > 0) {
-      ex.run(K_SLOPES + d, c.nslope(d), [=] PMHD_HD(int64_t i) { k_slopes(c, d, bc, i); });
-      ex.run(K_EDGES + d, c.ncell(), [=] PMHD_HD(int64_t i) { k_edges(c, d, bc, i); });
-    } else {
-      // the reference's 3D first-order z states use the x boundary condition
-      const int bcq = d == 2 ? c.bc_zrec : bc;
-      ex.run(K_COPY + d, c.nface(d), [=] PMHD_HD(int64_t i) { k_copy_states(c, d, bcq, i); });
-    }
-    ex.run(K_PREDICT + d, c.nface(d), [=] PMHD_HD(int64_t i) { k_predict(c, d, i); });
-  }
+  predictor_sweep<0>(ex, c);
+  if (nd >= 2) predictor_sweep<1>(ex, c);
+  if (nd == 3) predictor_sweep<2>(ex, c);
   if (nd == 1) {
-    ex.run(K_UPDATE, c.ncell(), [=] PMHD_HD(int64_t i) { k_update(c, i); });
+    ex.run3(K_UPDATE, c.ny, c.nx, c.nz, [=] PMHD_HD(I3 p) { k_update(c, p); });
     return;
   }
-  ex.run(K_CELL_E, c.ncell(), [=] PMHD_HD(int64_t i) { k_cell_e(c, i); });
-  ex.run(K_EMF, c.nedge(), [=] PMHD_HD(int64_t i) { k_emf(c, i); });
-  for (int d = 0; d < nd; ++d) ex.run(K_CTU + d, c.nface(d), [=] PMHD_HD(int64_t i) { k_ctu(c, d, i); });
-  for (int d = 0; d < nd; ++d) {
-    ex.run(K_CT_HALF + d, c.nface(d), [=] PMHD_HD(int64_t i) {
-      T h[3] = {c.hdtd[0], c.hdtd[1], c.hdtd[2]};
-      k_ct(c, d, c.fb[d], c.nfb[d], h, i);
-    });
-  }
-  ex.run(K_HALF_E, c.ncell(), [=] PMHD_HD(int64_t i) { k_half_e(c, i); });
-  for (int d = 0; d < nd; ++d) ex.run(K_CORRECT + d, c.nface(d), [=] PMHD_HD(int64_t i) { k_correct(c, d, i); });
-  ex.run(K_EMF, c.nedge(), [=] PMHD_HD(int64_t i) { k_emf(c, i); });
-  for (int d = 0; d < nd; ++d) {
-    ex.run(K_CT_FULL + d, c.nface(d), [=] PMHD_HD(int64_t i) {
-      T h[3] = {c.dtd[0], c.dtd[1], c.dtd[2]};
-      k_ct(c, d, c.fb[d], c.fb2[d], h, i);
-    });
-  }
-  ex.run(K_UPDATE, c.ncell(), [=] PMHD_HD(int64_t i) { k_update(c, i); });
+  ex.run3(K_CELL_E, c.ny, c.nx, c.nz, [=] PMHD_HD(I3 p) { k_cell_e(c, p); });
+  ex.run3(K_EMF, c.eny(), c.enx(), c.enz(), [=] PMHD_HD(I3 p) { k_emf(c, p); });
+  face_passes(ex, c, 0);   // CTU transverse corrections
+  face_passes(ex, c, 1);   // half-step face field
+  ex.run3(K_HALF_E, c.ny, c.nx, c.nz, [=] PMHD_HD(I3 p) { k_half_e(c, p); });
+  face_passes(ex, c, 2);   // corrector fluxes
+  ex.run3(K_EMF, c.eny(), c.enx(), c.enz(), [=] PMHD_HD(I3 p) { k_emf(c, p); });
+  face_passes(ex, c, 3);   // full-step face field
+  ex.run3(K_UPDATE, c.ny, c.nx, c.nz, [=] PMHD_HD(I3 p) { k_update(c, p); });
 }
 
 }  // namespace pmhd

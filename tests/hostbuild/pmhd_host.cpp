@@ -12,9 +12,11 @@ namespace pmhd {
 
 struct HostExec {
   template <class F>
-  void run(int, int64_t n, F f) {
-#pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < n; ++i) f(i);
+  void run3(int, int64_t n0, int64_t n1, int64_t n2, F f) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t j = 0; j < n0; ++j)
+      for (int64_t i = 0; i < n1; ++i)
+        for (int64_t k = 0; k < n2; ++k) f(I3{j, i, k});
   }
   template <class F>
   void reduce(int, int64_t n, F f, double sums[9], double maxes[2]) {
