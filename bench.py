@@ -34,7 +34,8 @@ ALG_WORDS = {
     'predict': 20 + 21,     # two states + face/cell B terms in; U_L, U_R, flux out (per face)
     'cell_e': 7 + 3,
     'emf': 12 + 3,          # three face-flux components per face array + cell E in (per edge point)
-    'ctu': 34 + 14,         # states, transverse fluxes, EMF, faces, cell B/v in; states out
+    'ctu_correct': 34 + 1 + 7,  # states, transverse fluxes, EMF, faces, cell B/v, half-step B in; flux out
+    'emf_corr': 12 + 3,
     'ct_half': 3 + 1,
     'ct_full': 3 + 1,
     'half_e': 19 + 3,
@@ -146,7 +147,7 @@ def roofline(prof, word_bytes):
     best = max(prof.items(), key=lambda kv: kv[1][0])
     name, (ms, nl, ne) = best
     base = name.rsplit('_', 1)[0] if name.endswith(('_x', '_y', '_z')) else name
-    words = ALG_WORDS.get(base, ALG_WORDS.get(name))
+    words = ALG_WORDS.get(base, ALG_WORDS.get(name, 0))
     per_launch_s = ms / nl / 1e3
     bytes_per_launch = words * word_bytes * ne / nl
     ach = bytes_per_launch / per_launch_s / 1e9

@@ -109,6 +109,7 @@ struct Handle : public pmhd_handle {
         c.UL[d] = dalloc(used ? 7 * c.nface(d) : 1);
         c.UR[d] = dalloc(used ? 7 * c.nface(d) : 1);
         c.F[d] = dalloc(used ? 7 * c.nface(d) : 1);
+        c.Fc[d] = dalloc(used && g.ndim >= 2 ? 7 * c.nface(d) : 1);
         c.nfb[d] = dalloc(used && g.ndim >= 2 ? c.nface(d) : 1);
         c.E[d] = dalloc(g.ndim == 3 || (g.ndim == 2 && d == 2) ? c.nedge() : 1);
       }

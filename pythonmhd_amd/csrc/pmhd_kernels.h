@@ -16,15 +16,15 @@ enum { OUTFLOW = 0, PERIODIC = 1 };
 
 // launch identifiers (for per-kernel timing); "+ d" for the per-direction ones
 enum KernelId {
-  K_SLOPES = 0, K_EDGES = 3, K_COPY = 6, K_PREDICT = 9, K_CELL_E = 12, K_EMF = 13, K_CTU = 14,
+  K_SLOPES = 0, K_EDGES = 3, K_COPY = 6, K_PREDICT = 9, K_CELL_E = 12, K_EMF = 13, K_EMF2 = 14,
   K_CT_HALF = 17, K_HALF_E = 20, K_CORRECT = 21, K_CT_FULL = 24, K_UPDATE = 27, K_CFL = 28,
   K_DIAG = 29, K_COUNT = 30
 };
 inline const char* kernel_name(int id) {
   static const char* names[K_COUNT] = {
       "slopes_x", "slopes_y", "slopes_z", "edges_x", "edges_y", "edges_z", "copy_x", "copy_y", "copy_z",
-      "predict_x", "predict_y", "predict_z", "cell_e", "emf", "ctu_x", "ctu_y", "ctu_z",
-      "ct_half_x", "ct_half_y", "ct_half_z", "half_e", "correct_x", "correct_y", "correct_z",
+      "predict_x", "predict_y", "predict_z", "cell_e", "emf", "emf_corr", "unused_15", "unused_16",
+      "ct_half_x", "ct_half_y", "ct_half_z", "half_e", "ctu_correct_x", "ctu_correct_y", "ctu_correct_z",
       "ct_full_x", "ct_full_y", "ct_full_z", "update", "cfl", "diag"};
   return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
@@ -62,7 +62,9 @@ struct Ctx {
   T* S;
   T* UL[3];
   T* UR[3];
-  T* F[3];
+  T* F[3];           // predictor fluxes
+  T* Fc[3];          // corrector fluxes
+  const T* Fe[3];    // fluxes the corner-EMF launch reads (predictor or corrector)
   T* Ec;
   T* E[3];
   // element counts
@@ -442,7 +444,7 @@ PMHD_FI void k_half_e(const Ctx<T>& c, I3 p) {
 // ===========================================================================
 template <class T>
 PMHD_FI T fluxv(const Ctx<T>& c, int d, int v, int64_t j, int64_t i, int64_t k) {
-  return c.F[d][v * c.nface(d) + c.fid(d, j, i, k)];
+  return c.Fe[d][v * c.nface(d) + c.fid(d, j, i, k)];
 }
 
 template <class T>
@@ -594,14 +596,13 @@ PMHD_FI void ctu_apply_3d(const Ctx<T>& c, int d, int tc, I3 pcell, int64_t hi_o
 }
 
 template <class T>
-PMHD_FI void k_ctu(const Ctx<T>& c, int d, I3 f) {
+PMHD_FI void ctu_states(const Ctx<T>& c, int d, I3 f, T ul[7], T ur[7]) {
   const int64_t N = c.ncell();
   const int64_t NF = c.nface(d);
   const int64_t idx = c.fid(d, f.j, f.i, f.k);
   const int64_t n = c.n_along(d);
   const int bc = c.bc[d];
   const int64_t e = co(f, d);
-  T ul[7], ur[7];
   load7(c.UL[d], NF, idx, ul);
   load7(c.UR[d], NF, idx, ur);
   I3 pl = with(f, d, map1(e - 1, n, bc));
@@ -684,8 +685,6 @@ PMHD_FI void k_ctu(const Ctx<T>& c, int d, I3 f) {
       apply_floors(u, c.min_dens, c.min_energy);
     }
   }
-  store7(c.UL[d], NF, idx, ul);
-  store7(c.UR[d], NF, idx, ur);
 }
 
 // ===========================================================================
@@ -724,14 +723,14 @@ PMHD_FI void k_ct(const Ctx<T>& c, int d, const T* fin, T* fout, const T h[3], I
 // ===========================================================================
 // corrector: cons -> prim with the half-step normal field, Roe flux
 // ===========================================================================
+// CTU-corrected states -> primitive with the half-step normal field -> Roe flux
 template <class T>
 PMHD_FI void k_correct(const Ctx<T>& c, int d, I3 f) {
   const int64_t NF = c.nface(d);
   const int64_t idx = c.fid(d, f.j, f.i, f.k);
   T ul[7], ur[7], wl[7], wr[7];
-  load7(c.UL[d], NF, idx, ul);
-  load7(c.UR[d], NF, idx, ur);
-  T bn = c.ndim == 1 ? c.bx1d : c.nfb[d][idx];
+  ctu_states(c, d, f, ul, ur);
+  T bn = c.nfb[d][idx];
   int ke = c.ndim == 3 ? d : 0;
   cons_to_prim(ul, bn, c.gamma, ke, wl);
   cons_to_prim(ur, bn, c.gamma, ke, wr);
@@ -742,7 +741,7 @@ PMHD_FI void k_correct(const Ctx<T>& c, int d, I3 f) {
   to_rot(ur, d, rur);
   roe_flux(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf);
   from_rot(rf, d, df);
-  store7(c.F[d], NF, idx, df);
+  store7(c.Fc[d], NF, idx, df);
 }
 
 // ===========================================================================
@@ -768,12 +767,12 @@ PMHD_FI void k_update(const Ctx<T>& c, I3 p) {
   } else if (c.ndim == 2) {
     const int64_t NY = c.nface(1);
     int64_t fy0 = c.fid(1, p.j, p.i, p.k), fy1 = c.fid(1, p.j + 1, p.i, p.k);
-    T nbz = u[5] - c.dtd[0] * (c.F[0][5 * NX + fx1] - c.F[0][5 * NX + fx0])
-            - c.dtd[1] * (c.F[1][4 * NY + fy1] - c.F[1][4 * NY + fy0]);
+    T nbz = u[5] - c.dtd[0] * (c.Fc[0][5 * NX + fx1] - c.Fc[0][5 * NX + fx0])
+            - c.dtd[1] * (c.Fc[1][4 * NY + fy1] - c.Fc[1][4 * NY + fy0]);
 #pragma unroll
     for (int v = 0; v < 7; ++v) {
-      u[v] = u[v] - c.dtd[0] * (c.F[0][v * NX + fx1] - c.F[0][v * NX + fx0]);
-      u[v] = u[v] - c.dtd[1] * (c.F[1][v * NY + fy1] - c.F[1][v * NY + fy0]);
+      u[v] = u[v] - c.dtd[0] * (c.Fc[0][v * NX + fx1] - c.Fc[0][v * NX + fx0]);
+      u[v] = u[v] - c.dtd[1] * (c.Fc[1][v * NY + fy1] - c.Fc[1][v * NY + fy0]);
     }
     bx = (c.fb2[0][fx0] + c.fb2[0][fx1]) / T(2);
     T by = (c.fb2[1][fy0] + c.fb2[1][fy1]) / T(2);
@@ -786,9 +785,9 @@ PMHD_FI void k_update(const Ctx<T>& c, I3 p) {
     int64_t fz0 = c.fid(2, p.j, p.i, p.k), fz1 = c.fid(2, p.j, p.i, p.k + 1);
 #pragma unroll
     for (int v = 0; v < 7; ++v) {
-      u[v] = u[v] - c.dtd[0] * (c.F[0][v * NX + fx1] - c.F[0][v * NX + fx0]);
-      u[v] = u[v] - c.dtd[1] * (c.F[1][v * NY + fy1] - c.F[1][v * NY + fy0]);
-      u[v] = u[v] - c.dtd[2] * (c.F[2][v * NZ + fz1] - c.F[2][v * NZ + fz0]);
+      u[v] = u[v] - c.dtd[0] * (c.Fc[0][v * NX + fx1] - c.Fc[0][v * NX + fx0]);
+      u[v] = u[v] - c.dtd[1] * (c.Fc[1][v * NY + fy1] - c.Fc[1][v * NY + fy0]);
+      u[v] = u[v] - c.dtd[2] * (c.Fc[2][v * NZ + fz1] - c.Fc[2][v * NZ + fz0]);
     }
     bx = T(0.5) * (c.fb2[0][fx0] + c.fb2[0][fx1]);
     T by = T(0.5) * (c.fb2[1][fy0] + c.fb2[1][fy1]);

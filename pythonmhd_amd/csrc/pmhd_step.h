@@ -47,9 +47,7 @@ void predictor_sweep(X& ex, const Ctx<T>& c) {
 template <int D, class T, class X>
 void face_pass(X& ex, const Ctx<T>& c, int which) {
   const int64_t n0 = c.fny(D), n1 = c.fnx(D), n2 = c.fnz(D);
-  if (which == 0) {
-    ex.run3(K_CTU + D, n0, n1, n2, [=] PMHD_HD(I3 p) { k_ctu(c, D, p); });
-  } else if (which == 1) {
+  if (which == 1) {
     ex.run3(K_CT_HALF + D, n0, n1, n2, [=] PMHD_HD(I3 p) {
       T h[3] = {c.hdtd[0], c.hdtd[1], c.hdtd[2]};
       k_ct(c, D, c.fb[D], c.nfb[D], h, p);
@@ -82,12 +80,15 @@ void cycle(X& ex, const Ctx<T>& c) {
     return;
   }
   ex.run3(K_CELL_E, c.ny, c.nx, c.nz, [=] PMHD_HD(I3 p) { k_cell_e(c, p); });
-  ex.run3(K_EMF, c.eny(), c.enx(), c.enz(), [=] PMHD_HD(I3 p) { k_emf(c, p); });
-  face_passes(ex, c, 0);   // CTU transverse corrections
+  Ctx<T> cp = c;           // corner EMFs of the predictor fluxes
+  for (int d = 0; d < 3; ++d) cp.Fe[d] = c.F[d];
+  ex.run3(K_EMF, c.eny(), c.enx(), c.enz(), [=] PMHD_HD(I3 p) { k_emf(cp, p); });
   face_passes(ex, c, 1);   // half-step face field
   ex.run3(K_HALF_E, c.ny, c.nx, c.nz, [=] PMHD_HD(I3 p) { k_half_e(c, p); });
-  face_passes(ex, c, 2);   // corrector fluxes
-  ex.run3(K_EMF, c.eny(), c.enx(), c.enz(), [=] PMHD_HD(I3 p) { k_emf(c, p); });
+  face_passes(ex, c, 2);   // CTU transverse corrections + corrector fluxes
+  Ctx<T> cc = c;           // corner EMFs of the corrector fluxes
+  for (int d = 0; d < 3; ++d) cc.Fe[d] = c.Fc[d];
+  ex.run3(K_EMF2, c.eny(), c.enx(), c.enz(), [=] PMHD_HD(I3 p) { k_emf(cc, p); });
   face_passes(ex, c, 3);   // full-step face field
   ex.run3(K_UPDATE, c.ny, c.nx, c.nz, [=] PMHD_HD(I3 p) { k_update(c, p); });
 }
