@@ -14,6 +14,8 @@ import io
 import os
 import sys
 
+sys.dont_write_bytecode = True    # import the reference read-only: no __pycache__ in its tree
+
 import numpy as np
 
 OUT = os.path.dirname(os.path.abspath(__file__))
