@@ -102,6 +102,51 @@ int pmhd_get_profile(pmhd_handle* h, int32_t max_entries, char* names, double* t
 /* Last error message of the handle (or of the last failed pmhd_create when h is NULL). */
 const char* pmhd_last_error(const pmhd_handle* h);
 
+/* ------------------------------------------------------------------------
+ * Batches of independent boxes.  No counterpart in the reference, whose
+ * Simulation runs one grid at a time (Source/Simulation.py:312-377); a batch is
+ * the same step applied to nbox grids at once.  All boxes share ndim and the
+ * parameters; sizes, widths and boundary conditions may differ per box.  Every
+ * phase of a cycle is one launch over the tiles of all boxes; each box keeps
+ * its own CFL timestep and clock on the device, so pmhd_batch_advance runs
+ * many cycles without a host round trip.  Box b evolves exactly as a single
+ * handle with the same grid, parameters and state would (bit for bit).
+ * ------------------------------------------------------------------------ */
+typedef struct pmhd_batch pmhd_batch;
+
+int pmhd_batch_create(int32_t nbox, const pmhd_grid* grids, const pmhd_params* params,
+                      pmhd_batch** out);
+int pmhd_batch_destroy(pmhd_batch* b);
+
+/* pmhd_set_state / pmhd_get_state of one box. */
+int pmhd_batch_set_state(pmhd_batch* b, int32_t box, const double* prim, const double* cons,
+                         const double* fbx, const double* fby, const double* fbz);
+int pmhd_batch_get_state(pmhd_batch* b, int32_t box, int32_t which, double* prim, double* cons,
+                         double* fbx, double* fby, double* fbz);
+
+/* The evolve loop of every box from (t0, cycle0): each box runs until its own
+ * time reaches t_end or its cycle count max_cycles.  t_out, cycle_out and
+ * t_second_last receive one value per box (each may be NULL). */
+int pmhd_batch_advance(pmhd_batch* b, double t0, double t_end, int64_t cycle0, int64_t max_cycles,
+                       double* t_out, int64_t* cycle_out, double* t_second_last);
+
+/* Tile tuner: for every launch of a cycle, times each candidate tile shape
+ * (threads along z by threads along x, one block per tile or a persistent grid)
+ * reps times on this batch's boxes and keeps the fastest.  The current state is
+ * unchanged; the second-last state is dropped. */
+int pmhd_batch_autotune(pmhd_batch* b, int32_t reps);
+
+/* The tile of each tuned launch: name k at names + 32*k, threads along z (bk)
+ * and along x (bi), grid (0 = one block per tile) and the tuned time in ms. */
+int pmhd_batch_get_tiles(pmhd_batch* b, int32_t max_entries, char* names, int32_t* bk,
+                         int32_t* bi, int32_t* grid, double* ms, int32_t* n_entries);
+
+int pmhd_batch_synchronize(pmhd_batch* b);
+int pmhd_batch_set_profiling(pmhd_batch* b, int32_t enable);
+int pmhd_batch_get_profile(pmhd_batch* b, int32_t max_entries, char* names, double* total_ms,
+                           int64_t* launches, int64_t* elements, int32_t* n_entries);
+const char* pmhd_batch_last_error(const pmhd_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

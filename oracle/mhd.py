@@ -961,7 +961,9 @@ def ctu_transverse_2d_y(prim_x, prim_y, F, fby, dt, dx, dy, bcy):
     hdt = 0.5 * dt
     ny, nx = prim_x.shape[1:]
     if bcy == OUTFLOW:
-        ce = [1] + list(range(ny)) + [ny - 1]
+        # the reference's top ghost is cell row 1 (it raises IndexError when ny == 1;
+        # here row 0, the only row, is used instead)
+        ce = [min(1, ny - 1)] + list(range(ny)) + [ny - 1]
         tb = _take(fby, [1] + list(range(ny + 1)), 0)
         bb = _take(fby, list(range(ny + 1)) + [ny - 1], 0)
         fe_t = [0] + list(range(ny))
@@ -1025,7 +1027,9 @@ def ctu_transverse_3d(d, UL, UR, prim_x, prim_y, flux, faces, emf, dt, dx, dy, d
     rows_l = ce[0:n + 1]
     rows_r = ce[1:n + 2]
     if d == 'y' and bc == OUTFLOW:
-        rows_r_fx = list(range(ny)) + [nx - 1]   # reference quirk (row index nx-1)
+        # reference quirk: row index nx-1 (Source/CTU_HelperModule.py:724); it raises
+        # IndexError when nx > ny, where the last row ny-1 is used instead
+        rows_r_fx = list(range(ny)) + [min(nx - 1, ny - 1)]
     else:
         rows_r_fx = rows_r
 

@@ -14,8 +14,10 @@ DEFAULT_LIB = os.path.join(_HERE, 'libpmhd.so')
 
 EXPORTS = ('pmhd_create', 'pmhd_destroy', 'pmhd_set_state', 'pmhd_get_state', 'pmhd_compute_dt',
            'pmhd_step', 'pmhd_advance', 'pmhd_synchronize', 'pmhd_diagnostics', 'pmhd_set_profiling',
-           'pmhd_get_profile',
-           'pmhd_last_error')
+           'pmhd_get_profile', 'pmhd_last_error',
+           'pmhd_batch_create', 'pmhd_batch_destroy', 'pmhd_batch_set_state', 'pmhd_batch_get_state',
+           'pmhd_batch_advance', 'pmhd_batch_autotune', 'pmhd_batch_get_tiles', 'pmhd_batch_synchronize',
+           'pmhd_batch_set_profiling', 'pmhd_batch_get_profile', 'pmhd_batch_last_error')
 
 
 class PmhdGrid(ctypes.Structure):
@@ -62,8 +64,21 @@ def _declare(lib):
                                      ctypes.POINTER(ctypes.c_int32)]
     lib.pmhd_last_error.argtypes = [H]
     lib.pmhd_last_error.restype = ctypes.c_char_p
+    I32, I64, D = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    lib.pmhd_batch_create.argtypes = [I32, ctypes.POINTER(PmhdGrid), ctypes.POINTER(PmhdParams), ctypes.POINTER(H)]
+    lib.pmhd_batch_destroy.argtypes = [H]
+    lib.pmhd_batch_set_state.argtypes = [H, I32, P, P, P, P, P]
+    lib.pmhd_batch_get_state.argtypes = [H, I32, I32, P, P, P, P, P]
+    lib.pmhd_batch_advance.argtypes = [H, D, D, I64, I64, P, P, P]
+    lib.pmhd_batch_autotune.argtypes = [H, I32]
+    lib.pmhd_batch_get_tiles.argtypes = [H, I32, ctypes.c_char_p, P, P, P, P, ctypes.POINTER(I32)]
+    lib.pmhd_batch_synchronize.argtypes = [H]
+    lib.pmhd_batch_set_profiling.argtypes = [H, I32]
+    lib.pmhd_batch_get_profile.argtypes = lib.pmhd_get_profile.argtypes
+    lib.pmhd_batch_last_error.argtypes = [H]
+    lib.pmhd_batch_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
-        if name != 'pmhd_last_error':
+        if not name.endswith('last_error'):
             getattr(lib, name).restype = ctypes.c_int
     return lib
 
@@ -85,6 +100,63 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
+def make_grid(shape, widths, bcs):
+    """PmhdGrid of a grid of cell shape (nx,) / (ny, nx) / (ny, nx, nz)."""
+    shape = tuple(int(s) for s in shape)
+    g = PmhdGrid()
+    g.ndim = len(shape)
+    if g.ndim == 1:
+        g.nx, g.ny, g.nz = shape[0], 1, 1
+    elif g.ndim == 2:
+        g.ny, g.nx, g.nz = shape[0], shape[1], 1
+    else:
+        g.ny, g.nx, g.nz = shape
+    w = list(widths) + [1.0] * (3 - len(widths))
+    b = list(bcs) + [bcs[0]] * (3 - len(bcs))
+    g.dx, g.dy, g.dz = float(w[0]), float(w[1]), float(w[2])
+    g.bcx, g.bcy, g.bcz = int(b[0]), int(b[1]), int(b[2])
+    return g
+
+
+def make_params(gamma, cfl, min_dens=1e-12, min_pres=1e-12, min_energy=1e-12, order=2,
+                entropy_fix=False, epsilon=0.5, precision=64):
+    p = PmhdParams()
+    p.gamma, p.cfl = float(gamma), float(cfl)
+    p.min_dens, p.min_pres, p.min_energy = float(min_dens), float(min_pres), float(min_energy)
+    p.epsilon = float(epsilon)
+    p.reconstruct_order = int(order)
+    p.entropy_fix = 1 if entropy_fix else 0
+    p.precision = int(precision)
+    p.flags = 0
+    return p
+
+
+def face_shapes(shape):
+    if len(shape) == 1:
+        return []
+    if len(shape) == 2:
+        ny, nx = shape
+        return [(ny, nx + 1), (ny + 1, nx)]
+    ny, nx, nz = shape
+    return [(ny, nx + 1, nz), (ny + 1, nx, nz), (ny, nx, nz + 1)]
+
+
+def _state_args(shape, prim, cons, faces):
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    cons = np.ascontiguousarray(cons, dtype=np.float64)
+    if prim.shape != (8,) + shape or cons.shape != (8,) + shape:
+        raise ValueError('state shape %s does not match the grid %s' % (prim.shape, shape))
+    fs = [np.ascontiguousarray(f, dtype=np.float64) for f in faces]
+    want = face_shapes(shape)
+    if len(fs) != len(want):
+        raise ValueError('expected %d face arrays' % len(want))
+    for f, s in zip(fs, want):
+        if f.shape != s:
+            raise ValueError('face shape %s, expected %s' % (f.shape, s))
+    keep = [prim, cons] + fs
+    return keep, [_ptr(prim), _ptr(cons)] + [_ptr(f) for f in fs] + [None] * (3 - len(fs))
+
+
 class Solver:
     """One device-resident grid state driven through the C ABI."""
 
@@ -94,26 +166,8 @@ class Solver:
         self.lib = lib if lib is not None else load()
         self.ndim = len(shape)
         self.shape = tuple(int(s) for s in shape)   # (nx,) / (ny, nx) / (ny, nx, nz)
-        g = PmhdGrid()
-        g.ndim = self.ndim
-        if self.ndim == 1:
-            g.nx, g.ny, g.nz = self.shape[0], 1, 1
-        elif self.ndim == 2:
-            g.ny, g.nx, g.nz = self.shape[0], self.shape[1], 1
-        else:
-            g.ny, g.nx, g.nz = self.shape
-        w = list(widths) + [1.0] * (3 - len(widths))
-        b = list(bcs) + [bcs[0]] * (3 - len(bcs))
-        g.dx, g.dy, g.dz = float(w[0]), float(w[1]), float(w[2])
-        g.bcx, g.bcy, g.bcz = int(b[0]), int(b[1]), int(b[2])
-        p = PmhdParams()
-        p.gamma, p.cfl = float(gamma), float(cfl)
-        p.min_dens, p.min_pres, p.min_energy = float(min_dens), float(min_pres), float(min_energy)
-        p.epsilon = float(epsilon)
-        p.reconstruct_order = int(order)
-        p.entropy_fix = 1 if entropy_fix else 0
-        p.precision = int(precision)
-        p.flags = 0
+        g = make_grid(self.shape, widths, bcs)
+        p = make_params(gamma, cfl, min_dens, min_pres, min_energy, order, entropy_fix, epsilon, precision)
         self.grid, self.params = g, p
         h = ctypes.c_void_p()
         rc = self.lib.pmhd_create(ctypes.byref(g), ctypes.byref(p), ctypes.byref(h))
@@ -126,28 +180,12 @@ class Solver:
             raise PmhdError('%s failed (%d): %s' % (what, rc, self.lib.pmhd_last_error(self.h).decode()))
 
     def face_shapes(self):
-        s = self.shape
-        if self.ndim == 1:
-            return []
-        if self.ndim == 2:
-            ny, nx = s
-            return [(ny, nx + 1), (ny + 1, nx)]
-        ny, nx, nz = s
-        return [(ny, nx + 1, nz), (ny + 1, nx, nz), (ny, nx, nz + 1)]
+        return face_shapes(self.shape)
 
     def set_state(self, prim, cons, faces=()):
-        prim = np.ascontiguousarray(prim, dtype=np.float64)
-        cons = np.ascontiguousarray(cons, dtype=np.float64)
-        if prim.shape != (8,) + self.shape or cons.shape != (8,) + self.shape:
-            raise ValueError('state shape %s does not match the grid %s' % (prim.shape, self.shape))
-        fs = [np.ascontiguousarray(f, dtype=np.float64) for f in faces]
-        for f, s in zip(fs, self.face_shapes()):
-            if f.shape != s:
-                raise ValueError('face shape %s, expected %s' % (f.shape, s))
-        if len(fs) != len(self.face_shapes()):
-            raise ValueError('expected %d face arrays' % len(self.face_shapes()))
-        fp = [_ptr(f) for f in fs] + [None] * (3 - len(fs))
-        self._check(self.lib.pmhd_set_state(self.h, _ptr(prim), _ptr(cons), *fp), 'pmhd_set_state')
+        keep, args = _state_args(self.shape, prim, cons, faces)
+        self._check(self.lib.pmhd_set_state(self.h, *args), 'pmhd_set_state')
+        del keep
 
     def get_state(self, which=0):
         prim = np.empty((8,) + self.shape)
@@ -209,6 +247,115 @@ class Solver:
     def close(self):
         if getattr(self, 'h', None) is not None and self.h.value:
             self.lib.pmhd_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Independent boxes advanced together: one launch per phase over all boxes,
+    per-box CFL timestep and clock on the device (pmhd_batch_* in include/pmhd.h)."""
+
+    def __init__(self, shapes, widths, bcs, gamma, cfl, min_dens=1e-12, min_pres=1e-12,
+                 min_energy=1e-12, order=2, entropy_fix=False, epsilon=0.5, precision=64, lib=None):
+        self.lib = lib if lib is not None else load()
+        self.shapes = [tuple(int(x) for x in s) for s in shapes]
+        n = len(self.shapes)
+        if n < 1:
+            raise ValueError('a batch needs at least one box')
+        widths = list(widths) if len(widths) == n and hasattr(widths[0], '__len__') else [widths] * n
+        bcs = list(bcs) if len(bcs) == n and hasattr(bcs[0], '__len__') else [bcs] * n
+        grids = (PmhdGrid * n)(*[make_grid(s, w, b) for s, w, b in zip(self.shapes, widths, bcs)])
+        p = make_params(gamma, cfl, min_dens, min_pres, min_energy, order, entropy_fix, epsilon, precision)
+        self.grids, self.params = grids, p
+        h = ctypes.c_void_p()
+        rc = self.lib.pmhd_batch_create(n, grids, ctypes.byref(p), ctypes.byref(h))
+        if rc != 0:
+            raise PmhdError('pmhd_batch_create failed (%d): %s' % (rc, self.lib.pmhd_batch_last_error(None).decode()))
+        self.h = h
+        self.nbox = n
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise PmhdError('%s failed (%d): %s' % (what, rc, self.lib.pmhd_batch_last_error(self.h).decode()))
+
+    def _shape(self, box):
+        if not 0 <= int(box) < self.nbox:
+            raise PmhdError('box index %d out of range (batch of %d boxes)' % (box, self.nbox))
+        return self.shapes[int(box)]
+
+    def set_state(self, box, prim, cons, faces=()):
+        keep, args = _state_args(self._shape(box), prim, cons, faces)
+        self._check(self.lib.pmhd_batch_set_state(self.h, int(box), *args), 'pmhd_batch_set_state')
+        del keep
+
+    def get_state(self, box, which=0):
+        shape = self._shape(box)
+        prim = np.empty((8,) + shape)
+        cons = np.empty((8,) + shape)
+        fs = [np.empty(s) for s in face_shapes(shape)]
+        fp = [_ptr(f) for f in fs] + [None] * (3 - len(fs))
+        self._check(self.lib.pmhd_batch_get_state(self.h, int(box), int(which), _ptr(prim), _ptr(cons), *fp),
+                    'pmhd_batch_get_state')
+        return prim, cons, fs
+
+    def advance(self, t0, t_end, cycle0, max_cycles):
+        """Returns per-box arrays (t, cycle, t_second_last)."""
+        t = np.zeros(self.nbox)
+        cyc = np.zeros(self.nbox, dtype=np.int64)
+        t2 = np.zeros(self.nbox)
+        self._check(self.lib.pmhd_batch_advance(self.h, float(t0), float(t_end), int(cycle0), int(max_cycles),
+                                                _ptr(t), _ptr(cyc), _ptr(t2)), 'pmhd_batch_advance')
+        return t, cyc, t2
+
+    def autotune(self, reps=3):
+        self._check(self.lib.pmhd_batch_autotune(self.h, int(reps)), 'pmhd_batch_autotune')
+
+    def tiles(self):
+        """{launch name: (threads along z, threads along x, grid, tuned ms)}."""
+        m = 64
+        names = ctypes.create_string_buffer(32 * m)
+        bk = np.zeros(m, dtype=np.int32)
+        bi = np.zeros(m, dtype=np.int32)
+        gr = np.zeros(m, dtype=np.int32)
+        ms = np.zeros(m)
+        n = ctypes.c_int32()
+        self._check(self.lib.pmhd_batch_get_tiles(self.h, m, names, _ptr(bk), _ptr(bi), _ptr(gr), _ptr(ms),
+                                                  ctypes.byref(n)), 'pmhd_batch_get_tiles')
+        out = {}
+        for k in range(n.value):
+            nm = names.raw[32 * k:32 * k + 32].split(b'\0', 1)[0].decode()
+            out[nm] = (int(bk[k]), int(bi[k]), int(gr[k]), float(ms[k]))
+        return out
+
+    def set_profiling(self, on):
+        self._check(self.lib.pmhd_batch_set_profiling(self.h, 1 if on else 0), 'pmhd_batch_set_profiling')
+
+    def profile(self):
+        m = 64
+        names = ctypes.create_string_buffer(32 * m)
+        ms = (ctypes.c_double * m)()
+        nl = (ctypes.c_int64 * m)()
+        ne = (ctypes.c_int64 * m)()
+        n = ctypes.c_int32()
+        self._check(self.lib.pmhd_batch_get_profile(self.h, m, names, ms, nl, ne, ctypes.byref(n)),
+                    'pmhd_batch_get_profile')
+        out = {}
+        for k in range(n.value):
+            nm = names.raw[32 * k:32 * k + 32].split(b'\0', 1)[0].decode()
+            out[nm] = (ms[k], nl[k], ne[k])
+        return out
+
+    def synchronize(self):
+        self._check(self.lib.pmhd_batch_synchronize(self.h), 'pmhd_batch_synchronize')
+
+    def close(self):
+        if getattr(self, 'h', None) is not None and self.h.value:
+            self.lib.pmhd_batch_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):

@@ -1,11 +1,15 @@
 // Implementation of the C ABI declared in include/pmhd.h, written once against a
-// backend B that provides device memory, copies and an executor:
-//   void* B::alloc(size_t); void B::release(void*);
+// backend B that provides device memory, copies and executors:
+//   void* B::alloc(size_t); void B::release(void*); void B::zero(void*, size_t);
 //   void B::h2d(void*, const void*, size_t); void B::d2h(void*, const void*, size_t);
 //   void B::sync(); const char* B::error();   (nullptr when no runtime error)
-//   B::Exec& B::exec();
+//   B::exec()                 -- single-grid executor
+//   B::batch_exec(BatchLayout*) -- executor over all boxes of a batch
 #pragma once
+#include <algorithm>
 #include <cstring>
+#include <functional>
+#include <limits>
 #include <new>
 #include <string>
 #include <vector>
@@ -33,19 +37,257 @@ struct pmhd_handle {
   }
 };
 
+struct pmhd_batch {
+  virtual ~pmhd_batch() {}
+  std::string err;
+  int nbox = 0;
+  virtual int set_state(int box, const double*, const double*, const double*, const double*, const double*) = 0;
+  virtual int get_state(int box, int which, double*, double*, double*, double*, double*) = 0;
+  virtual int advance(double t0, double t_end, int64_t cycle0, int64_t max_cycles, double* t_out,
+                      int64_t* cycle_out, double* t2_out) = 0;
+  virtual int autotune(int reps) = 0;
+  virtual int get_tiles(int maxn, char* names, int32_t* bk, int32_t* bi, int32_t* grid, double* ms,
+                        int32_t* n) = 0;
+  virtual int synchronize() = 0;
+  virtual int set_profiling(int on) = 0;
+  virtual int get_profile(int maxn, char* names, double* ms, int64_t* launches, int64_t* elems,
+                          int32_t* n) = 0;
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+};
+
 namespace pmhd {
 
-inline int64_t nfaces_of(const pmhd_grid& g, int d) {
-  return (g.ny + (d == 1)) * (g.nx + (d == 0)) * (g.nz + (d == 2));
+// ---------------------------------------------------------------------------
+// batch launch tables: per launch id, every box's index box and the tile shape
+// (bk threads along z by bi along x) and grid; prefix[id][b] = first tile of box b
+// ---------------------------------------------------------------------------
+struct TileCfg {
+  int bk, bi;
+  int grid;   // blocks of the launch; 0 = one block per tile
+};
+
+struct BatchLayout {
+  int nbox = 0;
+  int ndim = 0;
+  int cus = 256;
+  int64_t max_n2 = 1;                 // longest contiguous extent over launches and boxes
+  void* dctx = nullptr;               // device Ctx<T>[nbox]
+  BoxClock* dclk = nullptr;           // device clocks
+  unsigned long long* dred = nullptr; // device CFL maxima, 3 per box
+  int* dprefix = nullptr;             // device [K_COUNT][nbox + 1]
+  std::vector<int64_t> ext;           // [K_COUNT][nbox][3]
+  std::vector<int> hprefix;           // host copy of the prefix tables
+  TileCfg tiles[K_COUNT];
+  int total[K_COUNT] = {};
+  int64_t elems[K_COUNT] = {};
+  float tuned_ms[K_COUNT];
+  std::function<void(int)> upload;    // copies hprefix[id] to the device
+
+  BatchLayout() {
+    for (int id = 0; id < K_COUNT; ++id) {
+      tiles[id] = TileCfg{1, 256, 0};
+      tuned_ms[id] = -1.f;
+    }
+  }
+
+  // use tile t for launch id; false (and no change) when its tile count would
+  // not fit the 32-bit tile index of the batch kernels
+  bool apply_tile(int id, TileCfg t) {
+    int64_t acc = 0, el = 0;
+    for (int b = 0; b < nbox; ++b) {
+      const int64_t* e = ext.data() + (size_t(id) * size_t(nbox) + size_t(b)) * 3;
+      acc += e[0] * ((e[1] + t.bi - 1) / t.bi) * ((e[2] + t.bk - 1) / t.bk);
+      el += e[0] * e[1] * e[2];
+    }
+    if (acc > int64_t(std::numeric_limits<int>::max())) return false;
+    tiles[id] = t;
+    int* pre = hprefix.data() + size_t(id) * size_t(nbox + 1);
+    acc = 0;
+    for (int b = 0; b < nbox; ++b) {
+      pre[b] = int(acc);
+      const int64_t* e = ext.data() + (size_t(id) * size_t(nbox) + size_t(b)) * 3;
+      acc += e[0] * ((e[1] + t.bi - 1) / t.bi) * ((e[2] + t.bk - 1) / t.bk);
+    }
+    pre[nbox] = int(acc);
+    total[id] = int(acc);
+    elems[id] = el;
+    if (upload) upload(id);
+    return true;
+  }
+};
+
+// block of 256 threads, as wide along z as the boxes allow
+inline TileCfg default_tile(const BatchLayout& lay) {
+  int bk = 1;
+  while (bk < 64 && bk < lay.max_n2) bk <<= 1;
+  return TileCfg{bk, 256 / bk, 0};
 }
 
+// the shapes the tile tuner tries: 64-256 threads, one block per tile or a
+// persistent grid of a few blocks per compute unit
+inline std::vector<TileCfg> tile_candidates(const BatchLayout& lay, int) {
+  std::vector<std::pair<int, int>> shapes;
+  if (lay.max_n2 <= 1) {
+    shapes = {{1, 64}, {1, 128}, {1, 256}};
+  } else {
+    shapes = {{64, 1}, {64, 2}, {64, 4}, {32, 4}, {32, 8}, {16, 8}, {16, 16}};
+  }
+  std::vector<TileCfg> out;
+  for (const auto& s : shapes) {
+    out.push_back(TileCfg{s.first, s.second, 0});
+    out.push_back(TileCfg{s.first, s.second, 8 * lay.cus});
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// grid set-up shared by single grids and batch boxes
+// ---------------------------------------------------------------------------
+template <class T>
+void init_ctx(Ctx<T>& c, const pmhd_grid& g, const pmhd_params& p) {
+  std::memset(&c, 0, sizeof(c));
+  c.ndim = g.ndim;
+  c.active = 1;
+  c.ny = g.ndim >= 2 ? g.ny : 1;
+  c.nx = g.nx;
+  c.nz = g.ndim == 3 ? g.nz : 1;
+  c.bc[0] = g.bcx;
+  c.bc[1] = g.bcy;
+  c.bc[2] = g.bcz;
+  c.bc_zrec = g.bcx;
+  c.order = p.reconstruct_order;
+  c.efix = p.entropy_fix;
+  c.gamma = T(p.gamma);
+  c.eps = T(p.epsilon);
+  c.min_dens = T(p.min_dens);
+  c.min_pres = T(p.min_pres);
+  c.min_energy = T(p.min_energy);
+  const double w[3] = {g.dx, g.dy, g.dz};
+  for (int d = 0; d < 3; ++d) {
+    c.width[d] = T(w[d]);
+    c.wd[d] = w[d];
+    c.invd[d] = T(1.0 / w[d]);
+  }
+}
+
+// the two state buffers of a grid (the update writes the other one)
+template <class T>
+struct StateBufs {
+  T* U[2] = {nullptr, nullptr};
+  T* W[2] = {nullptr, nullptr};
+  T* FB[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+};
+
+// every device buffer of one grid; alloc(n) returns room for n values of T
+template <class T, class A>
+void alloc_grid(Ctx<T>& c, StateBufs<T>& s, int order, A&& alloc) {
+  const int nd = c.ndim;
+  const int64_t N = c.ncell();
+  int64_t maxf = 0, maxs = 0;
+  for (int d = 0; d < nd; ++d) {
+    maxf = std::max<int64_t>(maxf, c.nface(d));
+    maxs = std::max<int64_t>(maxs, c.nslope(d));
+  }
+  for (int b = 0; b < 2; ++b) {
+    s.U[b] = alloc(8 * N);
+    s.W[b] = alloc(8 * N);
+    for (int d = 0; d < 3; ++d) s.FB[b][d] = alloc(d < nd && nd >= 2 ? c.nface(d) : 1);
+  }
+  c.WL = alloc(7 * maxf);
+  c.WR = alloc(7 * maxf);
+  c.S = order > 0 ? alloc(7 * maxs) : alloc(1);
+  for (int d = 0; d < 3; ++d) {
+    const bool used = d < nd;
+    c.UL[d] = alloc(used ? 7 * c.nface(d) : 1);
+    c.UR[d] = alloc(used ? 7 * c.nface(d) : 1);
+    c.F[d] = alloc(used ? 7 * c.nface(d) : 1);
+    c.Fc[d] = alloc(used && nd >= 2 ? 7 * c.nface(d) : 1);
+    c.nfb[d] = alloc(used && nd >= 2 ? c.nface(d) : 1);
+    c.E[d] = alloc(nd == 3 || (nd == 2 && d == 2) ? c.nedge() : 1);
+  }
+  c.Ec = alloc(nd == 3 ? 3 * N : N);
+}
+
+// point the context at buffer set `cur` (the update writes 1 - cur)
+template <class T>
+void bind_bufs(Ctx<T>& c, const StateBufs<T>& s, int cur) {
+  const int nx = 1 - cur;
+  c.U = s.U[cur];
+  c.W = s.W[cur];
+  c.U2 = s.U[nx];
+  c.W2 = s.W[nx];
+  for (int d = 0; d < 3; ++d) {
+    c.fb[d] = s.FB[cur][d];
+    c.fb2[d] = s.FB[nx][d];
+  }
+}
+
+// host float64 <-> device T
+template <class T, class B>
+void upload_vals(B& be, T* dst, const double* src, int64_t n) {
+  if (sizeof(T) == sizeof(double)) {
+    be.h2d(dst, src, size_t(n) * sizeof(double));
+  } else {
+    std::vector<T> tmp(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) tmp[size_t(i)] = T(src[i]);
+    be.h2d(dst, tmp.data(), size_t(n) * sizeof(T));
+  }
+}
+template <class T, class B>
+void download_vals(B& be, double* dst, const T* src, int64_t n) {
+  if (sizeof(T) == sizeof(double)) {
+    be.d2h(dst, src, size_t(n) * sizeof(double));
+  } else {
+    std::vector<T> tmp(static_cast<size_t>(n));
+    be.d2h(tmp.data(), src, size_t(n) * sizeof(T));
+    for (int64_t i = 0; i < n; ++i) dst[i] = double(tmp[size_t(i)]);
+  }
+}
+
+// load / store the reference-layout state of one grid into buffer set `s`
+template <class T, class B>
+int load_state(B& be, Ctx<T>& c, const pmhd_grid& g, const double* prim, const double* cons,
+               const double* const fin[3], T* U, T* W, T* const FB[3], std::string& err) {
+  if (!prim || !cons) {
+    err = "prim and cons are required";
+    return PMHD_EINVAL;
+  }
+  for (int d = 0; d < g.ndim && g.ndim >= 2; ++d)
+    if (!fin[d]) {
+      err = "missing face array";
+      return PMHD_EINVAL;
+    }
+  const int64_t N = c.ncell();
+  upload_vals(be, U, cons, 8 * N);
+  upload_vals(be, W, prim, 8 * N);
+  if (g.ndim >= 2)
+    for (int d = 0; d < g.ndim; ++d) upload_vals(be, FB[d], fin[d], c.nface(d));
+  if (g.ndim == 1) c.bx1d = T(prim[4 * N]);
+  return PMHD_OK;
+}
+
+template <class T, class B>
+void store_state(B& be, const Ctx<T>& c, const pmhd_grid& g, const T* U, const T* W, const T* const FB[3],
+                 double* prim, double* cons, double* const fo[3]) {
+  const int64_t N = c.ncell();
+  if (cons) download_vals(be, cons, U, 8 * N);
+  if (prim) download_vals(be, prim, W, 8 * N);
+  if (g.ndim >= 2)
+    for (int d = 0; d < g.ndim; ++d)
+      if (fo[d]) download_vals(be, fo[d], FB[d], c.nface(d));
+}
+
+// ---------------------------------------------------------------------------
+// one grid
+// ---------------------------------------------------------------------------
 template <class T, class B>
 struct Handle : public pmhd_handle {
   B be;
   Ctx<T> c;
-  T* U[2] = {nullptr, nullptr};
-  T* W[2] = {nullptr, nullptr};
-  T* FB[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  StateBufs<T> sb;
   std::vector<void*> owned;
   int cur = 0;
   int64_t cycles_since_set = 0;
@@ -65,127 +307,37 @@ struct Handle : public pmhd_handle {
   int init(const pmhd_grid& g_, const pmhd_params& p_) {
     g = g_;
     p = p_;
-    std::memset(&c, 0, sizeof(c));
-    c.ndim = g.ndim;
-    c.ny = g.ndim >= 2 ? g.ny : 1;
-    c.nx = g.nx;
-    c.nz = g.ndim == 3 ? g.nz : 1;
-    g.ny = c.ny;
-    g.nz = c.nz;
-    c.bc[0] = g.bcx;
-    c.bc[1] = g.bcy;
-    c.bc[2] = g.bcz;
-    c.bc_zrec = g.bcx;
-    c.order = p.reconstruct_order;
-    c.efix = p.entropy_fix;
-    c.gamma = T(p.gamma);
-    c.eps = T(p.epsilon);
-    c.min_dens = T(p.min_dens);
-    c.min_pres = T(p.min_pres);
-    c.min_energy = T(p.min_energy);
-    const double w[3] = {g.dx, g.dy, g.dz};
-    for (int d = 0; d < 3; ++d) {
-      c.width[d] = T(w[d]);
-      c.wd[d] = w[d];
-      c.invd[d] = T(1.0 / w[d]);
-    }
+    init_ctx(c, g, p);
     try {
-      const int64_t N = c.ncell();
-      int64_t maxf = 0, maxs = 0;
-      for (int d = 0; d < g.ndim; ++d) {
-        maxf = std::max<int64_t>(maxf, c.nface(d));
-        maxs = std::max<int64_t>(maxs, c.nslope(d));
-      }
-      for (int s = 0; s < 2; ++s) {
-        U[s] = dalloc(8 * N);
-        W[s] = dalloc(8 * N);
-        for (int d = 0; d < 3; ++d) FB[s][d] = dalloc(d < g.ndim && g.ndim >= 2 ? c.nface(d) : 1);
-      }
-      c.WL = dalloc(7 * maxf);
-      c.WR = dalloc(7 * maxf);
-      c.S = p.reconstruct_order > 0 ? dalloc(7 * maxs) : dalloc(1);
-      for (int d = 0; d < 3; ++d) {
-        const bool used = d < g.ndim;
-        c.UL[d] = dalloc(used ? 7 * c.nface(d) : 1);
-        c.UR[d] = dalloc(used ? 7 * c.nface(d) : 1);
-        c.F[d] = dalloc(used ? 7 * c.nface(d) : 1);
-        c.Fc[d] = dalloc(used && g.ndim >= 2 ? 7 * c.nface(d) : 1);
-        c.nfb[d] = dalloc(used && g.ndim >= 2 ? c.nface(d) : 1);
-        c.E[d] = dalloc(g.ndim == 3 || (g.ndim == 2 && d == 2) ? c.nedge() : 1);
-      }
-      c.Ec = dalloc(g.ndim == 3 ? 3 * N : N);
+      alloc_grid(c, sb, p.reconstruct_order, [&](int64_t n) { return dalloc(n); });
     } catch (const std::bad_alloc&) {
       return fail(PMHD_ENOMEM, "device allocation failed");
     }
-    bind();
+    bind_bufs(c, sb, cur);
     const char* e = be.error();
     if (e) return fail(PMHD_EHIP, e);
     return PMHD_OK;
   }
 
-  void bind() {
-    const int nx = 1 - cur;
-    c.U = U[cur];
-    c.W = W[cur];
-    c.U2 = U[nx];
-    c.W2 = W[nx];
-    for (int d = 0; d < 3; ++d) {
-      c.fb[d] = FB[cur][d];
-      c.fb2[d] = FB[nx][d];
-    }
-  }
-
-  void upload(T* dst, const double* src, int64_t n) {
-    if (sizeof(T) == sizeof(double)) {
-      be.h2d(dst, src, size_t(n) * sizeof(double));
-    } else {
-      std::vector<T> tmp(static_cast<size_t>(n));
-      for (int64_t i = 0; i < n; ++i) tmp[size_t(i)] = T(src[i]);
-      be.h2d(dst, tmp.data(), size_t(n) * sizeof(T));
-    }
-  }
-  void download(double* dst, const T* src, int64_t n) {
-    if (sizeof(T) == sizeof(double)) {
-      be.d2h(dst, src, size_t(n) * sizeof(double));
-    } else {
-      std::vector<T> tmp(static_cast<size_t>(n));
-      be.d2h(tmp.data(), src, size_t(n) * sizeof(T));
-      for (int64_t i = 0; i < n; ++i) dst[i] = double(tmp[size_t(i)]);
-    }
-  }
-
   int set_state(const double* prim, const double* cons, const double* fbx, const double* fby,
                 const double* fbz) override {
-    if (!prim || !cons) return fail(PMHD_EINVAL, "prim and cons are required");
     const double* fin[3] = {fbx, fby, fbz};
-    for (int d = 0; d < g.ndim && g.ndim >= 2; ++d)
-      if (!fin[d]) return fail(PMHD_EINVAL, "missing face array");
-    const int64_t N = c.ncell();
     cur = 0;
     cycles_since_set = 0;
-    bind();
-    upload(U[0], cons, 8 * N);
-    upload(W[0], prim, 8 * N);
-    if (g.ndim >= 2)
-      for (int d = 0; d < g.ndim; ++d) upload(FB[0][d], fin[d], c.nface(d));
-    if (g.ndim == 1) c.bx1d = T(prim[4 * N]);
+    bind_bufs(c, sb, cur);
+    int rc = load_state(be, c, g, prim, cons, fin, sb.U[0], sb.W[0], sb.FB[0], err);
+    if (rc) return rc;
     be.sync();
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
   }
 
-  int get_state(int which, double* prim, double* cons, double* fbx, double* fby,
-                double* fbz) override {
+  int get_state(int which, double* prim, double* cons, double* fbx, double* fby, double* fbz) override {
     int s = cur;
     if (which == 1 && cycles_since_set > 0) s = 1 - cur;
-    const int64_t N = c.ncell();
-    be.sync();
-    if (cons) download(cons, U[s], 8 * N);
-    if (prim) download(prim, W[s], 8 * N);
     double* fo[3] = {fbx, fby, fbz};
-    if (g.ndim >= 2)
-      for (int d = 0; d < g.ndim; ++d)
-        if (fo[d]) download(fo[d], FB[s][d], c.nface(d));
+    be.sync();
+    store_state(be, c, g, sb.U[s], sb.W[s], sb.FB[s], prim, cons, fo);
     be.sync();
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
@@ -193,32 +345,15 @@ struct Handle : public pmhd_handle {
 
   int compute_dt(double* dt) override {
     double m[3] = {0, 0, 0};
-    cfl_maxima(be.exec(), c, m);
+    be.exec().cfl(c, m);
     const char* e = be.error();
     if (e) return fail(PMHD_EHIP, e);
-    // Source/TimestepCalculations_HelperModule.py:222-225, 285-289, 354-361
-    const double cfl = p.cfl;
-    if (g.ndim == 1) {
-      *dt = cfl / (m[0] / g.dx);
-    } else if (g.ndim == 2) {
-      *dt = cfl / npmax(m[0] / g.dx, m[1] / g.dy);
-    } else {
-      *dt = cfl / npmax(m[0] / g.dx, npmax(m[1] / g.dy, m[2] / g.dz));
-    }
+    *dt = dt_from_maxima(g.ndim, p.cfl, m, c.wd);
     return PMHD_OK;
   }
 
   int step(double dt) override {
-    const double w[3] = {g.dx, g.dy, g.dz};
-    c.dt = T(dt);
-    c.hdt = T(0.5 * dt);
-    for (int d = 0; d < 3; ++d) {
-      const double dtd = dt / w[d];
-      const double h = 0.5 * dtd;
-      c.dtd[d] = T(dtd);
-      c.hdtd[d] = T(h);
-      c.emfq[d] = T(h * 0.5);
-    }
+    set_step_scalars(c, dt);
     {
       // Source/CTU_HelperModule.py:455: dtdy = longdouble(dt)/longdouble(dy)
       const long double hl = 0.5L * (static_cast<long double>(dt) / static_cast<long double>(g.dy));
@@ -227,7 +362,7 @@ struct Handle : public pmhd_handle {
     }
     cycle(be.exec(), c);
     cur = 1 - cur;
-    bind();
+    bind_bufs(c, sb, cur);
     ++cycles_since_set;
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
@@ -258,13 +393,291 @@ struct Handle : public pmhd_handle {
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
   }
 
-  int get_profile(int maxn, char* names, double* ms, int64_t* launches, int64_t* elems,
-                  int32_t* n) override {
+  int get_profile(int maxn, char* names, double* ms, int64_t* launches, int64_t* elems, int32_t* n) override {
     *n = be.profile(maxn, names, ms, launches, elems);
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
   }
 };
+
+// ---------------------------------------------------------------------------
+// a batch of independent boxes (no counterpart in the reference): every phase is
+// one launch over the tiles of all boxes, each box keeps its own timestep and
+// clock on the device, so a run of many cycles needs no host round trip
+// ---------------------------------------------------------------------------
+template <class T, class B>
+struct BatchHandle : public pmhd_batch {
+  B be;
+  pmhd_params p;
+  std::vector<pmhd_grid> grids;
+  std::vector<Ctx<T>> hctx;           // host mirror of the device contexts
+  std::vector<StateBufs<T>> bufs;
+  std::vector<BoxClock> hclk;
+  BatchLayout lay;
+  void* arena = nullptr;
+  std::vector<void*> owned;
+
+  ~BatchHandle() override {
+    for (void* ptr : owned) be.release(ptr);
+  }
+
+  void* dalloc_bytes(size_t bytes) {
+    void* ptr = be.alloc(bytes > 0 ? bytes : 1);
+    if (!ptr) throw std::bad_alloc();
+    owned.push_back(ptr);
+    return ptr;
+  }
+
+  int init(int n, const pmhd_grid* gs, const pmhd_params& p_) {
+    nbox = n;
+    p = p_;
+    grids.assign(gs, gs + n);
+    hctx.resize(size_t(n));
+    bufs.resize(size_t(n));
+    hclk.assign(size_t(n), BoxClock{0, 0, 0, 0, 0});
+    for (int b = 0; b < n; ++b) init_ctx(hctx[size_t(b)], grids[size_t(b)], p);
+    try {
+      // one arena for all boxes: a sizing pass, then the carving pass
+      size_t total = 0;
+      const size_t align = 256;
+      for (int pass = 0; pass < 2; ++pass) {
+        char* base = static_cast<char*>(arena);
+        size_t off = 0;
+        for (int b = 0; b < n; ++b) {
+          alloc_grid(hctx[size_t(b)], bufs[size_t(b)], p.reconstruct_order, [&](int64_t cnt) {
+            const size_t bytes = size_t(std::max<int64_t>(cnt, 1)) * sizeof(T);
+            T* ptr = pass == 0 ? nullptr : reinterpret_cast<T*>(base + off);
+            off += (bytes + align - 1) / align * align;
+            return ptr;
+          });
+        }
+        if (pass == 0) {
+          total = off;
+          arena = dalloc_bytes(total);
+        }
+      }
+      lay.nbox = n;
+      lay.dctx = dalloc_bytes(sizeof(Ctx<T>) * size_t(n));
+      lay.dclk = static_cast<BoxClock*>(dalloc_bytes(sizeof(BoxClock) * size_t(n)));
+      lay.dred = static_cast<unsigned long long*>(dalloc_bytes(3 * sizeof(unsigned long long) * size_t(n)));
+      lay.dprefix = static_cast<int*>(dalloc_bytes(sizeof(int) * size_t(K_COUNT) * size_t(n + 1)));
+    } catch (const std::bad_alloc&) {
+      return fail(PMHD_ENOMEM, "device allocation failed");
+    }
+    for (int b = 0; b < n; ++b) bind_bufs(hctx[size_t(b)], bufs[size_t(b)], 0);
+    be.zero(lay.dred, 3 * sizeof(unsigned long long) * size_t(n));
+    // tile tables
+    lay.ndim = grids[0].ndim;
+    lay.cus = be.compute_units();
+    lay.ext.assign(size_t(K_COUNT) * size_t(n) * 3, 0);
+    lay.max_n2 = 1;
+    for (int id = 0; id < K_COUNT; ++id) {
+      int kind, dir;
+      phase_shape(id, kind, dir);
+      for (int b = 0; b < n; ++b) {
+        int64_t e[3];
+        hctx[size_t(b)].extent(kind, dir, e);
+        for (int q = 0; q < 3; ++q) lay.ext[(size_t(id) * size_t(n) + size_t(b)) * 3 + size_t(q)] = e[q];
+        lay.max_n2 = std::max<int64_t>(lay.max_n2, e[2]);
+      }
+    }
+    lay.hprefix.assign(size_t(K_COUNT) * size_t(n + 1), 0);
+    lay.upload = [this](int id) {
+      be.h2d(lay.dprefix + size_t(id) * size_t(nbox + 1), lay.hprefix.data() + size_t(id) * size_t(nbox + 1),
+             sizeof(int) * size_t(nbox + 1));
+    };
+    for (int id = 0; id < K_COUNT; ++id)
+      if (!lay.apply_tile(id, default_tile(lay))) return fail(PMHD_EINVAL, "batch too large: more than 2^31 tiles");
+    push_ctx();
+    const char* e = be.error();
+    if (e) return fail(PMHD_EHIP, e);
+    return PMHD_OK;
+  }
+
+  void push_ctx() { be.h2d(lay.dctx, hctx.data(), sizeof(Ctx<T>) * size_t(nbox)); }
+  void pull_ctx() { be.d2h(hctx.data(), lay.dctx, sizeof(Ctx<T>) * size_t(nbox)); }
+
+  int check_box(int box) {
+    if (box < 0 || box >= nbox) return fail(PMHD_EINVAL, "box index out of range");
+    return PMHD_OK;
+  }
+
+  int set_state(int box, const double* prim, const double* cons, const double* fbx, const double* fby,
+                const double* fbz) override {
+    if (check_box(box)) return PMHD_EINVAL;
+    Ctx<T>& c = hctx[size_t(box)];
+    const double* fin[3] = {fbx, fby, fbz};
+    bind_bufs(c, bufs[size_t(box)], 0);
+    int rc = load_state(be, c, grids[size_t(box)], prim, cons, fin, bufs[size_t(box)].U[0], bufs[size_t(box)].W[0],
+                        bufs[size_t(box)].FB[0], err);
+    if (rc) return rc;
+    hclk[size_t(box)].done = 0;
+    be.h2d(static_cast<Ctx<T>*>(lay.dctx) + box, &c, sizeof(Ctx<T>));
+    be.sync();
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+
+  int get_state(int box, int which, double* prim, double* cons, double* fbx, double* fby, double* fbz) override {
+    if (check_box(box)) return PMHD_EINVAL;
+    const Ctx<T>& c = hctx[size_t(box)];
+    const bool second = which == 1 && hclk[size_t(box)].done > 0;
+    const T* fb[3] = {second ? c.fb2[0] : c.fb[0], second ? c.fb2[1] : c.fb[1], second ? c.fb2[2] : c.fb[2]};
+    double* fo[3] = {fbx, fby, fbz};
+    be.sync();
+    store_state(be, c, grids[size_t(box)], second ? c.U2 : c.U, second ? c.W2 : c.W, fb, prim, cons, fo);
+    be.sync();
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+
+  int advance(double t0, double t_end, int64_t cycle0, int64_t max_cycles, double* t_out, int64_t* cycle_out,
+              double* t2_out) override {
+    for (int b = 0; b < nbox; ++b) {
+      BoxClock& k = hclk[size_t(b)];
+      k.t = t0;
+      k.t2 = t0;
+      k.dt = 0;
+      k.cyc = cycle0;
+    }
+    be.h2d(lay.dclk, hclk.data(), sizeof(BoxClock) * size_t(nbox));
+    auto& bx = be.batch_exec(&lay);
+    bx.template activate_all<T>();
+    const int64_t iters = max_cycles - cycle0;
+    for (int64_t it = 0; it < iters; ++it) {
+      bx.cfl(hctx[0]);
+      bx.template begin<T>(p.cfl, t_end, max_cycles, 1);
+      cycle(bx, hctx[0]);
+      bx.template end<T>();
+      if ((it + 1) % 16 == 0 && it + 1 < iters) {
+        be.d2h(hclk.data(), lay.dclk, sizeof(BoxClock) * size_t(nbox));
+        bool any = false;
+        for (const BoxClock& k : hclk) any = any || (k.t < t_end && k.cyc < max_cycles);
+        if (!any) break;
+      }
+      const char* e = be.error();
+      if (e) return fail(PMHD_EHIP, e);
+    }
+    be.d2h(hclk.data(), lay.dclk, sizeof(BoxClock) * size_t(nbox));
+    pull_ctx();
+    for (int b = 0; b < nbox; ++b) {
+      if (t_out) t_out[b] = hclk[size_t(b)].t;
+      if (cycle_out) cycle_out[b] = hclk[size_t(b)].cyc;
+      if (t2_out) t2_out[b] = hclk[size_t(b)].t2;
+    }
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+
+  // time every candidate tile of every phase on this batch (one cycle's worth of
+  // launches per candidate, writing only work and next-state buffers) and keep
+  // the fastest; the current state is unchanged, the second-last is dropped
+  int autotune(int reps) override {
+    auto& bx = be.batch_exec(&lay);
+    bx.template activate_all<T>();
+    bx.tuning = true;
+    bx.tune_reps = reps > 0 ? reps : 3;
+    bx.cfl(hctx[0]);
+    bx.template begin<T>(p.cfl, std::numeric_limits<double>::infinity(), std::numeric_limits<int64_t>::max(), 0);
+    cycle(bx, hctx[0]);
+    bx.tuning = false;
+    be.zero(lay.dred, 3 * sizeof(unsigned long long) * size_t(nbox));
+    be.sync();
+    for (BoxClock& k : hclk) k.done = 0;
+    pull_ctx();
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+
+  int get_tiles(int maxn, char* names, int32_t* bk, int32_t* bi, int32_t* grid, double* ms, int32_t* n) override {
+    int k = 0;
+    for (int id = 0; id < K_COUNT && k < maxn; ++id) {
+      if (lay.tuned_ms[id] < 0.f) continue;
+      std::strncpy(names + 32 * k, kernel_name(id), 31);
+      names[32 * k + 31] = 0;
+      bk[k] = lay.tiles[id].bk;
+      bi[k] = lay.tiles[id].bi;
+      grid[k] = lay.tiles[id].grid;
+      ms[k] = double(lay.tuned_ms[id]);
+      ++k;
+    }
+    *n = k;
+    return PMHD_OK;
+  }
+
+  int synchronize() override {
+    be.sync();
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+  int set_profiling(int on) override {
+    be.set_profiling(on != 0);
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+  int get_profile(int maxn, char* names, double* ms, int64_t* launches, int64_t* elems, int32_t* n) override {
+    *n = be.profile(maxn, names, ms, launches, elems);
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+};
+
+inline int check_grid(const pmhd_grid* g, std::string& err) {
+  if (g->ndim < 1 || g->ndim > 3) { err = "ndim must be 1, 2 or 3"; return PMHD_EINVAL; }
+  if (g->nx < 1 || (g->ndim >= 2 && g->ny < 1) || (g->ndim == 3 && g->nz < 1)) {
+    err = "number of cells must be positive"; return PMHD_EINVAL;
+  }
+  if (g->nx > (1 << 24) || g->ny > (1 << 24) || g->nz > (1 << 24)) {
+    err = "number of cells along an axis must not exceed 2^24"; return PMHD_EINVAL;
+  }
+  if (!(g->dx > 0) || (g->ndim >= 2 && !(g->dy > 0)) || (g->ndim == 3 && !(g->dz > 0))) {
+    err = "cell widths must be positive"; return PMHD_EINVAL;
+  }
+  const int bcs[3] = {g->bcx, g->bcy, g->bcz};
+  for (int d = 0; d < g->ndim; ++d)
+    if (bcs[d] != 0 && bcs[d] != 1) { err = "boundary condition must be 0 (outflow) or 1 (periodic)"; return PMHD_EINVAL; }
+  return PMHD_OK;
+}
+
+// every launch of a single grid fits the HIP grid limits (blocks along y and z
+// below 65536) with the block shape the single-grid executor uses: 256 threads,
+// the smallest power of two >= n2 (at most 64) of them along the contiguous axis
+inline int check_launch_extents(const pmhd_grid* g, std::string& err) {
+  Ctx<double> c;
+  std::memset(&c, 0, sizeof(c));
+  c.ndim = g->ndim;
+  c.ny = g->ndim >= 2 ? g->ny : 1;
+  c.nx = g->nx;
+  c.nz = g->ndim == 3 ? g->nz : 1;
+  for (int kind = 0; kind < 4; ++kind)
+    for (int d = 0; d < 3; ++d) {
+      int64_t n[3];
+      c.extent(kind, d, n);
+      int64_t bx = 1;
+      while (bx < 64 && bx < n[2]) bx <<= 1;
+      const int64_t by = 256 / bx;
+      if (n[0] > 65535 || (n[1] + by - 1) / by > 65535) {
+        err = "grid too large for one launch (cells along y, or along x per block row)";
+        return PMHD_EINVAL;
+      }
+    }
+  return PMHD_OK;
+}
+
+inline int check_params(const pmhd_params* p, std::string& err) {
+  if (p->reconstruct_order < 0 || p->reconstruct_order > 2) {
+    err = "reconstruct_order must be 0, 1 or 2"; return PMHD_EINVAL;
+  }
+  if (p->precision != 64 && p->precision != 32) { err = "precision must be 64 or 32"; return PMHD_EINVAL; }
+  return PMHD_OK;
+}
+
+inline pmhd_grid normal_grid(const pmhd_grid& g) {
+  pmhd_grid gg = g;
+  if (gg.ndim < 3) { gg.nz = 1; gg.bcz = gg.bcx; gg.dz = 1.0; }
+  if (gg.ndim < 2) { gg.ny = 1; gg.bcy = gg.bcx; gg.dy = 1.0; }
+  return gg;
+}
 
 template <class B>
 int create_handle(const pmhd_grid* g, const pmhd_params* p, pmhd_handle** out, std::string& err) {
@@ -273,25 +686,14 @@ int create_handle(const pmhd_grid* g, const pmhd_params* p, pmhd_handle** out, s
     return PMHD_EINVAL;
   }
   *out = nullptr;
-  if (g->ndim < 1 || g->ndim > 3) { err = "ndim must be 1, 2 or 3"; return PMHD_EINVAL; }
-  if (g->nx < 1 || (g->ndim >= 2 && g->ny < 1) || (g->ndim == 3 && g->nz < 1)) {
-    err = "number of cells must be positive"; return PMHD_EINVAL;
-  }
-  if (!(g->dx > 0) || (g->ndim >= 2 && !(g->dy > 0)) || (g->ndim == 3 && !(g->dz > 0))) {
-    err = "cell widths must be positive"; return PMHD_EINVAL;
-  }
-  const int bcs[3] = {g->bcx, g->bcy, g->bcz};
-  for (int d = 0; d < g->ndim; ++d)
-    if (bcs[d] != 0 && bcs[d] != 1) { err = "boundary condition must be 0 (outflow) or 1 (periodic)"; return PMHD_EINVAL; }
-  if (p->reconstruct_order < 0 || p->reconstruct_order > 2) {
-    err = "reconstruct_order must be 0, 1 or 2"; return PMHD_EINVAL;
-  }
-  if (p->precision != 64 && p->precision != 32) { err = "precision must be 64 or 32"; return PMHD_EINVAL; }
-  pmhd_grid gg = *g;
-  if (gg.ndim < 3) { gg.nz = 1; gg.bcz = gg.bcx; gg.dz = 1.0; }
-  if (gg.ndim < 2) { gg.ny = 1; gg.bcy = gg.bcx; gg.dy = 1.0; }
+  int rc = check_grid(g, err);
+  if (rc) return rc;
+  rc = check_launch_extents(g, err);
+  if (rc) return rc;
+  rc = check_params(p, err);
+  if (rc) return rc;
+  const pmhd_grid gg = normal_grid(*g);
   pmhd_handle* h = nullptr;
-  int rc;
   if (p->precision == 64) {
     auto* hh = new (std::nothrow) Handle<double, B>();
     if (!hh) { err = "out of host memory"; return PMHD_ENOMEM; }
@@ -312,6 +714,44 @@ int create_handle(const pmhd_grid* g, const pmhd_params* p, pmhd_handle** out, s
   return PMHD_OK;
 }
 
+template <class B>
+int create_batch(int nbox, const pmhd_grid* gs, const pmhd_params* p, pmhd_batch** out, std::string& err) {
+  if (!gs || !p || !out) {
+    err = "null argument";
+    return PMHD_EINVAL;
+  }
+  *out = nullptr;
+  if (nbox < 1) { err = "a batch needs at least one box"; return PMHD_EINVAL; }
+  int rc = check_params(p, err);
+  if (rc) return rc;
+  std::vector<pmhd_grid> gg(static_cast<size_t>(nbox));
+  for (int b = 0; b < nbox; ++b) {
+    rc = check_grid(&gs[b], err);
+    if (rc) { err = "box " + std::to_string(b) + ": " + err; return rc; }
+    if (gs[b].ndim != gs[0].ndim) { err = "all boxes of a batch must have the same ndim"; return PMHD_EINVAL; }
+    gg[size_t(b)] = normal_grid(gs[b]);
+  }
+  pmhd_batch* h = nullptr;
+  if (p->precision == 64) {
+    auto* hh = new (std::nothrow) BatchHandle<double, B>();
+    if (!hh) { err = "out of host memory"; return PMHD_ENOMEM; }
+    rc = hh->init(nbox, gg.data(), *p);
+    h = hh;
+  } else {
+    auto* hh = new (std::nothrow) BatchHandle<float, B>();
+    if (!hh) { err = "out of host memory"; return PMHD_ENOMEM; }
+    rc = hh->init(nbox, gg.data(), *p);
+    h = hh;
+  }
+  if (rc != PMHD_OK) {
+    err = h->err;
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return PMHD_OK;
+}
+
 inline int advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int64_t max_cycles,
                    double* t_out, int64_t* cycle_out, double* t2_out) {
   double t = t0, t2 = t0;
@@ -321,11 +761,7 @@ inline int advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int6
     double dt;
     int rc = h->compute_dt(&dt);
     if (rc) return rc;
-    if (h->g.ndim == 3) {
-      if (t_end - t < dt) dt = t_end - t;   // Source/EvolveFunctions_MHD.py:851-852
-    } else {
-      if (t + dt > t_end) dt = t_end - t;   // Source/EvolveFunctions_MHD.py:259-260, 472-473
-    }
+    dt = clip_dt(h->g.ndim, t, t_end, dt);
     rc = h->step(dt);
     if (rc) return rc;
     t += dt;
@@ -339,7 +775,7 @@ inline int advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int6
 
 }  // namespace pmhd
 
-// C ABI bodies; the including translation unit defines PMHD_BACKEND.
+// C ABI bodies; the including translation unit names its backend.
 #define PMHD_DEFINE_C_ABI(BACKEND)                                                          \
   static std::string g_pmhd_create_error;                                                   \
   extern "C" int pmhd_create(const pmhd_grid* g, const pmhd_params* p, pmhd_handle** out) { \
@@ -401,4 +837,66 @@ inline int advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int6
   }                                                                                         \
   extern "C" const char* pmhd_last_error(const pmhd_handle* h) {                            \
     return h ? h->err.c_str() : g_pmhd_create_error.c_str();                                \
+  }                                                                                         \
+  extern "C" int pmhd_batch_create(int32_t nbox, const pmhd_grid* grids,                    \
+                                   const pmhd_params* p, pmhd_batch** out) {                \
+    try {                                                                                   \
+      return pmhd::create_batch<BACKEND>(nbox, grids, p, out, g_pmhd_create_error);         \
+    } catch (...) {                                                                         \
+      g_pmhd_create_error = "unexpected exception in pmhd_batch_create";                    \
+      return PMHD_EINVAL;                                                                   \
+    }                                                                                       \
+  }                                                                                         \
+  extern "C" int pmhd_batch_destroy(pmhd_batch* b) {                                        \
+    delete b;                                                                               \
+    return PMHD_OK;                                                                         \
+  }                                                                                         \
+  extern "C" int pmhd_batch_set_state(pmhd_batch* b, int32_t box, const double* prim,       \
+                                      const double* cons, const double* fbx,                \
+                                      const double* fby, const double* fbz) {               \
+    if (!b) return PMHD_EINVAL;                                                             \
+    try { return b->set_state(box, prim, cons, fbx, fby, fbz); }                            \
+    catch (...) { return b->fail(PMHD_EINVAL, "exception in pmhd_batch_set_state"); }       \
+  }                                                                                         \
+  extern "C" int pmhd_batch_get_state(pmhd_batch* b, int32_t box, int32_t which,            \
+                                      double* prim, double* cons, double* fbx,              \
+                                      double* fby, double* fbz) {                           \
+    if (!b) return PMHD_EINVAL;                                                             \
+    try { return b->get_state(box, which, prim, cons, fbx, fby, fbz); }                     \
+    catch (...) { return b->fail(PMHD_EINVAL, "exception in pmhd_batch_get_state"); }       \
+  }                                                                                         \
+  extern "C" int pmhd_batch_advance(pmhd_batch* b, double t0, double t_end, int64_t cycle0, \
+                                    int64_t max_cycles, double* t_out, int64_t* cycle_out,  \
+                                    double* t2_out) {                                       \
+    if (!b) return PMHD_EINVAL;                                                             \
+    try { return b->advance(t0, t_end, cycle0, max_cycles, t_out, cycle_out, t2_out); }     \
+    catch (...) { return b->fail(PMHD_EINVAL, "exception in pmhd_batch_advance"); }         \
+  }                                                                                         \
+  extern "C" int pmhd_batch_autotune(pmhd_batch* b, int32_t reps) {                         \
+    if (!b) return PMHD_EINVAL;                                                             \
+    try { return b->autotune(reps); }                                                       \
+    catch (...) { return b->fail(PMHD_EINVAL, "exception in pmhd_batch_autotune"); }        \
+  }                                                                                         \
+  extern "C" int pmhd_batch_get_tiles(pmhd_batch* b, int32_t max_entries, char* names,      \
+                                      int32_t* bk, int32_t* bi, int32_t* grid, double* ms,  \
+                                      int32_t* n_entries) {                                 \
+    if (!b || !names || !bk || !bi || !grid || !ms || !n_entries) return PMHD_EINVAL;       \
+    return b->get_tiles(max_entries, names, bk, bi, grid, ms, n_entries);                   \
+  }                                                                                         \
+  extern "C" int pmhd_batch_synchronize(pmhd_batch* b) {                                    \
+    if (!b) return PMHD_EINVAL;                                                             \
+    return b->synchronize();                                                                \
+  }                                                                                         \
+  extern "C" int pmhd_batch_set_profiling(pmhd_batch* b, int32_t enable) {                  \
+    if (!b) return PMHD_EINVAL;                                                             \
+    return b->set_profiling(enable);                                                        \
+  }                                                                                         \
+  extern "C" int pmhd_batch_get_profile(pmhd_batch* b, int32_t max_entries, char* names,    \
+                                        double* total_ms, int64_t* launches,                \
+                                        int64_t* elements, int32_t* n_entries) {            \
+    if (!b || !names || !total_ms || !launches || !elements || !n_entries) return PMHD_EINVAL; \
+    return b->get_profile(max_entries, names, total_ms, launches, elements, n_entries);     \
+  }                                                                                         \
+  extern "C" const char* pmhd_batch_last_error(const pmhd_batch* b) {                       \
+    return b ? b->err.c_str() : g_pmhd_create_error.c_str();                                \
   }

@@ -1,5 +1,7 @@
 // HIP backend of libpmhd.so for gfx950 (MI355X): one thread per element, wave64
-// shuffle reductions for the CFL maximum, one HIP stream per handle.
+// shuffle reductions for the CFL maximum, one HIP stream per handle.  A single
+// grid launches each phase over its own index box; a batch launches each phase
+// once over the tiles of every box (a persistent grid walks the tile list).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -11,52 +13,65 @@
 
 namespace pmhd {
 
+// ---------------------------------------------------------------------------
+// single-grid kernels
+// ---------------------------------------------------------------------------
 // one thread per element of an (n0, n1, n2) box: x covers the contiguous axis
-template <class F>
-__global__ void __launch_bounds__(256) k_each3(int n0, int n1, int n2, F f) {
+template <class Ph, class T>
+__global__ void __launch_bounds__(256) k_phase(const Ctx<T> c, int n1, int n2) {
   const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
   const int i = int(blockIdx.y * blockDim.y + threadIdx.y);
   const int j = int(blockIdx.z);
-  if (k < n2 && i < n1) f(I3{j, i, k});
-  (void)n0;
+  if (k < n2 && i < n1) Ph()(c, I3{j, i, k});
 }
 
 __device__ inline unsigned long long umax64(unsigned long long a, unsigned long long b) {
   return a > b ? a : b;
 }
 
-// Maximum of three non-negative per-element values.  The IEEE bit patterns of
-// non-negative doubles order like the values (NaN above +inf, as np.max does), so
-// the reduction is an integer max: deterministic in any order.
-template <class F>
-__global__ void __launch_bounds__(256) k_max3(int64_t n, F f, unsigned long long* out) {
-  unsigned long long m[3] = {0ull, 0ull, 0ull};
-  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-    double o[3];
-    f(i, o);
-#pragma unroll
-    for (int d = 0; d < 3; ++d) m[d] = umax64(m[d], (unsigned long long)__double_as_longlong(o[d]));
-  }
+// maximum of three non-negative values over a 256-thread block, then one atomic
+// per component.  The IEEE bit patterns of non-negative doubles order like the
+// values (NaN above +inf, as np.max propagates it), so the reduction is an
+// integer max: deterministic in any order.
+__device__ inline void block_max3_atomic(unsigned long long m[3], unsigned long long* out) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) m[d] = umax64(m[d], __shfl_xor(m[d], off, 64));
   }
   __shared__ unsigned long long sm[3][4];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int tid = int(threadIdx.y * blockDim.x + threadIdx.x);
+  const int lane = tid & 63, wv = tid >> 6;
   if (lane == 0) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) sm[d][wv] = m[d];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
+    const int nw = int(blockDim.x * blockDim.y + 63) / 64;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      unsigned long long v = umax64(umax64(sm[d][0], sm[d][1]), umax64(sm[d][2], sm[d][3]));
+      unsigned long long v = sm[d][0];
+      for (int w = 1; w < nw; ++w) v = umax64(v, sm[d][w]);
       atomicMax(&out[d], v);
     }
   }
+  __syncthreads();
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_cfl(const Ctx<T> c, int n1, int n2, unsigned long long* out) {
+  const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+  const int i = int(blockIdx.y * blockDim.y + threadIdx.y);
+  const int j = int(blockIdx.z);
+  unsigned long long m[3] = {0ull, 0ull, 0ull};
+  if (k < n2 && i < n1) {
+    double o[3];
+    CflSpeeds()(c, I3{j, i, k}, o);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) m[d] = (unsigned long long)__double_as_longlong(o[d]);
+  }
+  block_max3_atomic(m, out);
 }
 
 // Deterministic two-level reduction of 9 sums and 2 maxima in double: each
@@ -98,6 +113,101 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce(int64_t n, F f, double* 
   }
 }
 
+// ---------------------------------------------------------------------------
+// batch kernels: the tiles of all boxes form one list; box b owns tiles
+// [prefix[b], prefix[b+1]); a tile is bk (along z) x bi (along x) elements of one
+// y row.  The grid walks the list with a stride (persistent when smaller).
+// ---------------------------------------------------------------------------
+struct TileRef {
+  int box, j, i, k;
+  bool in;
+};
+
+template <class T>
+__device__ inline TileRef tile_of(const Ctx<T>* __restrict__ ctxs, const int* __restrict__ prefix, int nbox,
+                                  int vb, int kind, int dir, int bk, int bi) {
+  int lo = 0, hi = nbox;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= vb) lo = mid;
+    else hi = mid;
+  }
+  TileRef r;
+  r.box = lo;
+  int64_t n[3];
+  ctxs[lo].extent(kind, dir, n);
+  const int gk = (int(n[2]) + bk - 1) / bk;
+  const int gi = (int(n[1]) + bi - 1) / bi;
+  const int local = vb - prefix[lo];
+  const int tk = local % gk;
+  const int rest = local / gk;
+  const int ti = rest % gi;
+  r.j = rest / gi;
+  r.k = tk * bk + int(threadIdx.x) % bk;
+  r.i = ti * bi + int(threadIdx.x) / bk;
+  r.in = r.k < n[2] && r.i < n[1];
+  return r;
+}
+
+template <class Ph, class T>
+__global__ void __launch_bounds__(256) k_batch(const Ctx<T>* __restrict__ ctxs, const int* __restrict__ prefix,
+                                               int nbox, int bk, int bi) {
+  const int total = prefix[nbox];
+  for (int vb = int(blockIdx.x); vb < total; vb += int(gridDim.x)) {
+    const TileRef r = tile_of(ctxs, prefix, nbox, vb, Ph::kind, Ph::dir, bk, bi);
+    const Ctx<T>& c = ctxs[r.box];
+    if (!c.active) continue;
+    if (r.in) Ph()(c, I3{r.j, r.i, r.k});
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_batch_cfl(const Ctx<T>* __restrict__ ctxs, const int* __restrict__ prefix,
+                                                   int nbox, int bk, int bi, unsigned long long* red) {
+  const int total = prefix[nbox];
+  for (int vb = int(blockIdx.x); vb < total; vb += int(gridDim.x)) {
+    const TileRef r = tile_of(ctxs, prefix, nbox, vb, EXT_CELL, 0, bk, bi);
+    const Ctx<T>& c = ctxs[r.box];
+    if (!c.active) continue;
+    unsigned long long m[3] = {0ull, 0ull, 0ull};
+    if (r.in) {
+      double o[3];
+      CflSpeeds()(c, I3{r.j, r.i, r.k}, o);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) m[d] = (unsigned long long)__double_as_longlong(o[d]);
+    }
+    block_max3_atomic(m, red + 3 * r.box);
+  }
+}
+
+template <class T>
+__global__ void k_box_begin(Ctx<T>* ctxs, BoxClock* clk, unsigned long long* red, int nbox, double cfl,
+                            double t_end, int64_t max_cycles, int commit) {
+  const int b = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (b >= nbox) return;
+  double m[3];
+  for (int d = 0; d < 3; ++d) {
+    m[d] = __longlong_as_double((long long)red[3 * b + d]);
+    red[3 * b + d] = 0ull;
+  }
+  box_begin(ctxs[b], clk[b], m, cfl, t_end, max_cycles, commit);
+}
+
+template <class T>
+__global__ void k_box_end(Ctx<T>* ctxs, BoxClock* clk, int nbox) {
+  const int b = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (b < nbox) box_end(ctxs[b], clk[b]);
+}
+
+template <class T>
+__global__ void k_box_activate(Ctx<T>* ctxs, int nbox) {
+  const int b = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (b < nbox) ctxs[b].active = 1;
+}
+
+// ---------------------------------------------------------------------------
+// executors
+// ---------------------------------------------------------------------------
 struct HipExec {
   hipStream_t s = nullptr;
   unsigned long long* red = nullptr;
@@ -170,17 +280,42 @@ struct HipExec {
     pend.push_back(Rec{id, n, a, b});
   }
 
-  template <class F>
-  void run3(int id, int64_t n0, int64_t n1, int64_t n2, F f) {
-    if (n0 <= 0 || n1 <= 0 || n2 <= 0) return;
+  static void grid_of(const int64_t n[3], dim3& grid, dim3& block) {
     unsigned bx = 1;
-    while (bx < 64 && int64_t(bx) < n2) bx <<= 1;
+    while (bx < 64 && int64_t(bx) < n[2]) bx <<= 1;
     const unsigned by = 256 / bx;
-    dim3 grid(unsigned((n2 + bx - 1) / bx), unsigned((n1 + by - 1) / by), unsigned(n0));
-    timed(id, n0 * n1 * n2, [&] {
-      hipLaunchKernelGGL(k_each3<F>, grid, dim3(bx, by, 1), 0, s, int(n0), int(n1), int(n2), f);
+    grid = dim3(unsigned((n[2] + bx - 1) / bx), unsigned((n[1] + by - 1) / by), unsigned(n[0]));
+    block = dim3(bx, by, 1);
+  }
+
+  template <class Ph, class T>
+  void run(const Ctx<T>& c) {
+    int64_t n[3];
+    c.extent(Ph::kind, Ph::dir, n);
+    if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return;
+    dim3 grid, block;
+    grid_of(n, grid, block);
+    timed(Ph::id, n[0] * n[1] * n[2], [&] {
+      hipLaunchKernelGGL((k_phase<Ph, T>), grid, block, 0, s, c, int(n[1]), int(n[2]));
     });
     note(hipGetLastError());
+  }
+
+  template <class T>
+  void cfl(const Ctx<T>& c, double out[3]) {
+    note(hipMemsetAsync(red, 0, 3 * sizeof(unsigned long long), s));
+    int64_t n[3];
+    c.extent(EXT_CELL, 0, n);
+    dim3 grid, block;
+    grid_of(n, grid, block);
+    timed(K_CFL, n[0] * n[1] * n[2], [&] {
+      hipLaunchKernelGGL((k_cfl<T>), grid, block, 0, s, c, int(n[1]), int(n[2]), red);
+    });
+    note(hipGetLastError());
+    unsigned long long h[3] = {0, 0, 0};
+    note(hipMemcpyAsync(h, red, sizeof(h), hipMemcpyDeviceToHost, s));
+    note(hipStreamSynchronize(s));
+    for (int d = 0; d < 3; ++d) std::memcpy(&out[d], &h[d], sizeof(double));
   }
 
   double* part = nullptr;
@@ -203,30 +338,136 @@ struct HipExec {
       maxes[1] = npmax(maxes[1], h[size_t(b * 11 + 10)]);
     }
   }
+};
 
-  template <class F>
-  void max3(int id, int64_t n, F f, double out[3]) {
-    note(hipMemsetAsync(red, 0, 3 * sizeof(unsigned long long), s));
-    const int tb = 256;
-    int64_t nb = (n + tb - 1) / tb;
-    if (nb > 8192) nb = 8192;
-    if (nb < 1) nb = 1;
-    timed(id, n, [&] { hipLaunchKernelGGL(k_max3<F>, dim3(unsigned(nb)), dim3(tb), 0, s, n, f, red); });
-    note(hipGetLastError());
-    unsigned long long h[3] = {0, 0, 0};
-    note(hipMemcpyAsync(h, red, sizeof(h), hipMemcpyDeviceToHost, s));
-    note(hipStreamSynchronize(s));
-    for (int d = 0; d < 3; ++d) std::memcpy(&out[d], &h[d], sizeof(double));
+// one launch per phase over every box of a batch
+struct HipBatchExec {
+  HipExec* ex = nullptr;
+  BatchLayout* lay = nullptr;    // device contexts, clocks, tile tables
+  // when set, run() times every tile candidate of each phase and keeps the best
+  bool tuning = false;
+  int tune_reps = 3;
+
+  void launch_cfg(int id, int& bk, int& bi, int& grid) const {
+    bk = lay->tiles[id].bk;
+    bi = lay->tiles[id].bi;
+    const int total = lay->total[id];
+    grid = lay->tiles[id].grid > 0 && lay->tiles[id].grid < total ? lay->tiles[id].grid : total;
+    if (grid < 1) grid = 1;
+  }
+
+  template <class Ph, class T>
+  void launch(int bk, int bi, int grid) {
+    const Ctx<T>* dctx = static_cast<const Ctx<T>*>(lay->dctx);
+    const int* prefix = lay->dprefix + Ph::id * (lay->nbox + 1);
+    ex->timed(Ph::id, lay->elems[Ph::id], [&] {
+      hipLaunchKernelGGL((k_batch<Ph, T>), dim3(unsigned(grid)), dim3(unsigned(bk * bi)), 0, ex->s, dctx, prefix,
+                         lay->nbox, bk, bi);
+    });
+    ex->note(hipGetLastError());
+  }
+
+  template <class Ph, class T>
+  void run(const Ctx<T>&) {
+    if (tuning) tune<Ph, T>();
+    int bk, bi, grid;
+    launch_cfg(Ph::id, bk, bi, grid);
+    launch<Ph, T>(bk, bi, grid);
+  }
+
+  // time each candidate tile (and grid) of launch id and keep the fastest;
+  // launch_with(bk, bi, grid) issues one launch
+  template <class L>
+  void tune_id(int id, L&& launch_with) {
+    const std::vector<TileCfg> cands = tile_candidates(*lay, id);
+    hipEvent_t a = ex->event(), b = ex->event();
+    float best = -1.f;
+    TileCfg keep = lay->tiles[id];
+    for (const TileCfg& t : cands) {
+      if (!lay->apply_tile(id, t)) continue;
+      int bk, bi, grid;
+      launch_cfg(id, bk, bi, grid);
+      launch_with(bk, bi, grid);   // warm
+      ex->note(hipEventRecord(a, ex->s));
+      for (int r = 0; r < tune_reps; ++r) launch_with(bk, bi, grid);
+      ex->note(hipEventRecord(b, ex->s));
+      ex->note(hipEventSynchronize(b));
+      float ms = 0.f;
+      ex->note(hipEventElapsedTime(&ms, a, b));
+      if (best < 0.f || ms < best) {
+        best = ms;
+        keep = t;
+      }
+    }
+    (void)lay->apply_tile(id, keep);
+    lay->tuned_ms[id] = best / float(tune_reps);
+    ex->pool.push_back(a);
+    ex->pool.push_back(b);
+  }
+
+  template <class Ph, class T>
+  void tune() {
+    tune_id(Ph::id, [&](int bk, int bi, int grid) { launch<Ph, T>(bk, bi, grid); });
+  }
+
+  template <class T>
+  void cfl_launch(int bk, int bi, int grid) {
+    const Ctx<T>* dctx = static_cast<const Ctx<T>*>(lay->dctx);
+    const int* prefix = lay->dprefix + K_CFL * (lay->nbox + 1);
+    ex->timed(K_CFL, lay->elems[K_CFL], [&] {
+      hipLaunchKernelGGL((k_batch_cfl<T>), dim3(unsigned(grid)), dim3(unsigned(bk * bi)), 0, ex->s, dctx, prefix,
+                         lay->nbox, bk, bi, lay->dred);
+    });
+    ex->note(hipGetLastError());
+  }
+
+  // per-box CFL maxima into lay->dred (3 per box; cleared by begin())
+  template <class T>
+  void cfl(const Ctx<T>&) {
+    if (tuning) tune_id(K_CFL, [&](int bk, int bi, int grid) { cfl_launch<T>(bk, bi, grid); });
+    int bk, bi, grid;
+    launch_cfg(K_CFL, bk, bi, grid);
+    cfl_launch<T>(bk, bi, grid);
+  }
+
+  template <class T>
+  void begin(double cfl, double t_end, int64_t max_cycles, int commit) {
+    const unsigned nb = unsigned((lay->nbox + 63) / 64);
+    hipLaunchKernelGGL((k_box_begin<T>), dim3(nb), dim3(64), 0, ex->s, static_cast<Ctx<T>*>(lay->dctx), lay->dclk,
+                       lay->dred, lay->nbox, cfl, t_end, max_cycles, commit);
+    ex->note(hipGetLastError());
+  }
+  template <class T>
+  void end() {
+    const unsigned nb = unsigned((lay->nbox + 63) / 64);
+    hipLaunchKernelGGL((k_box_end<T>), dim3(nb), dim3(64), 0, ex->s, static_cast<Ctx<T>*>(lay->dctx), lay->dclk,
+                       lay->nbox);
+    ex->note(hipGetLastError());
+  }
+  template <class T>
+  void activate_all() {
+    const unsigned nb = unsigned((lay->nbox + 63) / 64);
+    hipLaunchKernelGGL((k_box_activate<T>), dim3(nb), dim3(64), 0, ex->s, static_cast<Ctx<T>*>(lay->dctx),
+                       lay->nbox);
+    ex->note(hipGetLastError());
   }
 };
 
 struct HipBackend {
   HipExec ex;
+  HipBatchExec bex;
   std::string msg;
+  int cus = 256;
 
   HipBackend() {
     ex.note(hipStreamCreate(&ex.s));
     ex.note(hipMalloc(reinterpret_cast<void**>(&ex.red), 3 * sizeof(unsigned long long)));
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      int n = 0;
+      if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0) cus = n;
+    }
+    bex.ex = &ex;
   }
   ~HipBackend() {
     ex.destroy_events();
@@ -236,6 +477,11 @@ struct HipBackend {
     if (ex.s) hipStreamDestroy(ex.s);
   }
   HipExec& exec() { return ex; }
+  HipBatchExec& batch_exec(BatchLayout* lay) {
+    bex.lay = lay;
+    return bex;
+  }
+  int compute_units() const { return cus; }
   void* alloc(size_t bytes) {
     void* ptr = nullptr;
     if (hipMalloc(&ptr, bytes) != hipSuccess) {
@@ -251,6 +497,10 @@ struct HipBackend {
   }
   void d2h(void* dst, const void* src, size_t bytes) {
     ex.note(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ex.s));
+    ex.note(hipStreamSynchronize(ex.s));
+  }
+  void zero(void* dst, size_t bytes) {
+    ex.note(hipMemsetAsync(dst, 0, bytes, ex.s));
     ex.note(hipStreamSynchronize(ex.s));
   }
   void sync() { ex.note(hipStreamSynchronize(ex.s)); }

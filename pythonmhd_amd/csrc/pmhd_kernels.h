@@ -13,6 +13,8 @@
 namespace pmhd {
 
 enum { OUTFLOW = 0, PERIODIC = 1 };
+// element kinds a launch runs over
+enum ExtKind { EXT_CELL = 0, EXT_FACE = 1, EXT_SLOPE = 2, EXT_EDGE = 3 };
 
 // launch identifiers (for per-kernel timing); "+ d" for the per-direction ones
 enum KernelId {
@@ -28,10 +30,26 @@ inline const char* kernel_name(int id) {
       "ct_full_x", "ct_full_y", "ct_full_z", "update", "cfl", "diag"};
   return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
+// element kind (ExtKind) and direction of each launch id
+inline void phase_shape(int id, int& kind, int& dir) {
+  dir = 0;
+  kind = EXT_CELL;
+  if (id < K_EDGES) { kind = EXT_SLOPE; dir = id - K_SLOPES; }
+  else if (id < K_COPY) { dir = id - K_EDGES; }
+  else if (id < K_PREDICT) { kind = EXT_FACE; dir = id - K_COPY; }
+  else if (id < K_CELL_E) { kind = EXT_FACE; dir = id - K_PREDICT; }
+  else if (id == K_EMF || id == K_EMF2) { kind = EXT_EDGE; }
+  else if (id >= K_CT_HALF && id < K_HALF_E) { kind = EXT_FACE; dir = id - K_CT_HALF; }
+  else if (id >= K_CORRECT && id < K_UPDATE) {
+    kind = EXT_FACE;
+    dir = id < K_CT_FULL ? id - K_CORRECT : id - K_CT_FULL;
+  }
+}
 
 template <class T>
 struct Ctx {
   int ndim;
+  int active;        // 0: a batch box that has finished (its launches are skipped)
   int64_t ny, nx, nz;
   int bc[3];         // boundary condition per direction (x, y, z)
   int bc_zrec;       // boundary condition the z reconstruction/corrections use
@@ -64,7 +82,6 @@ struct Ctx {
   T* UR[3];
   T* F[3];           // predictor fluxes
   T* Fc[3];          // corrector fluxes
-  const T* Fe[3];    // fluxes the corner-EMF launch reads (predictor or corrector)
   T* Ec;
   T* E[3];
   // element counts
@@ -88,6 +105,18 @@ struct Ctx {
   PMHD_FI int64_t snx(int d) const { return nx + 2 * (d == 0); }
   PMHD_FI int64_t snz(int d) const { return nz + 2 * (d == 2); }
   PMHD_FI int64_t nslope(int d) const { return sny(d) * snx(d) * snz(d); }
+  // index box (n0, n1, n2) = (y, x, z) extents of one launch kind along d
+  PMHD_FI void extent(int kind, int d, int64_t n[3]) const {
+    if (kind == EXT_FACE) {
+      n[0] = fny(d); n[1] = fnx(d); n[2] = fnz(d);
+    } else if (kind == EXT_SLOPE) {
+      n[0] = sny(d); n[1] = snx(d); n[2] = snz(d);
+    } else if (kind == EXT_EDGE) {
+      n[0] = eny(); n[1] = enx(); n[2] = enz();
+    } else {
+      n[0] = ny; n[1] = nx; n[2] = nz;
+    }
+  }
 };
 
 struct I3 {
@@ -174,9 +203,9 @@ PMHD_FI void store7(T* A, int64_t NF, int64_t f, const T v[7]) {
 // CFL speeds of one cell
 // ===========================================================================
 template <class T>
-PMHD_FI void k_cfl(const Ctx<T>& c, int64_t idx, T out[3]) {
+PMHD_FI void k_cfl(const Ctx<T>& c, I3 p, T out[3]) {
   const int64_t N = c.ncell();
-  I3 p = unflat(idx, c.ny, c.nx, c.nz);
+  const int64_t idx = c.cid(p.j, p.i, p.k);
   T u[8];
 #pragma unroll
   for (int v = 0; v < 8; ++v) u[v] = c.U[v * N + idx];
@@ -443,22 +472,23 @@ PMHD_FI void k_half_e(const Ctx<T>& c, I3 p) {
 // corner EMFs (Source/CornerEMF_HelperModule.py:440-952)
 // ===========================================================================
 template <class T>
-PMHD_FI T fluxv(const Ctx<T>& c, int d, int v, int64_t j, int64_t i, int64_t k) {
-  return c.Fe[d][v * c.nface(d) + c.fid(d, j, i, k)];
+PMHD_FI T fluxv(const Ctx<T>& c, T* const* Fs, int d, int v, int64_t j, int64_t i, int64_t k) {
+  return Fs[d][v * c.nface(d) + c.fid(d, j, i, k)];
 }
 
+// Fs: the flux set (predictor c.F or corrector c.Fc) the EMFs are formed from
 template <class T>
-PMHD_FI void k_emf(const Ctx<T>& c, I3 p) {
+PMHD_FI void k_emf(const Ctx<T>& c, T* const* Fs, I3 p) {
   const int64_t N = c.ncell();
   const int64_t idx = c.eid(p.j, p.i, p.k);
   const int bx = c.bc[0], by = c.bc[1], bz = c.bc[2];
   const int64_t xl = edge_lo(p.i, c.nx, bx), xr = edge_hi(p.i, c.nx, bx);
   const int64_t yl = edge_lo(p.j, c.ny, by), yr = edge_hi(p.j, c.ny, by);
   if (c.ndim == 2) {
-    T t0 = fluxv(c, 1, 0, p.j, xr, 0), t5 = fluxv(c, 1, 5, p.j, xr, 0);
-    T lt0 = fluxv(c, 1, 0, p.j, xl, 0), lt5 = fluxv(c, 1, 5, p.j, xl, 0);
-    T lf0 = fluxv(c, 0, 0, yr, p.i, 0), lf4 = fluxv(c, 0, 4, yr, p.i, 0);
-    T ul0 = fluxv(c, 0, 0, yl, p.i, 0), ul4 = fluxv(c, 0, 4, yl, p.i, 0);
+    T t0 = fluxv(c, Fs, 1, 0, p.j, xr, 0), t5 = fluxv(c, Fs, 1, 5, p.j, xr, 0);
+    T lt0 = fluxv(c, Fs, 1, 0, p.j, xl, 0), lt5 = fluxv(c, Fs, 1, 5, p.j, xl, 0);
+    T lf0 = fluxv(c, Fs, 0, 0, yr, p.i, 0), lf4 = fluxv(c, Fs, 0, 4, yr, p.i, 0);
+    T ul0 = fluxv(c, Fs, 0, 0, yl, p.i, 0), ul4 = fluxv(c, Fs, 0, 4, yl, p.i, 0);
     T cE = c.Ec[c.cid(yr, xr, 0)], lE = c.Ec[c.cid(yr, xl, 0)];
     T uE = c.Ec[c.cid(yl, xr, 0)], ulE = c.Ec[c.cid(yl, xl, 0)];
     T x1 = upwind(lt0, lt5 - ul4 - ulE, lt5 - lf4 - lE, lt5 - T(0.5) * (lf4 + ul4 + lE + ulE));
@@ -473,10 +503,10 @@ PMHD_FI void k_emf(const Ctx<T>& c, I3 p) {
   const T* Eyc = c.Ec + N;
   const T* Ezc = c.Ec + 2 * N;
   {  // E_x (getXCornerEMF_3D): faces y (G) and z (H)
-    T b0 = fluxv(c, 2, 0, yr, xr, p.k), b5 = fluxv(c, 2, 5, yr, xr, p.k);
-    T ub0 = fluxv(c, 2, 0, yl, xr, p.k), ub5 = fluxv(c, 2, 5, yl, xr, p.k);
-    T t0 = fluxv(c, 1, 0, p.j, xr, zr), t4 = fluxv(c, 1, 4, p.j, xr, zr);
-    T bt0 = fluxv(c, 1, 0, p.j, xr, zl), bt4 = fluxv(c, 1, 4, p.j, xr, zl);
+    T b0 = fluxv(c, Fs, 2, 0, yr, xr, p.k), b5 = fluxv(c, Fs, 2, 5, yr, xr, p.k);
+    T ub0 = fluxv(c, Fs, 2, 0, yl, xr, p.k), ub5 = fluxv(c, Fs, 2, 5, yl, xr, p.k);
+    T t0 = fluxv(c, Fs, 1, 0, p.j, xr, zr), t4 = fluxv(c, Fs, 1, 4, p.j, xr, zr);
+    T bt0 = fluxv(c, Fs, 1, 0, p.j, xr, zl), bt4 = fluxv(c, Fs, 1, 4, p.j, xr, zl);
     T cE = Exc[c.cid(yr, xr, zr)], bE = Exc[c.cid(yr, xr, zl)];
     T uE = Exc[c.cid(yl, xr, zr)], ubE = Exc[c.cid(yl, xr, zl)];
     T z1 = upwind(bt0, ub5 - ubE, b5 - bE, T(0.5) * (ub5 + b5 - ubE - bE));
@@ -486,10 +516,10 @@ PMHD_FI void k_emf(const Ctx<T>& c, I3 p) {
     c.E[0][idx] = T(0.25) * (b5 + ub5 - t4 - bt4 + y1 + y2 + z1 + z2);
   }
   {  // E_y (getYCornerEMF_3D): faces x (F) and z (H)
-    T b0 = fluxv(c, 2, 0, yr, xr, p.k), b4 = fluxv(c, 2, 4, yr, xr, p.k);
-    T lb0 = fluxv(c, 2, 0, yr, xl, p.k), lb4 = fluxv(c, 2, 4, yr, xl, p.k);
-    T lf0 = fluxv(c, 0, 0, yr, p.i, zr), lf5 = fluxv(c, 0, 5, yr, p.i, zr);
-    T blf0 = fluxv(c, 0, 0, yr, p.i, zl), blf5 = fluxv(c, 0, 5, yr, p.i, zl);
+    T b0 = fluxv(c, Fs, 2, 0, yr, xr, p.k), b4 = fluxv(c, Fs, 2, 4, yr, xr, p.k);
+    T lb0 = fluxv(c, Fs, 2, 0, yr, xl, p.k), lb4 = fluxv(c, Fs, 2, 4, yr, xl, p.k);
+    T lf0 = fluxv(c, Fs, 0, 0, yr, p.i, zr), lf5 = fluxv(c, Fs, 0, 5, yr, p.i, zr);
+    T blf0 = fluxv(c, Fs, 0, 0, yr, p.i, zl), blf5 = fluxv(c, Fs, 0, 5, yr, p.i, zl);
     T cE = Eyc[c.cid(yr, xr, zr)], bE = Eyc[c.cid(yr, xr, zl)];
     T lE = Eyc[c.cid(yr, xl, zr)], blE = Eyc[c.cid(yr, xl, zl)];
     T z1 = upwind(blf0, -lb4 - blE, -b4 - bE, T(0.5) * (-lb4 - b4 - blE - bE));
@@ -499,10 +529,10 @@ PMHD_FI void k_emf(const Ctx<T>& c, I3 p) {
     c.E[1][idx] = T(0.25) * (lf5 + blf5 - b4 - lb4 + x1 + x2 + z1 + z2);
   }
   {  // E_z (getZCornerEMF_3D): faces x (F) and y (G)
-    T t0 = fluxv(c, 1, 0, p.j, xr, zr), t5 = fluxv(c, 1, 5, p.j, xr, zr);
-    T lt0 = fluxv(c, 1, 0, p.j, xl, zr), lt5 = fluxv(c, 1, 5, p.j, xl, zr);
-    T lf0 = fluxv(c, 0, 0, yr, p.i, zr), lf4 = fluxv(c, 0, 4, yr, p.i, zr);
-    T ul0 = fluxv(c, 0, 0, yl, p.i, zr), ul4 = fluxv(c, 0, 4, yl, p.i, zr);
+    T t0 = fluxv(c, Fs, 1, 0, p.j, xr, zr), t5 = fluxv(c, Fs, 1, 5, p.j, xr, zr);
+    T lt0 = fluxv(c, Fs, 1, 0, p.j, xl, zr), lt5 = fluxv(c, Fs, 1, 5, p.j, xl, zr);
+    T lf0 = fluxv(c, Fs, 0, 0, yr, p.i, zr), lf4 = fluxv(c, Fs, 0, 4, yr, p.i, zr);
+    T ul0 = fluxv(c, Fs, 0, 0, yl, p.i, zr), ul4 = fluxv(c, Fs, 0, 4, yl, p.i, zr);
     T cE = Ezc[c.cid(yr, xr, zr)], lE = Ezc[c.cid(yr, xl, zr)];
     T uE = Ezc[c.cid(yl, xr, zr)], ulE = Ezc[c.cid(yl, xl, zr)];
     T x1 = upwind(lt0, -ul4 - ulE, -lf4 - lE, T(-0.5) * (lf4 + ul4 + lE + ulE));
@@ -614,7 +644,9 @@ PMHD_FI void ctu_states(const Ctx<T>& c, int d, I3 f, T ul[7], T ur[7]) {
     int64_t layer_l = e == 0 ? (bc == PERIODIC ? n - 1 : 0) : e - 1;
     int64_t layer_r = e;
     // y-outflow: the bottom ghost of the high x-face flux sits in row nx-1
-    int64_t hov = (d == 1 && bc == OUTFLOW && e == n) ? c.nx - 1 : -1;
+    // (Source/CTU_HelperModule.py:724); where that row does not exist (nx > ny, the
+    // reference raises IndexError) the last row ny-1 is used
+    int64_t hov = (d == 1 && bc == OUTFLOW && e == n) ? (c.nx < c.ny ? c.nx - 1 : c.ny - 1) : -1;
     for (int s = 0; s < 2; ++s) {
       int tc = s == 0 ? first : second;
       ctu_apply_3d(c, d, tc, pl, -1, f, layer_l, ul);
@@ -643,7 +675,8 @@ PMHD_FI void ctu_states(const Ctx<T>& c, int d, I3 f, T ul[7], T ur[7]) {
         flo = ee < 0 ? (bc == OUTFLOW ? 0 : n - 1) : ee;
         fhi = ee + 1 > n ? (bc == OUTFLOW ? n : 1) : ee + 1;
       } else if (bc == OUTFLOW) {
-        cm = ee < 0 ? 1 : (ee >= n ? n - 1 : ee);
+        // the reference's top ghost is cell row 1 (row 0 when ny == 1, where it raises)
+        cm = ee < 0 ? (n > 1 ? 1 : 0) : (ee >= n ? n - 1 : ee);
         flo = ee < 0 ? 1 : ee;
         fhi = ee + 1 > n ? n - 1 : ee + 1;
       } else {

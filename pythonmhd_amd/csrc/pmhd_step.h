@@ -1,25 +1,107 @@
-// One CTU+CT cycle as a sequence of launches over an executor X that provides
-//   X::run3(id, n0, n1, n2, f)    -- f(I3{j, i, k}) over a (n0, n1, n2) index box
-//   X::max3(id, n, f, out)        -- out[d] = max over i in [0, n) of f(i)[d]
+// One CTU+CT cycle as a sequence of named phases.  A phase is a functor type
+// with a launch id, the kind of element it runs over (cells, faces, slope cells or
+// edge points) and a direction; an executor X runs it over every element:
+//   X::run<Ph>(c)              -- Ph()(c, I3{j, i, k}) over the phase's index box
+//   X::cfl(c, out)             -- out[d] = max over cells of the CFL speed along d
 //   X::reduce(id, n, f, sums, maxes)
-// The direction of each sweep is a template parameter so the rotations and index
-// arithmetic of the kernel bodies fold at compile time.
+// The single-grid executors take c as the grid; the batch executors run the same
+// phase over every box of a batch, each with its own context.  The direction is a
+// template parameter so the rotations and index arithmetic fold at compile time.
 // (Source/EvolveFunctions_MHD.py:252-317 for 1D, :464-692 for 2D, :840-1152 for 3D)
 #pragma once
 #include "pmhd_kernels.h"
 
 namespace pmhd {
 
-template <class T, class X>
-void cfl_maxima(X& ex, const Ctx<T>& c, double out[3]) {
-  ex.max3(K_CFL, c.ncell(), [=] PMHD_HD(int64_t i, double o[3]) {
+// ---------------------------------------------------------------------------
+// phases
+// ---------------------------------------------------------------------------
+#define PMHD_PHASE_TRAITS(ID, KIND, DIR) \
+  static constexpr int id = (ID);        \
+  static constexpr int kind = (KIND);    \
+  static constexpr int dir = (DIR);
+
+template <int D>
+struct Slopes {
+  PMHD_PHASE_TRAITS(K_SLOPES + D, EXT_SLOPE, D)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_slopes(c, D, c.bc[D], p); }
+};
+template <int D>
+struct Edges {
+  PMHD_PHASE_TRAITS(K_EDGES + D, EXT_CELL, D)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_edges(c, D, c.bc[D], p); }
+};
+template <int D>
+struct CopyStates {
+  PMHD_PHASE_TRAITS(K_COPY + D, EXT_FACE, D)
+  // the reference's 3D first-order z states use the x boundary condition
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const {
+    k_copy_states(c, D, D == 2 ? c.bc_zrec : c.bc[D], p);
+  }
+};
+template <int D>
+struct Predict {
+  PMHD_PHASE_TRAITS(K_PREDICT + D, EXT_FACE, D)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_predict(c, D, p); }
+};
+struct CellE {
+  PMHD_PHASE_TRAITS(K_CELL_E, EXT_CELL, 0)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_cell_e(c, p); }
+};
+struct EmfPredictor {
+  PMHD_PHASE_TRAITS(K_EMF, EXT_EDGE, 0)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_emf(c, c.F, p); }
+};
+struct EmfCorrector {
+  PMHD_PHASE_TRAITS(K_EMF2, EXT_EDGE, 0)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_emf(c, c.Fc, p); }
+};
+template <int D>
+struct CtHalf {  // half-step face field
+  PMHD_PHASE_TRAITS(K_CT_HALF + D, EXT_FACE, D)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_ct(c, D, c.fb[D], c.nfb[D], c.hdtd, p); }
+};
+struct HalfE {
+  PMHD_PHASE_TRAITS(K_HALF_E, EXT_CELL, 0)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_half_e(c, p); }
+};
+template <int D>
+struct Correct {  // CTU transverse corrections + corrector flux
+  PMHD_PHASE_TRAITS(K_CORRECT + D, EXT_FACE, D)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_correct(c, D, p); }
+};
+template <int D>
+struct CtFull {  // full-step face field
+  PMHD_PHASE_TRAITS(K_CT_FULL + D, EXT_FACE, D)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_ct(c, D, c.fb[D], c.fb2[D], c.dtd, p); }
+};
+struct Update {
+  PMHD_PHASE_TRAITS(K_UPDATE, EXT_CELL, 0)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_update(c, p); }
+};
+struct CflSpeeds {  // per-cell speeds, maximised by X::cfl
+  PMHD_PHASE_TRAITS(K_CFL, EXT_CELL, 0)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p, double o[3]) const {
     T t[3];
-    k_cfl(c, i, t);
+    k_cfl(c, p, t);
     o[0] = double(t[0]);
     o[1] = double(t[1]);
     o[2] = double(t[2]);
-  }, out);
-}
+  }
+};
 
 // FP64 diagnostics: sums[0..7] conserved totals, sums[8] non-finite count,
 // maxes[0] max |div B| * min width, maxes[1] max |B| on faces
@@ -32,43 +114,23 @@ void diagnostics(X& ex, const Ctx<T>& c, double sums[9], double maxes[2]) {
 // interface states and predictor fluxes of direction D
 template <int D, class T, class X>
 void predictor_sweep(X& ex, const Ctx<T>& c) {
-  const int bc = c.bc[D];
   if (c.order > 0) {
-    ex.run3(K_SLOPES + D, c.sny(D), c.snx(D), c.snz(D), [=] PMHD_HD(I3 p) { k_slopes(c, D, bc, p); });
-    ex.run3(K_EDGES + D, c.ny, c.nx, c.nz, [=] PMHD_HD(I3 p) { k_edges(c, D, bc, p); });
+    ex.template run<Slopes<D>>(c);
+    ex.template run<Edges<D>>(c);
   } else {
-    // the reference's 3D first-order z states use the x boundary condition
-    const int bcq = D == 2 ? c.bc_zrec : bc;
-    ex.run3(K_COPY + D, c.fny(D), c.fnx(D), c.fnz(D), [=] PMHD_HD(I3 p) { k_copy_states(c, D, bcq, p); });
+    ex.template run<CopyStates<D>>(c);
   }
-  ex.run3(K_PREDICT + D, c.fny(D), c.fnx(D), c.fnz(D), [=] PMHD_HD(I3 p) { k_predict(c, D, p); });
+  ex.template run<Predict<D>>(c);
 }
 
-template <int D, class T, class X>
-void face_pass(X& ex, const Ctx<T>& c, int which) {
-  const int64_t n0 = c.fny(D), n1 = c.fnx(D), n2 = c.fnz(D);
-  if (which == 1) {
-    ex.run3(K_CT_HALF + D, n0, n1, n2, [=] PMHD_HD(I3 p) {
-      T h[3] = {c.hdtd[0], c.hdtd[1], c.hdtd[2]};
-      k_ct(c, D, c.fb[D], c.nfb[D], h, p);
-    });
-  } else if (which == 2) {
-    ex.run3(K_CORRECT + D, n0, n1, n2, [=] PMHD_HD(I3 p) { k_correct(c, D, p); });
-  } else {
-    ex.run3(K_CT_FULL + D, n0, n1, n2, [=] PMHD_HD(I3 p) {
-      T h[3] = {c.dtd[0], c.dtd[1], c.dtd[2]};
-      k_ct(c, D, c.fb[D], c.fb2[D], h, p);
-    });
-  }
+template <template <int> class Ph, class T, class X>
+void each_dir(X& ex, const Ctx<T>& c) {
+  ex.template run<Ph<0>>(c);
+  ex.template run<Ph<1>>(c);
+  if (c.ndim == 3) ex.template run<Ph<2>>(c);
 }
 
-template <class T, class X>
-void face_passes(X& ex, const Ctx<T>& c, int which) {
-  face_pass<0>(ex, c, which);
-  face_pass<1>(ex, c, which);
-  if (c.ndim == 3) face_pass<2>(ex, c, which);
-}
-
+// c supplies the launch-invariant decisions (ndim, order), which a batch shares
 template <class T, class X>
 void cycle(X& ex, const Ctx<T>& c) {
   const int nd = c.ndim;
@@ -76,21 +138,119 @@ void cycle(X& ex, const Ctx<T>& c) {
   if (nd >= 2) predictor_sweep<1>(ex, c);
   if (nd == 3) predictor_sweep<2>(ex, c);
   if (nd == 1) {
-    ex.run3(K_UPDATE, c.ny, c.nx, c.nz, [=] PMHD_HD(I3 p) { k_update(c, p); });
+    ex.template run<Update>(c);
     return;
   }
-  ex.run3(K_CELL_E, c.ny, c.nx, c.nz, [=] PMHD_HD(I3 p) { k_cell_e(c, p); });
-  Ctx<T> cp = c;           // corner EMFs of the predictor fluxes
-  for (int d = 0; d < 3; ++d) cp.Fe[d] = c.F[d];
-  ex.run3(K_EMF, c.eny(), c.enx(), c.enz(), [=] PMHD_HD(I3 p) { k_emf(cp, p); });
-  face_passes(ex, c, 1);   // half-step face field
-  ex.run3(K_HALF_E, c.ny, c.nx, c.nz, [=] PMHD_HD(I3 p) { k_half_e(c, p); });
-  face_passes(ex, c, 2);   // CTU transverse corrections + corrector fluxes
-  Ctx<T> cc = c;           // corner EMFs of the corrector fluxes
-  for (int d = 0; d < 3; ++d) cc.Fe[d] = c.Fc[d];
-  ex.run3(K_EMF2, c.eny(), c.enx(), c.enz(), [=] PMHD_HD(I3 p) { k_emf(cc, p); });
-  face_passes(ex, c, 3);   // full-step face field
-  ex.run3(K_UPDATE, c.ny, c.nx, c.nz, [=] PMHD_HD(I3 p) { k_update(c, p); });
+  ex.template run<CellE>(c);
+  ex.template run<EmfPredictor>(c);   // corner EMFs of the predictor fluxes
+  each_dir<CtHalf>(ex, c);            // half-step face field
+  ex.template run<HalfE>(c);
+  each_dir<Correct>(ex, c);           // CTU transverse corrections + corrector fluxes
+  ex.template run<EmfCorrector>(c);   // corner EMFs of the corrector fluxes
+  each_dir<CtFull>(ex, c);            // full-step face field
+  ex.template run<Update>(c);
+}
+
+// ---------------------------------------------------------------------------
+// timestep (Source/TimestepCalculations_HelperModule.py:222-225, 285-289, 354-361)
+// and the per-cycle scalars of a context
+// ---------------------------------------------------------------------------
+PMHD_FI double dt_from_maxima(int ndim, double cfl, const double m[3], const double w[3]) {
+  if (ndim == 1) return cfl / (m[0] / w[0]);
+  if (ndim == 2) return cfl / npmax(m[0] / w[0], m[1] / w[1]);
+  return cfl / npmax(m[0] / w[0], npmax(m[1] / w[1], m[2] / w[2]));
+}
+
+// the reference's end-of-interval clipping (Source/EvolveFunctions_MHD.py:259-260,
+// 472-473 for 1D/2D; :851-852 for 3D)
+PMHD_FI double clip_dt(int ndim, double t, double t_end, double dt) {
+  if (ndim == 3) {
+    if (t_end - t < dt) dt = t_end - t;
+  } else {
+    if (t + dt > t_end) dt = t_end - t;
+  }
+  return dt;
+}
+
+// 0.5*dt/dy as the reference's 80-bit value (Source/CTU_HelperModule.py:455),
+// from a double-double quotient rounded to a 64-bit significand; equal to the
+// host's long double result unless dt/dy lies within 2^-106 (relative) of a
+// 64-bit rounding midpoint
+PMHD_FI void half_ratio_80(double dt, double dy, double out[2]) {
+  const double qh = dt / dy;
+  const double r = fma(-qh, dy, dt);
+  const double ql = r / dy;
+  const double lo64 = round64_lo(qh, ql);
+  double s, e;
+  two_sum(qh, lo64, s, e);
+  out[0] = 0.5 * s;
+  out[1] = 0.5 * e;
+}
+
+template <class T>
+PMHD_FI void set_step_scalars(Ctx<T>& c, double dt) {
+  c.dt = T(dt);
+  c.hdt = T(0.5 * dt);
+  for (int d = 0; d < 3; ++d) {
+    const double dtd = dt / c.wd[d];
+    const double h = 0.5 * dtd;
+    c.dtd[d] = T(dtd);
+    c.hdtd[d] = T(h);
+    c.emfq[d] = T(h * 0.5);
+  }
+}
+
+// the double-buffered state of a context after a completed cycle
+template <class T>
+PMHD_FI void swap_buffers(Ctx<T>& c) {
+  T* u = c.U2;
+  T* w = c.W2;
+  c.U2 = const_cast<T*>(c.U);
+  c.W2 = const_cast<T*>(c.W);
+  c.U = u;
+  c.W = w;
+  for (int d = 0; d < 3; ++d) {
+    T* f = c.fb2[d];
+    c.fb2[d] = const_cast<T*>(c.fb[d]);
+    c.fb[d] = f;
+  }
+}
+
+// clock of one batch box; the second-last state is the other buffer once done > 0
+struct BoxClock {
+  double t, t2, dt;
+  int64_t cyc;   // cycle number
+  int64_t done;  // cycles completed since the box's state was set
+};
+
+// start of a cycle of one batch box, from its CFL maxima m: decide whether the box
+// still runs (the reference's loop condition), clip dt to the interval end and form
+// the cycle's scalars.  commit = 0 only forms the scalars (used by tile tuning).
+template <class T>
+PMHD_FI void box_begin(Ctx<T>& c, BoxClock& k, const double m[3], double cfl, double t_end, int64_t max_cycles,
+                       int commit) {
+  if (commit && !(k.t < t_end && k.cyc < max_cycles)) {
+    c.active = 0;
+    return;
+  }
+  double dt = dt_from_maxima(c.ndim, cfl, m, c.wd);
+  if (commit) {
+    dt = clip_dt(c.ndim, k.t, t_end, dt);
+    k.t2 = k.t;
+    k.dt = dt;
+  }
+  c.active = 1;
+  set_step_scalars(c, dt);
+  if (sizeof(T) == sizeof(double) && c.ndim == 3) half_ratio_80(dt, c.wd[1], c.ldh);
+}
+
+template <class T>
+PMHD_FI void box_end(Ctx<T>& c, BoxClock& k) {
+  if (!c.active) return;
+  swap_buffers(c);
+  k.t += k.dt;
+  k.cyc += 1;
+  k.done += 1;
 }
 
 }  // namespace pmhd

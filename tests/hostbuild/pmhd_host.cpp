@@ -10,13 +10,41 @@
 
 namespace pmhd {
 
-struct HostExec {
-  template <class F>
-  void run3(int, int64_t n0, int64_t n1, int64_t n2, F f) {
+template <class Ph, class T>
+void host_each(const Ctx<T>& c) {
+  int64_t n[3];
+  c.extent(Ph::kind, Ph::dir, n);
 #pragma omp parallel for collapse(2) schedule(static)
-    for (int64_t j = 0; j < n0; ++j)
-      for (int64_t i = 0; i < n1; ++i)
-        for (int64_t k = 0; k < n2; ++k) f(I3{j, i, k});
+  for (int64_t j = 0; j < n[0]; ++j)
+    for (int64_t i = 0; i < n[1]; ++i)
+      for (int64_t k = 0; k < n[2]; ++k) Ph()(c, I3{j, i, k});
+}
+
+template <class T>
+void host_cfl(const Ctx<T>& c, double out[3]) {
+  uint64_t m[3] = {0, 0, 0};
+  for (int64_t j = 0; j < c.ny; ++j)
+    for (int64_t i = 0; i < c.nx; ++i)
+      for (int64_t k = 0; k < c.nz; ++k) {
+        double o[3];
+        CflSpeeds()(c, I3{j, i, k}, o);
+        for (int d = 0; d < 3; ++d) {
+          uint64_t b;
+          std::memcpy(&b, &o[d], 8);
+          if (b > m[d]) m[d] = b;
+        }
+      }
+  for (int d = 0; d < 3; ++d) std::memcpy(&out[d], &m[d], 8);
+}
+
+struct HostExec {
+  template <class Ph, class T>
+  void run(const Ctx<T>& c) {
+    host_each<Ph>(c);
+  }
+  template <class T>
+  void cfl(const Ctx<T>& c, double out[3]) {
+    host_cfl(c, out);
   }
   template <class F>
   void reduce(int, int64_t n, F f, double sums[9], double maxes[2]) {
@@ -30,29 +58,76 @@ struct HostExec {
       maxes[1] = npmax(maxes[1], b[1]);
     }
   }
-  template <class F>
-  void max3(int, int64_t n, F f, double out[3]) {
-    uint64_t m[3] = {0, 0, 0};
-    for (int64_t i = 0; i < n; ++i) {
-      double o[3];
-      f(i, o);
+};
+
+// the batch executor of the host build: the same phases, box after box
+struct HostBatchExec {
+  BatchLayout* lay = nullptr;
+  bool tuning = false;
+  int tune_reps = 1;
+
+  template <class T>
+  Ctx<T>* ctxs() const {
+    return static_cast<Ctx<T>*>(lay->dctx);
+  }
+  template <class Ph, class T>
+  void run(const Ctx<T>&) {
+    if (tuning) lay->tuned_ms[Ph::id] = 0.f;
+    for (int b = 0; b < lay->nbox; ++b) {
+      const Ctx<T>& c = ctxs<T>()[b];
+      if (c.active) host_each<Ph>(c);
+    }
+  }
+  template <class T>
+  void cfl(const Ctx<T>&) {
+    for (int b = 0; b < lay->nbox; ++b) {
+      const Ctx<T>& c = ctxs<T>()[b];
+      if (!c.active) continue;
+      double m[3];
+      host_cfl(c, m);
       for (int d = 0; d < 3; ++d) {
-        uint64_t b;
-        std::memcpy(&b, &o[d], 8);
-        if (b > m[d]) m[d] = b;
+        uint64_t v;
+        std::memcpy(&v, &m[d], 8);
+        if (v > lay->dred[3 * b + d]) lay->dred[3 * b + d] = v;
       }
     }
-    for (int d = 0; d < 3; ++d) std::memcpy(&out[d], &m[d], 8);
+  }
+  template <class T>
+  void begin(double cfl, double t_end, int64_t max_cycles, int commit) {
+    for (int b = 0; b < lay->nbox; ++b) {
+      double m[3];
+      for (int d = 0; d < 3; ++d) {
+        uint64_t v = lay->dred[3 * b + d];
+        std::memcpy(&m[d], &v, 8);
+        lay->dred[3 * b + d] = 0;
+      }
+      box_begin(ctxs<T>()[b], lay->dclk[b], m, cfl, t_end, max_cycles, commit);
+    }
+  }
+  template <class T>
+  void end() {
+    for (int b = 0; b < lay->nbox; ++b) box_end(ctxs<T>()[b], lay->dclk[b]);
+  }
+  template <class T>
+  void activate_all() {
+    for (int b = 0; b < lay->nbox; ++b) ctxs<T>()[b].active = 1;
   }
 };
 
 struct HostBackend {
   HostExec ex;
+  HostBatchExec bex;
   HostExec& exec() { return ex; }
+  HostBatchExec& batch_exec(BatchLayout* lay) {
+    bex.lay = lay;
+    return bex;
+  }
+  int compute_units() const { return 1; }
   void* alloc(size_t bytes) { return std::calloc(bytes, 1); }
   void release(void* p) { std::free(p); }
   void h2d(void* d, const void* s, size_t b) { std::memcpy(d, s, b); }
   void d2h(void* d, const void* s, size_t b) { std::memcpy(d, s, b); }
+  void zero(void* d, size_t b) { std::memset(d, 0, b); }
   void sync() {}
   const char* error() { return nullptr; }
   void set_profiling(bool) {}
