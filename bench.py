@@ -111,6 +111,75 @@ def run_gpu(prim, cons, faces, n, precision, steps, warmup, profile_steps=3):
     return s, el, prof
 
 
+def kh_boxes(nbox=256, seed=7):
+    """Sides n ~ U{64..256} and (prim, cons, faces) of each Kelvin-Helmholtz box:
+    rho = 1 and v_x = +0.5 for |y - 0.5| < 0.25, rho = 2 and v_x = -0.5 elsewhere,
+    p = 2.5, B = (0.1, 0, 0), v_y = 0.01 * uniform white noise in [-1, 1]."""
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(64, 257, nbox)
+    out = []
+    for n in sides:
+        n = int(n)
+        y = (np.arange(n) + 0.5) / n
+        inner = (np.abs(y - 0.5) < 0.25)[:, None] * np.ones((1, n), dtype=bool)
+        prim = np.zeros((8, n, n))
+        prim[0] = np.where(inner, 1.0, 2.0)
+        prim[1] = np.where(inner, 0.5, -0.5)
+        prim[2] = 0.01 * rng.uniform(-1.0, 1.0, (n, n))
+        prim[4] = 0.1
+        prim[7] = 2.5
+        cons = prim.copy()
+        cons[1:4] = prim[1:4] * prim[0]
+        cons[7] = prim[7] / (GAMMA - 1.0) + 0.5 * prim[0] * (prim[1] ** 2 + prim[2] ** 2 + prim[3] ** 2) \
+            + 0.5 * (prim[4] ** 2 + prim[5] ** 2 + prim[6] ** 2)
+        faces = (np.full((n, n + 1), 0.1), np.zeros((n + 1, n)))
+        out.append((n, prim, cons, faces))
+    return out
+
+
+def run_kh(cycles=200, seq_cycles=50, nbox=256):
+    """The batch of KH boxes: one batch launch per phase (tile-tuned) vs one
+    handle per box advanced box after box (per-box launches and dt reads)."""
+    from pythonmhd_amd._lib import Batch, Solver
+    boxes = kh_boxes(nbox)
+    shapes = [(n, n) for n, _, _, _ in boxes]
+    widths = [(1.0 / n, 1.0 / n) for n, _, _, _ in boxes]
+    cells = sum(n * n for n, _, _, _ in boxes)
+    b = Batch(shapes, widths, (1, 1), GAMMA, 0.8, order=1, entropy_fix=False, precision=32)
+    for k, (n, prim, cons, faces) in enumerate(boxes):
+        b.set_state(k, prim, cons, faces)
+    t0 = time.perf_counter()
+    b.autotune(3)
+    tune_s = time.perf_counter() - t0
+    b.advance(0.0, 1e30, 0, 2)          # warm-up (state restarts below)
+    for k, (n, prim, cons, faces) in enumerate(boxes):
+        b.set_state(k, prim, cons, faces)
+    b.synchronize()
+    t0 = time.perf_counter()
+    _, cyc, _ = b.advance(0.0, 1e30, 0, cycles)
+    b.synchronize()
+    el = time.perf_counter() - t0
+    tiles = b.tiles()
+    b.close()
+    assert int(cyc.min()) == cycles
+    t0 = time.perf_counter()
+    for n, prim, cons, faces in boxes:
+        s = Solver((n, n), (1.0 / n, 1.0 / n), (1, 1), GAMMA, 0.8, order=1, entropy_fix=False, precision=32)
+        s.set_state(prim, cons, faces)
+        s.advance(0.0, 1e30, 0, seq_cycles)
+        s.synchronize()
+        s.close()
+    el_seq = time.perf_counter() - t0
+    v = cells * cycles / el
+    vs = cells * seq_cycles / el_seq
+    return {"boxes": nbox, "sides": "U{64..256} seed 7", "cells": cells, "cycles": cycles,
+            "value": v, "unit": "cell-updates/s", "seconds": el, "autotune_seconds": tune_s,
+            "tiles": {k: list(t) for k, t in sorted(tiles.items())},
+            "sequential": {"value": vs, "cycles": seq_cycles, "seconds": el_seq,
+                           "how": "one handle per box, box after box, host dt read per cycle"},
+            "speedup_vs_sequential": v / vs}
+
+
 def cpu_baseline(n=64, cycles=2):
     """The FP64 NumPy oracle (single thread) on a bounded sample of the workload."""
     from oracle import mhd as M
@@ -164,7 +233,21 @@ def main():
     ap.add_argument('--n', type=int, default=256)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
+    ap.add_argument('--no-kh', action='store_true')
+    ap.add_argument('--kh-cycles', type=int, default=200)
+    ap.add_argument('--copy-probe', action='store_true',
+                    help='only run a plain 1 GiB device copy three times (HBM counter calibration)')
     a = ap.parse_args()
+    if a.copy_probe:
+        import torch
+        x = torch.ones(2 ** 28, device='cuda', dtype=torch.float32)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print('copy probe: 3 copies of %d bytes' % (4 * 2 ** 28), file=sys.stderr)
+        return
     if a.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d refused)' % a.gpus,
               file=sys.stderr)
@@ -201,6 +284,10 @@ def main():
         out["fp32_vs_fp64_steps"] = W + K + 3
     else:
         s32.close()
+    if not a.no_kh:
+        out["kh_batch"] = run_kh(cycles=a.kh_cycles)
+        print('bench: KH batch %.3e cell-updates/s (sequential %.3e)' % (
+            out["kh_batch"]["value"], out["kh_batch"]["sequential"]["value"]), file=sys.stderr, flush=True)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         print('bench: cpu baseline %.3e cell-updates/s' % out["cpu_baseline"]["value"], file=sys.stderr,

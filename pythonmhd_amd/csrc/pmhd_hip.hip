@@ -118,35 +118,10 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce(int64_t n, F f, double* 
 // [prefix[b], prefix[b+1]); a tile is bk (along z) x bi (along x) elements of one
 // y row.  The grid walks the list with a stride (persistent when smaller).
 // ---------------------------------------------------------------------------
-struct TileRef {
-  int box, j, i, k;
-  bool in;
-};
-
 template <class T>
 __device__ inline TileRef tile_of(const Ctx<T>* __restrict__ ctxs, const int* __restrict__ prefix, int nbox,
                                   int vb, int kind, int dir, int bk, int bi) {
-  int lo = 0, hi = nbox;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (prefix[mid] <= vb) lo = mid;
-    else hi = mid;
-  }
-  TileRef r;
-  r.box = lo;
-  int64_t n[3];
-  ctxs[lo].extent(kind, dir, n);
-  const int gk = (int(n[2]) + bk - 1) / bk;
-  const int gi = (int(n[1]) + bi - 1) / bi;
-  const int local = vb - prefix[lo];
-  const int tk = local % gk;
-  const int rest = local / gk;
-  const int ti = rest % gi;
-  r.j = rest / gi;
-  r.k = tk * bk + int(threadIdx.x) % bk;
-  r.i = ti * bi + int(threadIdx.x) / bk;
-  r.in = r.k < n[2] && r.i < n[1];
-  return r;
+  return locate_tile(ctxs, prefix, nbox, vb, kind, dir, bk, bi, int(threadIdx.x));
 }
 
 template <class Ph, class T>

@@ -216,6 +216,40 @@ PMHD_FI void swap_buffers(Ctx<T>& c) {
   }
 }
 
+// the element of thread `tid` in tile vb of a batch launch: box b owns tiles
+// [prefix[b], prefix[b+1]); a tile is bk (along z) x bi (along x) elements of one
+// y row, tiles ordered z-fastest, then x, then y
+struct TileRef {
+  int box, j, i, k;
+  bool in;
+};
+
+template <class T>
+PMHD_FI TileRef locate_tile(const Ctx<T>* ctxs, const int* prefix, int nbox, int vb, int kind, int dir, int bk,
+                            int bi, int tid) {
+  int lo = 0, hi = nbox;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= vb) lo = mid;
+    else hi = mid;
+  }
+  TileRef r;
+  r.box = lo;
+  int64_t n[3];
+  ctxs[lo].extent(kind, dir, n);
+  const int gk = (int(n[2]) + bk - 1) / bk;
+  const int gi = (int(n[1]) + bi - 1) / bi;
+  const int local = vb - prefix[lo];
+  const int tk = local % gk;
+  const int rest = local / gk;
+  const int ti = rest % gi;
+  r.j = rest / gi;
+  r.k = tk * bk + tid % bk;
+  r.i = ti * bi + tid / bk;
+  r.in = r.k < n[2] && r.i < n[1] && r.j < n[0];
+  return r;
+}
+
 // clock of one batch box; the second-last state is the other buffer once done > 0
 struct BoxClock {
   double t, t2, dt;

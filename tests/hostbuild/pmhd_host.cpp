@@ -5,6 +5,9 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
 
 #include "pmhd_api.h"
 
@@ -70,12 +73,52 @@ struct HostBatchExec {
   Ctx<T>* ctxs() const {
     return static_cast<Ctx<T>*>(lay->dctx);
   }
+  // walk the launch exactly as the HIP batch kernel does (blocks striding over the
+  // tile list, threads of a tile through locate_tile) and check that every element
+  // of every active box is visited once
+  // tuning on the host walks the phase with every candidate tile (checking each
+  // walk) and keeps a candidate that varies with the launch id, so later launches
+  // exercise many shapes
   template <class Ph, class T>
-  void run(const Ctx<T>&) {
-    if (tuning) lay->tuned_ms[Ph::id] = 0.f;
+  void run(const Ctx<T>& c0) {
+    if (tuning) {
+      const std::vector<TileCfg> cands = tile_candidates(*lay, Ph::id);
+      for (const TileCfg& t : cands)
+        if (lay->apply_tile(Ph::id, t)) walk<Ph, T>();
+      (void)lay->apply_tile(Ph::id, cands[size_t(Ph::id) % cands.size()]);
+      lay->tuned_ms[Ph::id] = 0.f;
+    }
+    (void)c0;
+    walk<Ph, T>();
+  }
+
+  template <class Ph, class T>
+  void walk() {
+    const TileCfg t = lay->tiles[Ph::id];
+    const int total = lay->total[Ph::id];
+    const int grid = t.grid > 0 && t.grid < total ? t.grid : total;
+    const int* prefix = lay->dprefix + Ph::id * (lay->nbox + 1);
+    std::vector<std::vector<unsigned char>> seen(static_cast<size_t>(lay->nbox));
     for (int b = 0; b < lay->nbox; ++b) {
-      const Ctx<T>& c = ctxs<T>()[b];
-      if (c.active) host_each<Ph>(c);
+      int64_t n[3];
+      ctxs<T>()[b].extent(Ph::kind, Ph::dir, n);
+      seen[size_t(b)].assign(size_t(n[0] * n[1] * n[2]), 0);
+    }
+    for (int blk = 0; blk < grid; ++blk)
+      for (int vb = blk; vb < total; vb += grid)
+        for (int tid = 0; tid < t.bk * t.bi; ++tid) {
+          const TileRef r = locate_tile(ctxs<T>(), prefix, lay->nbox, vb, Ph::kind, Ph::dir, t.bk, t.bi, tid);
+          const Ctx<T>& c = ctxs<T>()[r.box];
+          if (!c.active || !r.in) continue;
+          Ph()(c, I3{r.j, r.i, r.k});
+          int64_t n[3];
+          c.extent(Ph::kind, Ph::dir, n);
+          seen[size_t(r.box)][size_t((int64_t(r.j) * n[1] + r.i) * n[2] + r.k)] += 1;
+        }
+    for (int b = 0; b < lay->nbox; ++b) {
+      if (!ctxs<T>()[b].active) continue;
+      for (unsigned char v : seen[size_t(b)])
+        if (v != 1) throw std::runtime_error(std::string("tile walk of ") + kernel_name(Ph::id) + " not exactly once");
     }
   }
   template <class T>
