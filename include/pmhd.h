@@ -43,7 +43,10 @@ typedef struct pmhd_params {
   int32_t reconstruct_order;             /* 0 none, 1 PLM, 2 PPM */
   int32_t entropy_fix;                   /* 0/1 */
   int32_t precision;                     /* 64: FP64 throughout; 32: FP32 bulk, FP64 islands */
-  int32_t flags;                         /* reserved, 0 */
+  int32_t physics;                       /* 0 ideal MHD (evolve_*D_mhd): cells (8, ...) + faces;
+                                            1 hydrodynamics (evolve_*D_hydro,
+                                            Source/EvolveFunctions_Hydro.py:232, 350, 559):
+                                            cells (5, ...) = (rho, v/M, p/E), no faces */
 } pmhd_params;
 
 typedef struct pmhd_handle pmhd_handle;

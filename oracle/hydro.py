@@ -30,7 +30,7 @@ Deviations (inputs where the reference raises instead of returning a result):
     (Source/EvolveFunctions_Hydro.py:717) and raises ValueError; here the edge
     column nx-1 is used, as for every other outflow ghost.
   * reconstructOrder 1 runs first order in the reference (only 2 selects PPM,
-    Source/EvolveFunctions_Hydro.py:272, 455, 657); here it selects PLM.
+    Source/EvolveFunctions_Hydro.py:272, 408, 628); here it selects PLM.
 """
 import numpy as np
 
@@ -330,7 +330,7 @@ def roe_flux(WL, WR, gamma, entropy_fix, eps, d):
 
 
 # ---------------------------------------------------------------------------
-# one cycle (Source/EvolveFunctions_Hydro.py:259-311 1D, :443-512 2D, :643-799 3D)
+# one cycle (Source/EvolveFunctions_Hydro.py:260-304 1D, :390-515 2D, :608-805 3D)
 # ---------------------------------------------------------------------------
 class Params:
     def __init__(self, gamma, cfl, min_dens=1e-12, min_pres=1e-12, min_energy=1e-12,

@@ -33,7 +33,7 @@ class PmhdParams(ctypes.Structure):
                 ('min_dens', ctypes.c_double), ('min_pres', ctypes.c_double),
                 ('min_energy', ctypes.c_double), ('epsilon', ctypes.c_double),
                 ('reconstruct_order', ctypes.c_int32), ('entropy_fix', ctypes.c_int32),
-                ('precision', ctypes.c_int32), ('flags', ctypes.c_int32)]
+                ('precision', ctypes.c_int32), ('physics', ctypes.c_int32)]
 
 
 class PmhdError(RuntimeError):
@@ -119,7 +119,7 @@ def make_grid(shape, widths, bcs):
 
 
 def make_params(gamma, cfl, min_dens=1e-12, min_pres=1e-12, min_energy=1e-12, order=2,
-                entropy_fix=False, epsilon=0.5, precision=64):
+                entropy_fix=False, epsilon=0.5, precision=64, physics=0):
     p = PmhdParams()
     p.gamma, p.cfl = float(gamma), float(cfl)
     p.min_dens, p.min_pres, p.min_energy = float(min_dens), float(min_pres), float(min_energy)
@@ -127,12 +127,19 @@ def make_params(gamma, cfl, min_dens=1e-12, min_pres=1e-12, min_energy=1e-12, or
     p.reconstruct_order = int(order)
     p.entropy_fix = 1 if entropy_fix else 0
     p.precision = int(precision)
-    p.flags = 0
+    p.physics = int(physics)
     return p
 
 
-def face_shapes(shape):
-    if len(shape) == 1:
+MHD, HYDRO = 0, 1
+
+
+def nvars(physics):
+    return 5 if physics == HYDRO else 8
+
+
+def face_shapes(shape, physics=MHD):
+    if len(shape) == 1 or physics == HYDRO:
         return []
     if len(shape) == 2:
         ny, nx = shape
@@ -141,13 +148,14 @@ def face_shapes(shape):
     return [(ny, nx + 1, nz), (ny + 1, nx, nz), (ny, nx, nz + 1)]
 
 
-def _state_args(shape, prim, cons, faces):
+def _state_args(shape, prim, cons, faces, physics=MHD):
     prim = np.ascontiguousarray(prim, dtype=np.float64)
     cons = np.ascontiguousarray(cons, dtype=np.float64)
-    if prim.shape != (8,) + shape or cons.shape != (8,) + shape:
-        raise ValueError('state shape %s does not match the grid %s' % (prim.shape, shape))
+    nv = nvars(physics)
+    if prim.shape != (nv,) + shape or cons.shape != (nv,) + shape:
+        raise ValueError('state shape %s does not match the grid (%d,) + %s' % (prim.shape, nv, shape))
     fs = [np.ascontiguousarray(f, dtype=np.float64) for f in faces]
-    want = face_shapes(shape)
+    want = face_shapes(shape, physics)
     if len(fs) != len(want):
         raise ValueError('expected %d face arrays' % len(want))
     for f, s in zip(fs, want):
@@ -162,12 +170,14 @@ class Solver:
 
     def __init__(self, shape, widths, bcs, gamma, cfl, min_dens=1e-12, min_pres=1e-12,
                  min_energy=1e-12, order=2, entropy_fix=False, epsilon=0.5, precision=64,
-                 lib=None):
+                 lib=None, physics=MHD):
         self.lib = lib if lib is not None else load()
         self.ndim = len(shape)
         self.shape = tuple(int(s) for s in shape)   # (nx,) / (ny, nx) / (ny, nx, nz)
+        self.physics = int(physics)
         g = make_grid(self.shape, widths, bcs)
-        p = make_params(gamma, cfl, min_dens, min_pres, min_energy, order, entropy_fix, epsilon, precision)
+        p = make_params(gamma, cfl, min_dens, min_pres, min_energy, order, entropy_fix, epsilon, precision,
+                        physics)
         self.grid, self.params = g, p
         h = ctypes.c_void_p()
         rc = self.lib.pmhd_create(ctypes.byref(g), ctypes.byref(p), ctypes.byref(h))
@@ -180,16 +190,16 @@ class Solver:
             raise PmhdError('%s failed (%d): %s' % (what, rc, self.lib.pmhd_last_error(self.h).decode()))
 
     def face_shapes(self):
-        return face_shapes(self.shape)
+        return face_shapes(self.shape, self.physics)
 
     def set_state(self, prim, cons, faces=()):
-        keep, args = _state_args(self.shape, prim, cons, faces)
+        keep, args = _state_args(self.shape, prim, cons, faces, self.physics)
         self._check(self.lib.pmhd_set_state(self.h, *args), 'pmhd_set_state')
         del keep
 
     def get_state(self, which=0):
-        prim = np.empty((8,) + self.shape)
-        cons = np.empty((8,) + self.shape)
+        prim = np.empty((nvars(self.physics),) + self.shape)
+        cons = np.empty((nvars(self.physics),) + self.shape)
         fs = [np.empty(s) for s in self.face_shapes()]
         fp = [_ptr(f) for f in fs] + [None] * (3 - len(fs))
         self._check(self.lib.pmhd_get_state(self.h, which, _ptr(prim), _ptr(cons), *fp), 'pmhd_get_state')
@@ -261,8 +271,10 @@ class Batch:
     per-box CFL timestep and clock on the device (pmhd_batch_* in include/pmhd.h)."""
 
     def __init__(self, shapes, widths, bcs, gamma, cfl, min_dens=1e-12, min_pres=1e-12,
-                 min_energy=1e-12, order=2, entropy_fix=False, epsilon=0.5, precision=64, lib=None):
+                 min_energy=1e-12, order=2, entropy_fix=False, epsilon=0.5, precision=64, lib=None,
+                 physics=MHD):
         self.lib = lib if lib is not None else load()
+        self.physics = int(physics)
         self.shapes = [tuple(int(x) for x in s) for s in shapes]
         n = len(self.shapes)
         if n < 1:
@@ -270,7 +282,8 @@ class Batch:
         widths = list(widths) if len(widths) == n and hasattr(widths[0], '__len__') else [widths] * n
         bcs = list(bcs) if len(bcs) == n and hasattr(bcs[0], '__len__') else [bcs] * n
         grids = (PmhdGrid * n)(*[make_grid(s, w, b) for s, w, b in zip(self.shapes, widths, bcs)])
-        p = make_params(gamma, cfl, min_dens, min_pres, min_energy, order, entropy_fix, epsilon, precision)
+        p = make_params(gamma, cfl, min_dens, min_pres, min_energy, order, entropy_fix, epsilon, precision,
+                        physics)
         self.grids, self.params = grids, p
         h = ctypes.c_void_p()
         rc = self.lib.pmhd_batch_create(n, grids, ctypes.byref(p), ctypes.byref(h))
@@ -289,15 +302,15 @@ class Batch:
         return self.shapes[int(box)]
 
     def set_state(self, box, prim, cons, faces=()):
-        keep, args = _state_args(self._shape(box), prim, cons, faces)
+        keep, args = _state_args(self._shape(box), prim, cons, faces, self.physics)
         self._check(self.lib.pmhd_batch_set_state(self.h, int(box), *args), 'pmhd_batch_set_state')
         del keep
 
     def get_state(self, box, which=0):
         shape = self._shape(box)
-        prim = np.empty((8,) + shape)
-        cons = np.empty((8,) + shape)
-        fs = [np.empty(s) for s in face_shapes(shape)]
+        prim = np.empty((nvars(self.physics),) + shape)
+        cons = np.empty((nvars(self.physics),) + shape)
+        fs = [np.empty(s) for s in face_shapes(shape, self.physics)]
         fp = [_ptr(f) for f in fs] + [None] * (3 - len(fs))
         self._check(self.lib.pmhd_batch_get_state(self.h, int(box), int(which), _ptr(prim), _ptr(cons), *fp),
                     'pmhd_batch_get_state')

@@ -15,7 +15,7 @@
 #include <vector>
 
 #include "pmhd.h"
-#include "pmhd_step.h"
+#include "pmhd_hydro.h"
 
 struct pmhd_handle {
   virtual ~pmhd_handle() {}
@@ -151,6 +151,7 @@ void init_ctx(Ctx<T>& c, const pmhd_grid& g, const pmhd_params& p) {
   std::memset(&c, 0, sizeof(c));
   c.ndim = g.ndim;
   c.active = 1;
+  c.physics = p.physics;
   c.ny = g.ndim >= 2 ? g.ny : 1;
   c.nx = g.nx;
   c.nz = g.ndim == 3 ? g.nz : 1;
@@ -181,34 +182,40 @@ struct StateBufs {
   T* FB[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
 };
 
+// cell variables and interface-state variables of a physics
+inline int cell_vars(int physics) { return physics == 1 ? 5 : 8; }
+inline int face_vars(int physics) { return physics == 1 ? 5 : 7; }
+
 // every device buffer of one grid; alloc(n) returns room for n values of T
 template <class T, class A>
 void alloc_grid(Ctx<T>& c, StateBufs<T>& s, int order, A&& alloc) {
   const int nd = c.ndim;
   const int64_t N = c.ncell();
+  const int64_t cv = cell_vars(c.physics), fv = face_vars(c.physics);
+  const bool mhd = c.physics == 0;
   int64_t maxf = 0, maxs = 0;
   for (int d = 0; d < nd; ++d) {
     maxf = std::max<int64_t>(maxf, c.nface(d));
     maxs = std::max<int64_t>(maxs, c.nslope(d));
   }
   for (int b = 0; b < 2; ++b) {
-    s.U[b] = alloc(8 * N);
-    s.W[b] = alloc(8 * N);
-    for (int d = 0; d < 3; ++d) s.FB[b][d] = alloc(d < nd && nd >= 2 ? c.nface(d) : 1);
+    s.U[b] = alloc(cv * N);
+    s.W[b] = alloc(cv * N);
+    for (int d = 0; d < 3; ++d) s.FB[b][d] = alloc(mhd && d < nd && nd >= 2 ? c.nface(d) : 1);
   }
-  c.WL = alloc(7 * maxf);
-  c.WR = alloc(7 * maxf);
-  c.S = order > 0 ? alloc(7 * maxs) : alloc(1);
+  c.WL = alloc(fv * maxf);
+  c.WR = alloc(fv * maxf);
+  c.S = order > 0 ? alloc(fv * maxs) : alloc(1);
   for (int d = 0; d < 3; ++d) {
     const bool used = d < nd;
-    c.UL[d] = alloc(used ? 7 * c.nface(d) : 1);
-    c.UR[d] = alloc(used ? 7 * c.nface(d) : 1);
-    c.F[d] = alloc(used ? 7 * c.nface(d) : 1);
-    c.Fc[d] = alloc(used && nd >= 2 ? 7 * c.nface(d) : 1);
-    c.nfb[d] = alloc(used && nd >= 2 ? c.nface(d) : 1);
-    c.E[d] = alloc(nd == 3 || (nd == 2 && d == 2) ? c.nedge() : 1);
+    c.UL[d] = alloc(used ? fv * c.nface(d) : 1);
+    c.UR[d] = alloc(used ? fv * c.nface(d) : 1);
+    c.F[d] = alloc(used ? fv * c.nface(d) : 1);
+    c.Fc[d] = alloc(used && nd >= 2 ? fv * c.nface(d) : 1);
+    c.nfb[d] = alloc(mhd && used && nd >= 2 ? c.nface(d) : 1);
+    c.E[d] = alloc(mhd && (nd == 3 || (nd == 2 && d == 2)) ? c.nedge() : 1);
   }
-  c.Ec = alloc(nd == 3 ? 3 * N : N);
+  c.Ec = alloc(mhd ? (nd == 3 ? 3 * N : N) : 1);
 }
 
 // point the context at buffer set `cur` (the update writes 1 - cur)
@@ -255,17 +262,18 @@ int load_state(B& be, Ctx<T>& c, const pmhd_grid& g, const double* prim, const d
     err = "prim and cons are required";
     return PMHD_EINVAL;
   }
-  for (int d = 0; d < g.ndim && g.ndim >= 2; ++d)
+  const bool mhd = c.physics == 0;
+  for (int d = 0; mhd && d < g.ndim && g.ndim >= 2; ++d)
     if (!fin[d]) {
       err = "missing face array";
       return PMHD_EINVAL;
     }
   const int64_t N = c.ncell();
-  upload_vals(be, U, cons, 8 * N);
-  upload_vals(be, W, prim, 8 * N);
-  if (g.ndim >= 2)
+  upload_vals(be, U, cons, cell_vars(c.physics) * N);
+  upload_vals(be, W, prim, cell_vars(c.physics) * N);
+  if (mhd && g.ndim >= 2)
     for (int d = 0; d < g.ndim; ++d) upload_vals(be, FB[d], fin[d], c.nface(d));
-  if (g.ndim == 1) c.bx1d = T(prim[4 * N]);
+  if (mhd && g.ndim == 1) c.bx1d = T(prim[4 * N]);
   return PMHD_OK;
 }
 
@@ -273,9 +281,9 @@ template <class T, class B>
 void store_state(B& be, const Ctx<T>& c, const pmhd_grid& g, const T* U, const T* W, const T* const FB[3],
                  double* prim, double* cons, double* const fo[3]) {
   const int64_t N = c.ncell();
-  if (cons) download_vals(be, cons, U, 8 * N);
-  if (prim) download_vals(be, prim, W, 8 * N);
-  if (g.ndim >= 2)
+  if (cons) download_vals(be, cons, U, cell_vars(c.physics) * N);
+  if (prim) download_vals(be, prim, W, cell_vars(c.physics) * N);
+  if (c.physics == 0 && g.ndim >= 2)
     for (int d = 0; d < g.ndim; ++d)
       if (fo[d]) download_vals(be, fo[d], FB[d], c.nface(d));
 }
@@ -291,6 +299,7 @@ struct Handle : public pmhd_handle {
   std::vector<void*> owned;
   int cur = 0;
   int64_t cycles_since_set = 0;
+  bool speeds_ready = false;   // the last update left the new state's CFL maxima
 
   T* dalloc(int64_t n) {
     if (n <= 0) n = 1;
@@ -324,6 +333,7 @@ struct Handle : public pmhd_handle {
     const double* fin[3] = {fbx, fby, fbz};
     cur = 0;
     cycles_since_set = 0;
+    speeds_ready = false;
     bind_bufs(c, sb, cur);
     int rc = load_state(be, c, g, prim, cons, fin, sb.U[0], sb.W[0], sb.FB[0], err);
     if (rc) return rc;
@@ -345,10 +355,11 @@ struct Handle : public pmhd_handle {
 
   int compute_dt(double* dt) override {
     double m[3] = {0, 0, 0};
-    be.exec().cfl(c, m);
+    if (speeds_ready) be.exec().max_partials(m);
+    else be.exec().cfl(c, m);
     const char* e = be.error();
     if (e) return fail(PMHD_EHIP, e);
-    *dt = dt_from_maxima(g.ndim, p.cfl, m, c.wd);
+    *dt = dt_of(c, p.cfl, m);
     return PMHD_OK;
   }
 
@@ -360,10 +371,11 @@ struct Handle : public pmhd_handle {
       c.ldh[0] = static_cast<double>(hl);
       c.ldh[1] = static_cast<double>(hl - static_cast<long double>(c.ldh[0]));
     }
-    cycle(be.exec(), c);
+    run_cycle(be.exec(), c);
     cur = 1 - cur;
     bind_bufs(c, sb, cur);
     ++cycles_since_set;
+    speeds_ready = true;
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
   }
@@ -382,7 +394,7 @@ struct Handle : public pmhd_handle {
     if (totals)
       for (int q = 0; q < 8; ++q) totals[q] = sums[q];
     if (nonfinite) *nonfinite = int64_t(sums[8]);
-    if (max_divb) *max_divb = g.ndim >= 2 && maxes[1] > 0 ? maxes[0] / maxes[1] : 0.0;
+    if (max_divb) *max_divb = c.physics == 0 && g.ndim >= 2 && maxes[1] > 0 ? maxes[0] / maxes[1] : 0.0;
     if (max_b) *max_b = maxes[1];
     return PMHD_OK;
   }
@@ -478,7 +490,8 @@ struct BatchHandle : public pmhd_batch {
         int64_t e[3];
         hctx[size_t(b)].extent(kind, dir, e);
         for (int q = 0; q < 3; ++q) lay.ext[(size_t(id) * size_t(n) + size_t(b)) * 3 + size_t(q)] = e[q];
-        lay.max_n2 = std::max<int64_t>(lay.max_n2, e[2]);
+        // launches along a direction the boxes do not have never run
+        if (dir < lay.ndim) lay.max_n2 = std::max<int64_t>(lay.max_n2, e[2]);
       }
     }
     lay.hprefix.assign(size_t(K_COUNT) * size_t(n + 1), 0);
@@ -545,9 +558,10 @@ struct BatchHandle : public pmhd_batch {
     bx.template activate_all<T>();
     const int64_t iters = max_cycles - cycle0;
     for (int64_t it = 0; it < iters; ++it) {
-      bx.cfl(hctx[0]);
+      // later cycles take their speeds from the previous update's epilogue
+      if (it == 0) bx.cfl(hctx[0]);
       bx.template begin<T>(p.cfl, t_end, max_cycles, 1);
-      cycle(bx, hctx[0]);
+      run_cycle(bx, hctx[0]);
       bx.template end<T>();
       if ((it + 1) % 16 == 0 && it + 1 < iters) {
         be.d2h(hclk.data(), lay.dclk, sizeof(BoxClock) * size_t(nbox));
@@ -579,7 +593,7 @@ struct BatchHandle : public pmhd_batch {
     bx.tune_reps = reps > 0 ? reps : 3;
     bx.cfl(hctx[0]);
     bx.template begin<T>(p.cfl, std::numeric_limits<double>::infinity(), std::numeric_limits<int64_t>::max(), 0);
-    cycle(bx, hctx[0]);
+    run_cycle(bx, hctx[0]);
     bx.tuning = false;
     be.zero(lay.dred, 3 * sizeof(unsigned long long) * size_t(nbox));
     be.sync();
@@ -669,6 +683,7 @@ inline int check_params(const pmhd_params* p, std::string& err) {
     err = "reconstruct_order must be 0, 1 or 2"; return PMHD_EINVAL;
   }
   if (p->precision != 64 && p->precision != 32) { err = "precision must be 64 or 32"; return PMHD_EINVAL; }
+  if (p->physics != 0 && p->physics != 1) { err = "physics must be 0 (MHD) or 1 (hydrodynamics)"; return PMHD_EINVAL; }
   return PMHD_OK;
 }
 
@@ -761,7 +776,7 @@ inline int advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int6
     double dt;
     int rc = h->compute_dt(&dt);
     if (rc) return rc;
-    dt = clip_dt(h->g.ndim, t, t_end, dt);
+    dt = clip_dt(h->g.ndim, h->p.physics, t, t_end, dt);
     rc = h->step(dt);
     if (rc) return rc;
     t += dt;

@@ -29,11 +29,12 @@ __device__ inline unsigned long long umax64(unsigned long long a, unsigned long 
   return a > b ? a : b;
 }
 
-// maximum of three non-negative values over a 256-thread block, then one atomic
-// per component.  The IEEE bit patterns of non-negative doubles order like the
-// values (NaN above +inf, as np.max propagates it), so the reduction is an
-// integer max: deterministic in any order.
-__device__ inline void block_max3_atomic(unsigned long long m[3], unsigned long long* out) {
+// Maxima of three non-negative values.  The IEEE bit patterns of non-negative
+// doubles order like the values (NaN above +inf, as np.max propagates it), so the
+// reductions are integer maxima: deterministic in any order.
+// block_max3: reduce over a block of at most 256 threads; the result is valid in
+// thread 0.  Every thread of the block must call it.
+__device__ inline void block_max3(unsigned long long m[3]) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
@@ -53,25 +54,49 @@ __device__ inline void block_max3_atomic(unsigned long long m[3], unsigned long 
     for (int d = 0; d < 3; ++d) {
       unsigned long long v = sm[d][0];
       for (int w = 1; w < nw; ++w) v = umax64(v, sm[d][w]);
-      atomicMax(&out[d], v);
+      m[d] = v;
     }
   }
   __syncthreads();
 }
 
-template <class T>
-__global__ void __launch_bounds__(256) k_cfl(const Ctx<T> c, int n1, int n2, unsigned long long* out) {
+__device__ inline unsigned long long speed_bits(double x) { return (unsigned long long)__double_as_longlong(x); }
+
+// a phase whose elements also yield three speeds: each block writes its maxima
+// to part[3 * block] (no atomics; reduced by k_max_parts)
+template <class Ph, class T>
+__global__ void __launch_bounds__(256) k_phase_max(const Ctx<T> c, int n1, int n2, unsigned long long* part) {
   const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
   const int i = int(blockIdx.y * blockDim.y + threadIdx.y);
   const int j = int(blockIdx.z);
   unsigned long long m[3] = {0ull, 0ull, 0ull};
   if (k < n2 && i < n1) {
     double o[3];
-    CflSpeeds()(c, I3{j, i, k}, o);
+    Ph()(c, I3{j, i, k}, o);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) m[d] = (unsigned long long)__double_as_longlong(o[d]);
+    for (int d = 0; d < 3; ++d) m[d] = speed_bits(o[d]);
   }
-  block_max3_atomic(m, out);
+  block_max3(m);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const size_t b = size_t(blockIdx.x) + size_t(gridDim.x) * (size_t(blockIdx.y) + size_t(gridDim.y) * blockIdx.z);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) part[3 * b + d] = m[d];
+  }
+}
+
+// one block of 256 threads: out[d] = max over the nparts partial maxima
+__global__ void __launch_bounds__(256) k_max_parts(const unsigned long long* part, int64_t nparts,
+                                                   unsigned long long* out) {
+  unsigned long long m[3] = {0ull, 0ull, 0ull};
+  for (int64_t b = threadIdx.x; b < nparts; b += blockDim.x) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) m[d] = umax64(m[d], part[3 * b + d]);
+  }
+  block_max3(m);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[d] = m[d];
+  }
 }
 
 // Deterministic two-level reduction of 9 sums and 2 maxima in double: each
@@ -149,11 +174,40 @@ __global__ void __launch_bounds__(256) k_batch_cfl(const Ctx<T>* __restrict__ ct
       double o[3];
       CflSpeeds()(c, I3{r.j, r.i, r.k}, o);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) m[d] = (unsigned long long)__double_as_longlong(o[d]);
+      for (int d = 0; d < 3; ++d) m[d] = speed_bits(o[d]);
     }
-    block_max3_atomic(m, red + 3 * r.box);
+    block_max3(m);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) atomicMax(&red[3 * r.box + d], m[d]);
+    }
   }
 }
+
+// a batch phase that also yields speeds: per-box maxima into red (3 per box)
+template <class Ph, class T>
+__global__ void __launch_bounds__(256) k_batch_max(const Ctx<T>* __restrict__ ctxs, const int* __restrict__ prefix,
+                                                   int nbox, int bk, int bi, unsigned long long* red) {
+  const int total = prefix[nbox];
+  for (int vb = int(blockIdx.x); vb < total; vb += int(gridDim.x)) {
+    const TileRef r = tile_of(ctxs, prefix, nbox, vb, Ph::kind, Ph::dir, bk, bi);
+    const Ctx<T>& c = ctxs[r.box];
+    if (!c.active) continue;
+    unsigned long long m[3] = {0ull, 0ull, 0ull};
+    if (r.in) {
+      double o[3];
+      Ph()(c, I3{r.j, r.i, r.k}, o);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) m[d] = speed_bits(o[d]);
+    }
+    block_max3(m);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) atomicMax(&red[3 * r.box + d], m[d]);
+    }
+  }
+}
+
 
 template <class T>
 __global__ void k_box_begin(Ctx<T>* ctxs, BoxClock* clk, unsigned long long* red, int nbox, double cfl,
@@ -276,21 +330,45 @@ struct HipExec {
     note(hipGetLastError());
   }
 
-  template <class T>
-  void cfl(const Ctx<T>& c, double out[3]) {
-    note(hipMemsetAsync(red, 0, 3 * sizeof(unsigned long long), s));
+  // partial maxima of the last run_max launch (one triple per block)
+  unsigned long long* parts = nullptr;
+  int64_t parts_cap = 0;
+  int64_t parts_n = 0;
+
+  template <class Ph, class T>
+  void run_max(const Ctx<T>& c) {
     int64_t n[3];
-    c.extent(EXT_CELL, 0, n);
+    c.extent(Ph::kind, Ph::dir, n);
     dim3 grid, block;
     grid_of(n, grid, block);
-    timed(K_CFL, n[0] * n[1] * n[2], [&] {
-      hipLaunchKernelGGL((k_cfl<T>), grid, block, 0, s, c, int(n[1]), int(n[2]), red);
+    const int64_t nb = int64_t(grid.x) * grid.y * grid.z;
+    if (nb > parts_cap) {
+      if (parts) note(hipFree(parts));
+      parts = nullptr;
+      note(hipMalloc(reinterpret_cast<void**>(&parts), sizeof(unsigned long long) * 3 * size_t(nb)));
+      parts_cap = nb;
+    }
+    parts_n = nb;
+    timed(Ph::id, n[0] * n[1] * n[2], [&] {
+      hipLaunchKernelGGL((k_phase_max<Ph, T>), grid, block, 0, s, c, int(n[1]), int(n[2]), parts);
     });
+    note(hipGetLastError());
+  }
+
+  // the maxima of the last run_max (waits for it)
+  void max_partials(double out[3]) {
+    hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(256), 0, s, parts, parts_n, red);
     note(hipGetLastError());
     unsigned long long h[3] = {0, 0, 0};
     note(hipMemcpyAsync(h, red, sizeof(h), hipMemcpyDeviceToHost, s));
     note(hipStreamSynchronize(s));
     for (int d = 0; d < 3; ++d) std::memcpy(&out[d], &h[d], sizeof(double));
+  }
+
+  template <class T>
+  void cfl(const Ctx<T>& c, double out[3]) {
+    run_max<CflSpeeds>(c);
+    max_partials(out);
   }
 
   double* part = nullptr;
@@ -385,6 +463,26 @@ struct HipBatchExec {
     tune_id(Ph::id, [&](int bk, int bi, int grid) { launch<Ph, T>(bk, bi, grid); });
   }
 
+  template <class Ph, class T>
+  void launch_max(int bk, int bi, int grid) {
+    const Ctx<T>* dctx = static_cast<const Ctx<T>*>(lay->dctx);
+    const int* prefix = lay->dprefix + Ph::id * (lay->nbox + 1);
+    ex->timed(Ph::id, lay->elems[Ph::id], [&] {
+      hipLaunchKernelGGL((k_batch_max<Ph, T>), dim3(unsigned(grid)), dim3(unsigned(bk * bi)), 0, ex->s, dctx,
+                         prefix, lay->nbox, bk, bi, lay->dred);
+    });
+    ex->note(hipGetLastError());
+  }
+
+  // per-box speed maxima accumulate into lay->dred for the next cycle's begin()
+  template <class Ph, class T>
+  void run_max(const Ctx<T>&) {
+    if (tuning) tune_id(Ph::id, [&](int bk, int bi, int grid) { launch_max<Ph, T>(bk, bi, grid); });
+    int bk, bi, grid;
+    launch_cfg(Ph::id, bk, bi, grid);
+    launch_max<Ph, T>(bk, bi, grid);
+  }
+
   template <class T>
   void cfl_launch(int bk, int bi, int grid) {
     const Ctx<T>* dctx = static_cast<const Ctx<T>*>(lay->dctx);
@@ -449,6 +547,7 @@ struct HipBackend {
     if (ex.s) hipStreamSynchronize(ex.s);
     if (ex.red) hipFree(ex.red);
     if (ex.part) hipFree(ex.part);
+    if (ex.parts) hipFree(ex.parts);
     if (ex.s) hipStreamDestroy(ex.s);
   }
   HipExec& exec() { return ex; }

@@ -50,6 +50,7 @@ template <class T>
 struct Ctx {
   int ndim;
   int active;        // 0: a batch box that has finished (its launches are skipped)
+  int physics;       // 0 ideal MHD (8 cell variables + face B), 1 hydrodynamics (5 variables)
   int64_t ny, nx, nz;
   int bc[3];         // boundary condition per direction (x, y, z)
   int bc_zrec;       // boundary condition the z reconstruction/corrections use
@@ -202,19 +203,27 @@ PMHD_FI void store7(T* A, int64_t NF, int64_t f, const T v[7]) {
 // ===========================================================================
 // CFL speeds of one cell
 // ===========================================================================
+// of the state (U, fb): the current one, or the one an update has just written
 template <class T>
-PMHD_FI void k_cfl(const Ctx<T>& c, I3 p, T out[3]) {
+PMHD_FI void k_cfl(const Ctx<T>& c, const T* U, const T* const* fb, I3 p, T out[3]) {
   const int64_t N = c.ncell();
   const int64_t idx = c.cid(p.j, p.i, p.k);
+  if (c.physics == 1) {
+    T h[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) h[v] = U[v * N + idx];
+    hydro::cell_speeds(h, c.gamma, out);
+    return;
+  }
   T u[8];
 #pragma unroll
-  for (int v = 0; v < 8; ++v) u[v] = c.U[v * N + idx];
+  for (int v = 0; v < 8; ++v) u[v] = U[v * N + idx];
   T bf[3] = {T(0), T(0), T(0)};
   if (c.ndim >= 2) {
-    bf[0] = c.fb[0][c.fid(0, p.j, p.i, p.k)];
-    bf[1] = c.fb[1][c.fid(1, p.j, p.i, p.k)];
+    bf[0] = fb[0][c.fid(0, p.j, p.i, p.k)];
+    bf[1] = fb[1][c.fid(1, p.j, p.i, p.k)];
   }
-  if (c.ndim == 3) bf[2] = c.fb[2][c.fid(2, p.j, p.i, p.k)];
+  if (c.ndim == 3) bf[2] = fb[2][c.fid(2, p.j, p.i, p.k)];
   cell_speeds(u, bf, c.ndim, c.gamma, out);
 }
 
@@ -857,14 +866,15 @@ PMHD_FI void k_diag(const Ctx<T>& c, int64_t idx, double sums[9], double maxes[2
   const int64_t N = c.ncell();
   I3 p = unflat(idx, c.ny, c.nx, c.nz);
   double bad = 0.0;
+  const int nv = c.physics == 1 ? 5 : 8;
   for (int v = 0; v < 8; ++v) {
-    double u = double(c.U[v * N + idx]);
+    double u = v < nv ? double(c.U[v * N + idx]) : 0.0;
     sums[v] = u;
     if (!(u == u) || u - u != 0.0) bad += 1.0;
   }
   sums[8] = bad;
   double div = 0.0, bmax = 0.0, wmin = c.wd[0];
-  if (c.ndim >= 2) {
+  if (c.ndim >= 2 && c.physics == 0) {
     double fx0 = c.fb[0][c.fid(0, p.j, p.i, p.k)], fx1 = c.fb[0][c.fid(0, p.j, p.i + 1, p.k)];
     double fy0 = c.fb[1][c.fid(1, p.j, p.i, p.k)], fy1 = c.fb[1][c.fid(1, p.j + 1, p.i, p.k)];
     div = (fx1 - fx0) / c.wd[0] + (fy1 - fy0) / c.wd[1];

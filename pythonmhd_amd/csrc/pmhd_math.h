@@ -93,6 +93,21 @@ PMHD_FI void apply_floors(T v[7], T min_dens, T min_last) {
 // b_cell: cell B (cons[4..6]); b_face: left-face B of the cell (ignored when
 // use_face == false, as in the 1D step).
 // ---------------------------------------------------------------------------
+namespace hydro {
+// per-cell hydro CFL speeds |v_d| + c_s (Source/TimestepCalculations_HelperModule.py:24-170)
+template <class T>
+PMHD_FI void cell_speeds(const T u[5], T gamma, T out[3]) {
+  const T inv = T(1) / u[0];
+  const T vx = u[1] * inv, vy = u[2] * inv, vz = u[3] * inv;
+  const T vsq = vx * vx + vy * vy + vz * vz;
+  const T pres = npmax((gamma - T(1)) * (u[4] - T(0.5) * u[0] * vsq), T(kSmall));
+  const T cs = sqrt_(gamma * pres * inv);
+  out[0] = npabs(vx) + cs;
+  out[1] = npabs(vy) + cs;
+  out[2] = npabs(vz) + cs;
+}
+}  // namespace hydro
+
 template <class T>
 PMHD_FI T fast_speed_sq(T bsq, T inv, T cs2, T btsq) {
   T ssum = bsq * inv + cs2;

@@ -2,6 +2,8 @@
 // with a launch id, the kind of element it runs over (cells, faces, slope cells or
 // edge points) and a direction; an executor X runs it over every element:
 //   X::run<Ph>(c)              -- Ph()(c, I3{j, i, k}) over the phase's index box
+//   X::run_max<Ph>(c)          -- the same for a phase that also yields three speeds
+//                                 per element, maximised (kept for the next dt)
 //   X::cfl(c, out)             -- out[d] = max over cells of the CFL speed along d
 //   X::reduce(id, n, f, sums, maxes)
 // The single-grid executors take c as the grid; the batch executors run the same
@@ -86,20 +88,31 @@ struct CtFull {  // full-step face field
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_ct(c, D, c.fb[D], c.fb2[D], c.dtd, p); }
 };
+template <class T>
+PMHD_FI void speeds_out(const T t[3], double o[3]) {
+  o[0] = double(t[0]);
+  o[1] = double(t[1]);
+  o[2] = double(t[2]);
+}
+// conservative update, then the CFL speeds of the new state for the next
+// cycle's timestep (maximised by X::run_max)
 struct Update {
   PMHD_PHASE_TRAITS(K_UPDATE, EXT_CELL, 0)
   template <class T>
-  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_update(c, p); }
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p, double o[3]) const {
+    k_update(c, p);
+    T t[3];
+    k_cfl(c, c.U2, c.fb2, p, t);
+    speeds_out(t, o);
+  }
 };
-struct CflSpeeds {  // per-cell speeds, maximised by X::cfl
+struct CflSpeeds {  // per-cell speeds of the current state, maximised by X::cfl
   PMHD_PHASE_TRAITS(K_CFL, EXT_CELL, 0)
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 p, double o[3]) const {
     T t[3];
-    k_cfl(c, p, t);
-    o[0] = double(t[0]);
-    o[1] = double(t[1]);
-    o[2] = double(t[2]);
+    k_cfl(c, c.U, c.fb, p, t);
+    speeds_out(t, o);
   }
 };
 
@@ -138,7 +151,7 @@ void cycle(X& ex, const Ctx<T>& c) {
   if (nd >= 2) predictor_sweep<1>(ex, c);
   if (nd == 3) predictor_sweep<2>(ex, c);
   if (nd == 1) {
-    ex.template run<Update>(c);
+    ex.template run_max<Update>(c);
     return;
   }
   ex.template run<CellE>(c);
@@ -148,7 +161,7 @@ void cycle(X& ex, const Ctx<T>& c) {
   each_dir<Correct>(ex, c);           // CTU transverse corrections + corrector fluxes
   ex.template run<EmfCorrector>(c);   // corner EMFs of the corrector fluxes
   each_dir<CtFull>(ex, c);            // full-step face field
-  ex.template run<Update>(c);
+  ex.template run_max<Update>(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -161,10 +174,28 @@ PMHD_FI double dt_from_maxima(int ndim, double cfl, const double m[3], const dou
   return cfl / npmax(m[0] / w[0], npmax(m[1] / w[1], m[2] / w[2]));
 }
 
+namespace hydro {
+// the minimum of the per-direction dt (Source/TimestepCalculations_HelperModule.py:35-170)
+PMHD_FI double dt_from_maxima(int ndim, double cfl, const double m[3], const double w[3]) {
+  const double dx = cfl / (m[0] / w[0]);
+  if (ndim == 1) return dx;
+  const double dy = cfl / (m[1] / w[1]);
+  if (ndim == 2) return npmin(dx, dy);
+  const double dz = cfl / (m[2] / w[2]);
+  return npmin(dx, npmin(dy, dz));
+}
+}  // namespace hydro
+
+template <class T>
+PMHD_FI double dt_of(const Ctx<T>& c, double cfl, const double m[3]) {
+  return c.physics == 1 ? hydro::dt_from_maxima(c.ndim, cfl, m, c.wd) : dt_from_maxima(c.ndim, cfl, m, c.wd);
+}
+
 // the reference's end-of-interval clipping (Source/EvolveFunctions_MHD.py:259-260,
-// 472-473 for 1D/2D; :851-852 for 3D)
-PMHD_FI double clip_dt(int ndim, double t, double t_end, double dt) {
-  if (ndim == 3) {
+// 472-473 for 1D/2D MHD; :851-852 for 3D MHD; Source/EvolveFunctions_Hydro.py:268-269,
+// 398-399, 616-617 for hydro)
+PMHD_FI double clip_dt(int ndim, int physics, double t, double t_end, double dt) {
+  if (ndim == 3 && physics == 0) {
     if (t_end - t < dt) dt = t_end - t;
   } else {
     if (t + dt > t_end) dt = t_end - t;
@@ -267,15 +298,15 @@ PMHD_FI void box_begin(Ctx<T>& c, BoxClock& k, const double m[3], double cfl, do
     c.active = 0;
     return;
   }
-  double dt = dt_from_maxima(c.ndim, cfl, m, c.wd);
+  double dt = dt_of(c, cfl, m);
   if (commit) {
-    dt = clip_dt(c.ndim, k.t, t_end, dt);
+    dt = clip_dt(c.ndim, c.physics, k.t, t_end, dt);
     k.t2 = k.t;
     k.dt = dt;
   }
   c.active = 1;
   set_step_scalars(c, dt);
-  if (sizeof(T) == sizeof(double) && c.ndim == 3) half_ratio_80(dt, c.wd[1], c.ldh);
+  if (sizeof(T) == sizeof(double) && c.ndim == 3 && c.physics == 0) half_ratio_80(dt, c.wd[1], c.ldh);
 }
 
 template <class T>

@@ -1,6 +1,6 @@
 """Drop-in evolve functions with PythonMHD's signatures and return tuples
-(Source/EvolveFunctions_MHD.py:195, :393, :750), running the step on the MI355X
-through libpmhd (include/pmhd.h).
+(Source/EvolveFunctions_MHD.py:195, :393, :750; Source/EvolveFunctions_Hydro.py:232,
+:350, :559), running the step on the MI355X through libpmhd (include/pmhd.h).
 
 Each function reads the duck-typed grid attributes the reference reads
 (primVars, consVars, faceBx/faceBy/faceBz, dx/dy/dz, BcX/BcY/BcZ), advances from
@@ -13,7 +13,7 @@ keyword ``precision=32`` runs the FP32-bulk step.
 import numpy as np
 
 from . import constants as C
-from ._lib import Solver
+from ._lib import HYDRO, MHD, Solver
 
 _solver_cache = {}
 
@@ -40,11 +40,12 @@ def _check_common(tStart, tEnd, startCycle):
 
 
 def _run(shape, widths, bcs, prim, cons, faces, tStart, tEnd, startCycle, maxCycles, gamma, cfl,
-         minDens, minPres, minEnergy, reconstructOrder, entropyFix, epsilon, precision):
+         minDens, minPres, minEnergy, reconstructOrder, entropyFix, epsilon, precision, physics=MHD):
     key = (shape, tuple(widths), tuple(bcs), float(gamma), float(cfl), float(minDens), float(minPres),
-           float(minEnergy), int(reconstructOrder), bool(entropyFix), float(epsilon), int(precision))
+           float(minEnergy), int(reconstructOrder), bool(entropyFix), float(epsilon), int(precision),
+           int(physics))
     s = _solver(key, lambda: Solver(shape, widths, bcs, gamma, cfl, minDens, minPres, minEnergy,
-                                    reconstructOrder, entropyFix, epsilon, precision))
+                                    reconstructOrder, entropyFix, epsilon, precision, physics=physics))
     s.set_state(prim, cons, faces)
     t, cyc, t2 = s.advance(tStart, tEnd, startCycle, maxCycles)
     p, c, f = s.get_state(0)
@@ -112,3 +113,57 @@ def evolve_3D_mhd(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cf
         maxCycles, gamma, cfl, minDens, minPres, minEnergy, reconstructOrder, entropyFix, epsilon,
         precision)
     return p, c, f[0], f[1], f[2], t, cyc, p2, c2, f2[0], f2[1], f2[2], t2
+
+
+# ---------------------------------------------------------------------------
+# hydrodynamics: five variables (rho, v, p) / (rho, M, E), no face field; the
+# same tuple in 1D, 2D and 3D (Source/EvolveFunctions_Hydro.py:305, 516, 806)
+# ---------------------------------------------------------------------------
+def _hydro(simulationGrid, nd, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
+           minEnergy, reconstructOrder, entropyFix, epsilon, precision):
+    prim = np.asarray(simulationGrid.primVars, dtype=np.float64)
+    assert prim.ndim == nd + 1 and prim.shape[0] == 5
+    cons = np.asarray(simulationGrid.consVars, dtype=np.float64)
+    assert prim.shape == cons.shape
+    names = ('dx', 'dy', 'dz')[:nd]
+    bcn = ('BcX', 'BcY', 'BcZ')[:nd]
+    if nd >= 2:
+        # the reference reads the cell counts from the grid (EvolveFunctions_Hydro.py:363-368)
+        assert simulationGrid.numXCells > 1
+        assert simulationGrid.numYCells > 1
+        if nd == 3:
+            assert simulationGrid.numZCells > 1
+    widths = tuple(getattr(simulationGrid, n) for n in names)
+    for w in widths:
+        assert w > 0
+    bcs = tuple(getattr(simulationGrid, n) for n in bcn)
+    for b in bcs:
+        assert b == C.OUTFLOW or b == C.PERIODIC
+    _check_common(tStart, tEnd, startCycle)
+    assert maxCycles > 0
+    shape = prim.shape[1:]
+    p, c, _, t, cyc, p2, c2, _, t2 = _run(
+        shape, widths, bcs, prim, cons, (), tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens,
+        minPres, minEnergy, reconstructOrder, entropyFix, epsilon, precision, physics=HYDRO)
+    return p, c, t, cyc, p2, c2, t2
+
+
+def evolve_1D_hydro(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
+                    minEnergy, reconstructOrder=0, entropyFix=False, epsilon=0.5,
+                    precision=C.DEFAULT_PRECISION):
+    return _hydro(simulationGrid, 1, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
+                  minEnergy, reconstructOrder, entropyFix, epsilon, precision)
+
+
+def evolve_2D_hydro(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
+                    minEnergy, reconstructOrder=0, entropyFix=False, epsilon=0.5,
+                    precision=C.DEFAULT_PRECISION):
+    return _hydro(simulationGrid, 2, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
+                  minEnergy, reconstructOrder, entropyFix, epsilon, precision)
+
+
+def evolve_3D_hydro(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
+                    minEnergy, reconstructOrder=0, entropyFix=False, epsilon=0.5,
+                    precision=C.DEFAULT_PRECISION):
+    return _hydro(simulationGrid, 3, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
+                  minEnergy, reconstructOrder, entropyFix, epsilon, precision)

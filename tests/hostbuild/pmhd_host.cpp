@@ -23,27 +23,49 @@ void host_each(const Ctx<T>& c) {
       for (int64_t k = 0; k < n[2]; ++k) Ph()(c, I3{j, i, k});
 }
 
+inline void max_bits(uint64_t m[3], const double o[3]) {
+  for (int d = 0; d < 3; ++d) {
+    uint64_t b;
+    std::memcpy(&b, &o[d], 8);
+    if (b > m[d]) m[d] = b;
+  }
+}
+
+// a phase that also yields speeds, maximised over its elements (serially, so the
+// phase's writes need no other ordering)
+template <class Ph, class T>
+void host_each_max(const Ctx<T>& c, uint64_t m[3]) {
+  int64_t n[3];
+  c.extent(Ph::kind, Ph::dir, n);
+  for (int64_t j = 0; j < n[0]; ++j)
+    for (int64_t i = 0; i < n[1]; ++i)
+      for (int64_t k = 0; k < n[2]; ++k) {
+        double o[3];
+        Ph()(c, I3{j, i, k}, o);
+        max_bits(m, o);
+      }
+}
+
 template <class T>
 void host_cfl(const Ctx<T>& c, double out[3]) {
   uint64_t m[3] = {0, 0, 0};
-  for (int64_t j = 0; j < c.ny; ++j)
-    for (int64_t i = 0; i < c.nx; ++i)
-      for (int64_t k = 0; k < c.nz; ++k) {
-        double o[3];
-        CflSpeeds()(c, I3{j, i, k}, o);
-        for (int d = 0; d < 3; ++d) {
-          uint64_t b;
-          std::memcpy(&b, &o[d], 8);
-          if (b > m[d]) m[d] = b;
-        }
-      }
+  host_each_max<CflSpeeds>(c, m);
   for (int d = 0; d < 3; ++d) std::memcpy(&out[d], &m[d], 8);
 }
 
 struct HostExec {
+  uint64_t last[3] = {0, 0, 0};
   template <class Ph, class T>
   void run(const Ctx<T>& c) {
     host_each<Ph>(c);
+  }
+  template <class Ph, class T>
+  void run_max(const Ctx<T>& c) {
+    for (int d = 0; d < 3; ++d) last[d] = 0;
+    host_each_max<Ph>(c, last);
+  }
+  void max_partials(double out[3]) {
+    for (int d = 0; d < 3; ++d) std::memcpy(&out[d], &last[d], 8);
   }
   template <class T>
   void cfl(const Ctx<T>& c, double out[3]) {
@@ -93,6 +115,19 @@ struct HostBatchExec {
   }
 
   template <class Ph, class T>
+  void run_max(const Ctx<T>& c0) {
+    if (tuning) {
+      const std::vector<TileCfg> cands = tile_candidates(*lay, Ph::id);
+      for (const TileCfg& t : cands)
+        if (lay->apply_tile(Ph::id, t)) walk<Ph, T, true>();
+      (void)lay->apply_tile(Ph::id, cands[size_t(Ph::id) % cands.size()]);
+      lay->tuned_ms[Ph::id] = 0.f;
+    }
+    (void)c0;
+    walk<Ph, T, true>();
+  }
+
+  template <class Ph, class T, bool kMax = false>
   void walk() {
     const TileCfg t = lay->tiles[Ph::id];
     const int total = lay->total[Ph::id];
@@ -110,7 +145,7 @@ struct HostBatchExec {
           const TileRef r = locate_tile(ctxs<T>(), prefix, lay->nbox, vb, Ph::kind, Ph::dir, t.bk, t.bi, tid);
           const Ctx<T>& c = ctxs<T>()[r.box];
           if (!c.active || !r.in) continue;
-          Ph()(c, I3{r.j, r.i, r.k});
+          call<Ph, kMax>(c, I3{r.j, r.i, r.k}, lay->dred + 3 * r.box);
           int64_t n[3];
           c.extent(Ph::kind, Ph::dir, n);
           seen[size_t(r.box)][size_t((int64_t(r.j) * n[1] + r.i) * n[2] + r.k)] += 1;
@@ -121,6 +156,20 @@ struct HostBatchExec {
         if (v != 1) throw std::runtime_error(std::string("tile walk of ") + kernel_name(Ph::id) + " not exactly once");
     }
   }
+  template <class Ph, bool kMax, class T>
+  static void call(const Ctx<T>& c, I3 p, unsigned long long* red) {
+    if constexpr (kMax) {
+      double o[3];
+      Ph()(c, p, o);
+      uint64_t m[3] = {red[0], red[1], red[2]};
+      max_bits(m, o);
+      for (int d = 0; d < 3; ++d) red[d] = m[d];
+    } else {
+      (void)red;
+      Ph()(c, p);
+    }
+  }
+
   template <class T>
   void cfl(const Ctx<T>&) {
     for (int b = 0; b < lay->nbox; ++b) {

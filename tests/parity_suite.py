@@ -258,3 +258,87 @@ def test_outflow_ghost_rows_in_bounds(lib, shape, bcs, order):
     same(r[1], o[1])
     for a, e in zip(r[2], o[2]):
         same(a, e)
+
+
+# ---------------------------------------------------------------------------
+# hydrodynamics: golden trajectories of evolve_{1,2,3}D_hydro, bit for bit
+# ---------------------------------------------------------------------------
+from tests.test_hydro_oracle import EVOLVE as HYDRO_EVOLVE, evolve_case as hydro_case  # noqa: E402
+from oracle import hydro as H  # noqa: E402
+
+
+@pytest.mark.parametrize('tag', [c[0] for c in HYDRO_EVOLVE])
+def test_hydro_golden(lib, tag):
+    from pythonmhd_amd._lib import HYDRO
+    p0, widths, bcs, tend, ncyc, P, want = hydro_case(tag)
+    shape = p0.shape[1:]
+    print('case: hydro', tag, flush=True)
+    s = Solver()(shape, widths, bcs, P.gamma, P.cfl, order=P.order, entropy_fix=P.entropy_fix, lib=lib,
+                 physics=HYDRO)
+    s.set_state(p0, H.prim_to_cons(p0, P.gamma, 0))
+    t, cyc, t2 = s.advance(0.0, tend, 0, ncyc)
+    p, c, _ = s.get_state(0)
+    p2, c2, _ = s.get_state(1)
+    s.close()
+    for a, k in ((p, 'prim'), (c, 'cons'), (t, 't'), (cyc, 'cyc'), (p2, 'prim2'), (c2, 'cons2'), (t2, 't2')):
+        same(a, want[('prim', 'cons', 't', 'cyc', 'prim2', 'cons2', 't2').index(k)])
+
+
+@pytest.mark.parametrize('nd,order,bcs', [(1, 1, (1,)), (2, 1, (1, 0)), (3, 1, (0, 1, 1)), (1, 2, (1,)),
+                                          (3, 2, (0, 0, 0))])
+def test_hydro_oracle_fp64_and_fp32(lib, nd, order, bcs):
+    """PLM (no reference counterpart) and the deviations where the reference raises
+    (1D periodic PPM, 3D outflow x) against the hydro oracle: FP64 bit for bit,
+    FP32 within 1e-5 in density L1."""
+    from pythonmhd_amd._lib import HYDRO
+    rng = np.random.default_rng(70 + nd + 10 * order)
+    shape = {1: (40,), 2: (14, 18), 3: (8, 10, 12)}[nd]
+    g = np.meshgrid(*[np.arange(n) / n for n in shape], indexing='ij')
+    p0 = np.empty((5,) + shape)
+    p0[0] = 1.0 + 0.2 * np.sin(2 * np.pi * sum(g)) + 0.05 * rng.random(shape)
+    for k in (1, 2, 3):
+        p0[k] = 0.3 * np.cos(2 * np.pi * (g[0] + k * g[-1]))
+    p0[4] = 1.0 + 0.1 * rng.random(shape)
+    widths = tuple(1.0 / n for n in ((shape[0],) if nd == 1 else ((shape[1], shape[0]) if nd == 2
+                                                                   else (shape[1], shape[0], shape[2]))))
+    P = H.Params(1.4, 0.5, order=order, entropy_fix=False)
+    o = H.evolve(p0.copy(), H.prim_to_cons(p0, 1.4, 0), widths, bcs, 0.0, 10.0, 0, 4, P)
+    print('case: hydro oracle', nd, order, bcs, flush=True)
+    for prec in (64, 32):
+        s = Solver()(shape, widths, bcs, 1.4, 0.5, order=order, lib=lib, physics=HYDRO, precision=prec)
+        s.set_state(p0, H.prim_to_cons(p0, 1.4, 0))
+        s.advance(0.0, 10.0, 0, 4)
+        p, c, _ = s.get_state(0)
+        s.close()
+        if prec == 64:
+            same(c, o[1])
+        else:
+            assert float(np.mean(np.abs(c[0] - o[1][0]))) < 1e-5
+
+
+def test_hydro_batch_matches_single(lib):
+    from pythonmhd_amd._lib import HYDRO, Batch
+    shapes = [(10, 12), (16, 9), (7, 14)]
+    states = []
+    rng = np.random.default_rng(5)
+    for s_ in shapes:
+        p0 = np.empty((5,) + s_)
+        p0[0] = 1.0 + 0.3 * rng.random(s_)
+        p0[1:4] = 0.2 * rng.standard_normal((3,) + s_)
+        p0[4] = 1.0 + 0.3 * rng.random(s_)
+        states.append(p0)
+    widths = [(1.0 / s_[1], 1.0 / s_[0]) for s_ in shapes]
+    bcs = [(1, 1), (0, 1), (0, 0)]
+    b = Batch(shapes, widths, bcs, 1.4, 0.6, order=2, lib=lib, physics=HYDRO)
+    for k, p0 in enumerate(states):
+        b.set_state(k, p0, H.prim_to_cons(p0, 1.4, 0))
+    t, cyc, t2 = b.advance(0.0, 0.05, 0, 5)
+    for k, p0 in enumerate(states):
+        s = Solver()(shapes[k], widths[k], bcs[k], 1.4, 0.6, order=2, lib=lib, physics=HYDRO)
+        s.set_state(p0, H.prim_to_cons(p0, 1.4, 0))
+        tt, cc, _ = s.advance(0.0, 0.05, 0, 5)
+        same(b.get_state(k, 0)[1], s.get_state(0)[1])
+        same(t[k], tt)
+        same(cyc[k], cc)
+        s.close()
+    b.close()
