@@ -84,18 +84,20 @@ __global__ void __launch_bounds__(256) k_phase_max(const Ctx<T> c, int n1, int n
   }
 }
 
-// one block of 256 threads: out[d] = max over the nparts partial maxima
+// out[d] = max over the nparts partial maxima: each block reduces a strided
+// subset, then one atomic per block and component (out cleared beforehand)
 __global__ void __launch_bounds__(256) k_max_parts(const unsigned long long* part, int64_t nparts,
                                                    unsigned long long* out) {
   unsigned long long m[3] = {0ull, 0ull, 0ull};
-  for (int64_t b = threadIdx.x; b < nparts; b += blockDim.x) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; b < nparts; b += stride) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) m[d] = umax64(m[d], part[3 * b + d]);
   }
   block_max3(m);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) out[d] = m[d];
+    for (int d = 0; d < 3; ++d) atomicMax(&out[d], m[d]);
   }
 }
 
@@ -357,7 +359,10 @@ struct HipExec {
 
   // the maxima of the last run_max (waits for it)
   void max_partials(double out[3]) {
-    hipLaunchKernelGGL(k_max_parts, dim3(1), dim3(256), 0, s, parts, parts_n, red);
+    note(hipMemsetAsync(red, 0, 3 * sizeof(unsigned long long), s));
+    int64_t nb = (parts_n + 255) / 256;
+    nb = nb < 1 ? 1 : (nb > 128 ? 128 : nb);
+    hipLaunchKernelGGL(k_max_parts, dim3(unsigned(nb)), dim3(256), 0, s, parts, parts_n, red);
     note(hipGetLastError());
     unsigned long long h[3] = {0, 0, 0};
     note(hipMemcpyAsync(h, red, sizeof(h), hipMemcpyDeviceToHost, s));

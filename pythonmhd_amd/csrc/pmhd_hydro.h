@@ -337,13 +337,13 @@ PMHD_FI void roe_flux(const T wl[NV], const T wr[NV], T g, bool efix, T eps, int
   }
 }
 
-template <class T>
-PMHD_FI void load5(const T* A, int64_t N, int64_t i, T v[NV]) {
+template <class P, class T>
+PMHD_FI void load5(const P& A, int64_t N, int64_t i, T v[NV]) {
 #pragma unroll
   for (int r = 0; r < NV; ++r) v[r] = A[r * N + i];
 }
-template <class T>
-PMHD_FI void store5(T* A, int64_t N, int64_t i, const T v[NV]) {
+template <class P, class T>
+PMHD_FI void store5(const P& A, int64_t N, int64_t i, const T v[NV]) {
 #pragma unroll
   for (int r = 0; r < NV; ++r) A[r * N + i] = v[r];
 }
@@ -502,7 +502,7 @@ PMHD_FI void k_update(const Ctx<T>& c, I3 p) {
   T u[NV];
   load5(c.U, N, idx, u);
   for (int d = 0; d < c.ndim; ++d) {
-    const T* Fd = c.ndim >= 2 ? c.Fc[d] : c.F[d];
+    const auto Fd = c.ndim >= 2 ? c.Fc[d] : c.F[d];
     const int64_t NF = c.nface(d);
     const I3 hi = with(p, d, co(p, d) + 1);
     const int64_t f0 = c.fid(d, p.j, p.i, p.k), f1 = c.fid(d, hi.j, hi.i, hi.k);

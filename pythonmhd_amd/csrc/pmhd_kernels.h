@@ -46,6 +46,17 @@ inline void phase_shape(int id, int& kind, int& dir) {
   }
 }
 
+// pointer types of a context's arrays (the adjoint's dual contexts use accessors)
+template <class T>
+struct Mem {
+  using cp = const T*;
+  using p = T*;
+};
+template <class T>
+using CPtr = typename Mem<T>::cp;
+template <class T>
+using Ptr = typename Mem<T>::p;
+
 template <class T>
 struct Ctx {
   int ndim;
@@ -68,23 +79,23 @@ struct Ctx {
   T bx1d;            // uniform B_x of the 1D step
   double ldh[2];     // 0.5*dt/dy as the reference's 80-bit value, hi + lo (FP64 only)
   // state
-  const T* U;
-  const T* W;
-  T* U2;
-  T* W2;
-  const T* fb[3];
-  T* fb2[3];
-  T* nfb[3];
+  CPtr<T> U;
+  CPtr<T> W;
+  Ptr<T> U2;
+  Ptr<T> W2;
+  CPtr<T> fb[3];
+  Ptr<T> fb2[3];
+  Ptr<T> nfb[3];
   // work
-  T* WL;
-  T* WR;
-  T* S;
-  T* UL[3];
-  T* UR[3];
-  T* F[3];           // predictor fluxes
-  T* Fc[3];          // corrector fluxes
-  T* Ec;
-  T* E[3];
+  Ptr<T> WL;
+  Ptr<T> WR;
+  Ptr<T> S;
+  Ptr<T> UL[3];
+  Ptr<T> UR[3];
+  Ptr<T> F[3];           // predictor fluxes
+  Ptr<T> Fc[3];          // corrector fluxes
+  Ptr<T> Ec;
+  Ptr<T> E[3];
   // element counts
   PMHD_FI int64_t ncell() const { return ny * nx * nz; }
   PMHD_FI int64_t n_along(int d) const { return d == 0 ? nx : (d == 1 ? ny : nz); }
@@ -174,8 +185,8 @@ PMHD_FI int rot_to_dir(int d, int r) {
   }
 }
 
-template <class T>
-PMHD_FI void load_rot8(const T* A, int64_t N, int64_t c, int d, T w[7]) {
+template <class P, class T>
+PMHD_FI void load_rot8(const P& A, int64_t N, int64_t c, int d, T w[7]) {
 #pragma unroll
   for (int r = 0; r < 7; ++r) w[r] = A[rot8(d, r) * N + c];
 }
@@ -189,13 +200,13 @@ PMHD_FI void from_rot(const T rot[7], int d, T dir[7]) {
 #pragma unroll
   for (int r = 0; r < 7; ++r) dir[rot_to_dir(d, r)] = rot[r];
 }
-template <class T>
-PMHD_FI void load7(const T* A, int64_t NF, int64_t f, T v[7]) {
+template <class P, class T>
+PMHD_FI void load7(const P& A, int64_t NF, int64_t f, T v[7]) {
 #pragma unroll
   for (int r = 0; r < 7; ++r) v[r] = A[r * NF + f];
 }
-template <class T>
-PMHD_FI void store7(T* A, int64_t NF, int64_t f, const T v[7]) {
+template <class P, class T>
+PMHD_FI void store7(const P& A, int64_t NF, int64_t f, const T v[7]) {
 #pragma unroll
   for (int r = 0; r < 7; ++r) A[r * NF + f] = v[r];
 }
@@ -204,8 +215,8 @@ PMHD_FI void store7(T* A, int64_t NF, int64_t f, const T v[7]) {
 // CFL speeds of one cell
 // ===========================================================================
 // of the state (U, fb): the current one, or the one an update has just written
-template <class T>
-PMHD_FI void k_cfl(const Ctx<T>& c, const T* U, const T* const* fb, I3 p, T out[3]) {
+template <class T, class PU, class PF>
+PMHD_FI void k_cfl(const Ctx<T>& c, const PU& U, const PF* fb, I3 p, T out[3]) {
   const int64_t N = c.ncell();
   const int64_t idx = c.cid(p.j, p.i, p.k);
   if (c.physics == 1) {
@@ -327,8 +338,8 @@ PMHD_FI void k_copy_states(const Ctx<T>& c, int d, int bc, I3 f) {
 // predictor: floors, MHD source terms on the reconstructed transverse B,
 // prim -> cons, Roe flux
 // ===========================================================================
-template <class T>
-PMHD_FI T face_val(const Ctx<T>& c, const T* fb, int a, I3 p, int64_t fa) {
+template <class T, class P>
+PMHD_FI T face_val(const Ctx<T>& c, const P& fb, int a, I3 p, int64_t fa) {
   // value of face array of axis a at face coordinate fa along a, other coords of p
   I3 q = with(p, a, fa);
   return fb[c.fid(a, q.j, q.i, q.k)];
@@ -481,13 +492,13 @@ PMHD_FI void k_half_e(const Ctx<T>& c, I3 p) {
 // corner EMFs (Source/CornerEMF_HelperModule.py:440-952)
 // ===========================================================================
 template <class T>
-PMHD_FI T fluxv(const Ctx<T>& c, T* const* Fs, int d, int v, int64_t j, int64_t i, int64_t k) {
+PMHD_FI T fluxv(const Ctx<T>& c, const Ptr<T>* Fs, int d, int v, int64_t j, int64_t i, int64_t k) {
   return Fs[d][v * c.nface(d) + c.fid(d, j, i, k)];
 }
 
 // Fs: the flux set (predictor c.F or corrector c.Fc) the EMFs are formed from
 template <class T>
-PMHD_FI void k_emf(const Ctx<T>& c, T* const* Fs, I3 p) {
+PMHD_FI void k_emf(const Ctx<T>& c, const Ptr<T>* Fs, I3 p) {
   const int64_t N = c.ncell();
   const int64_t idx = c.eid(p.j, p.i, p.k);
   const int bx = c.bc[0], by = c.bc[1], bz = c.bc[2];
@@ -508,9 +519,9 @@ PMHD_FI void k_emf(const Ctx<T>& c, T* const* Fs, I3 p) {
     return;
   }
   const int64_t zl = edge_lo(p.k, c.nz, bz), zr = edge_hi(p.k, c.nz, bz);
-  const T* Exc = c.Ec;
-  const T* Eyc = c.Ec + N;
-  const T* Ezc = c.Ec + 2 * N;
+  const auto Exc = c.Ec;
+  const auto Eyc = c.Ec + N;
+  const auto Ezc = c.Ec + 2 * N;
   {  // E_x (getXCornerEMF_3D): faces y (G) and z (H)
     T b0 = fluxv(c, Fs, 2, 0, yr, xr, p.k), b5 = fluxv(c, Fs, 2, 5, yr, xr, p.k);
     T ub0 = fluxv(c, Fs, 2, 0, yl, xr, p.k), ub5 = fluxv(c, Fs, 2, 5, yl, xr, p.k);
@@ -619,7 +630,7 @@ PMHD_FI void ctu_apply_3d(const Ctx<T>& c, int d, int tc, I3 pcell, int64_t hi_o
   I3 b10 = with(b, t1, co(b, t1) + 1);
   I3 b01 = with(b, t2, co(b, t2) + 1);
   I3 b11 = with(b10, t2, co(b, t2) + 1);
-  const T* E = c.E[d];
+  const auto E = c.E[d];
   T e00 = E[c.eid(b.j, b.i, b.k)], e10 = E[c.eid(b10.j, b10.i, b10.k)];
   T e01 = E[c.eid(b01.j, b01.i, b01.k)], e11 = E[c.eid(b11.j, b11.i, b11.k)];
   T q = c.emfq[tc];
@@ -733,11 +744,11 @@ PMHD_FI void ctu_states(const Ctx<T>& c, int d, I3 f, T ul[7], T ur[7]) {
 // constrained transport of the face field (Source/EvolveFunctions_MHD.py:1003-1028,
 // :1082-1104 for 3D; :579-586, :643-649 for 2D)
 // ===========================================================================
-template <class T>
-PMHD_FI void k_ct(const Ctx<T>& c, int d, const T* fin, T* fout, const T h[3], I3 f) {
+template <class T, class PI, class PO>
+PMHD_FI void k_ct(const Ctx<T>& c, int d, const PI& fin, const PO& fout, const T h[3], I3 f) {
   const int64_t idx = c.fid(d, f.j, f.i, f.k);
   if (c.ndim == 2) {
-    const T* E = c.E[2];
+    const auto E = c.E[2];
     if (d == 0) {
       fout[idx] = fin[idx] - h[1] * (E[c.eid(f.j + 1, f.i, 0)] - E[c.eid(f.j, f.i, 0)]);
     } else {
@@ -745,9 +756,9 @@ PMHD_FI void k_ct(const Ctx<T>& c, int d, const T* fin, T* fout, const T h[3], I
     }
     return;
   }
-  const T* Ex = c.E[0];
-  const T* Ey = c.E[1];
-  const T* Ez = c.E[2];
+  const auto Ex = c.E[0];
+  const auto Ey = c.E[1];
+  const auto Ez = c.E[2];
   T a, b;
   if (d == 0) {
     a = h[2] * (Ey[c.eid(f.j, f.i, f.k + 1)] - Ey[c.eid(f.j, f.i, f.k)]);

@@ -29,6 +29,13 @@ constexpr double kSmall = 0.000000000001;        // Source/PythonMHD_Constants.p
 constexpr double kFourThirds = 1.333333333333333;  // Source/PythonMHD_Constants.py:8
 
 using N = double;  // precision of the wave-speed / normalisation islands
+// island type of a value type (dual numbers carry their island as dual doubles)
+template <class T>
+struct Island {
+  using type = double;
+};
+template <class T>
+using island_t = typename Island<T>::type;
 
 // NumPy element semantics: maximum/minimum propagate NaN and return the second
 // argument on ties (so the sign of zero follows NumPy); sign(0) is +0.
@@ -151,6 +158,7 @@ struct PrimEig {
 
 template <class T>
 PMHD_FI void eigen_prim(const T w[7], T bn, T gamma, PrimEig<T>& E) {
+  using N = island_t<T>;
   const T rho = w[0], vn = w[1], bt1 = w[4], bt2 = w[5], p = w[6];
   T btsq = bt1 * bt1 + bt2 * bt2;
   T bt = sqrt_(btsq);
@@ -431,6 +439,7 @@ PMHD_FI T entropy_fixed(T l, T eps) {
 template <class T>
 PMHD_HD inline void roe_flux(const T wl[7], const T wr[7], const T ul[7], const T ur[7], T bn,
                              T g, bool efix, T eps, T F[7]) {
+  using N = island_t<T>;
   const T rl = wl[0], rr = wr[0];
   T sl = sqrt_(rl), sr = sqrt_(rr);
   T isum = T(1) / (sl + sr);
