@@ -180,6 +180,127 @@ def run_kh(cycles=200, seq_cycles=50, nbox=256):
             "speedup_vs_sequential": v / vs}
 
 
+def grf(n, corr=0.1, seed=11):
+    """Zero-mean, unit-variance periodic Gaussian random field on n^2 cells with a
+    Gaussian correlation of length corr (spectral filter exp(-k^2 corr^2 / 2))."""
+    rng = np.random.default_rng(seed)
+    k1 = 2 * np.pi * np.fft.fftfreq(n, 1.0 / n)
+    ky, kx = np.meshgrid(k1, k1, indexing='ij')
+    f = np.real(np.fft.ifft2(np.fft.fft2(rng.standard_normal((n, n))) * np.exp(-0.5 * (kx * kx + ky * ky) * corr ** 2)))
+    f = f - f.mean()
+    return f / f.std()
+
+
+def adjoint_state(n, velocity=0.0, seed=11):
+    """rho = 1 + 0.05 * GRF, p = 1, B = (0.3, 0, 0); v = 0 (the configured entropy
+    mode) or, for the gradient check, a smooth velocity field of amplitude velocity."""
+    rho = 1.0 + 0.05 * grf(n, seed=seed)
+    prim = np.zeros((8, n, n))
+    prim[0] = rho
+    if velocity:
+        prim[1] = velocity * grf(n, corr=0.15, seed=seed + 1)
+        prim[2] = velocity * grf(n, corr=0.15, seed=seed + 2)
+    prim[4] = 0.3
+    prim[7] = 1.0
+    cons = prim.copy()
+    cons[1:4] = prim[1:4] * prim[0]
+    cons[7] = prim[7] / (GAMMA - 1.0) + 0.5 * prim[0] * (prim[1] ** 2 + prim[2] ** 2 + prim[3] ** 2) \
+        + 0.5 * (prim[4] ** 2 + prim[5] ** 2 + prim[6] ** 2)
+    faces = (np.full((n, n + 1), 0.3), np.zeros((n + 1, n)))
+    return prim, cons, faces
+
+
+def _ke_cotangent(c, area):
+    g = np.zeros_like(c)
+    g[0] = -0.5 * (c[1] ** 2 + c[2] ** 2 + c[3] ** 2) / c[0] ** 2 * area
+    g[1:4] = c[1:4] / c[0] * area
+    return g
+
+
+def _rho_gradient(gp, gc, prim):
+    """d loss / d rho_0 with cons built from prim (v, p, B held fixed)."""
+    v = prim[1:4]
+    return gp[0] + gc[0] + v[0] * gc[1] + v[1] * gc[2] + v[2] * gc[3] + 0.5 * (v * v).sum(0) * gc[7]
+
+
+def run_adjoint(n=256, steps=200, every=20, ncheck=32, check_steps=20):
+    """Loss = total kinetic energy after `steps` fixed-dt steps; gradient with respect
+    to the initial density through the checkpointed adjoint; timings against the
+    plain FP32 and FP64 steps of the same run."""
+    from pythonmhd_amd._lib import Solver
+    from pythonmhd_amd.adjoint import AdjointRun
+    h = 1.0 / n
+    area = h * h
+    prim, cons, faces = adjoint_state(n)
+    s = Solver((n, n), (h, h), (1, 1), GAMMA, 0.8, order=1, precision=64)
+    s.set_state(prim, cons, faces)
+    dt = s.compute_dt()
+    out = {"grid": "%d^2 periodic, PLM + Roe + CTU + CT" % n, "steps": steps, "checkpoint_every": every,
+           "dt": dt, "loss": "sum 0.5 rho v^2 dx dy after the last step"}
+    for prec in (32, 64):   # the plain step, same fixed dt
+        sp = Solver((n, n), (h, h), (1, 1), GAMMA, 0.8, order=1, precision=prec)
+        sp.set_state(prim, cons, faces)
+        sp.step(dt)
+        sp.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sp.step(dt)
+        sp.synchronize()
+        out["plain_fp%d_seconds" % prec] = time.perf_counter() - t0
+        sp.close()
+    grads = {}
+    for prec in (32, 64):
+        run = AdjointRun((n, n), (h, h), (1, 1), GAMMA, order=1, precision=prec, checkpoint_every=every)
+        t0 = time.perf_counter()
+        _, co, _ = run.forward(prim, cons, faces, dt, steps)
+        tf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        gp, gc, _ = run.backward(None, _ke_cotangent(co, area), None)
+        tb = time.perf_counter() - t0
+        run.close()
+        grads[prec] = _rho_gradient(gp, gc, prim)
+        out["fp%d" % prec] = {"forward_seconds": tf, "backward_seconds": tb,
+                              "loss": float(np.sum(0.5 * (co[1] ** 2 + co[2] ** 2 + co[3] ** 2) / co[0]) * area),
+                              "max_abs_grad": float(np.max(np.abs(grads[prec])))}
+        print('bench: adjoint fp%d forward %.2f s backward %.2f s' % (prec, tf, tb), file=sys.stderr, flush=True)
+    out["backward_over_plain_forward_fp32"] = out["fp32"]["backward_seconds"] / out["plain_fp32_seconds"]
+    out["backward_over_plain_forward_fp64"] = out["fp64"]["backward_seconds"] / out["plain_fp64_seconds"]
+    out["grad_fp32_vs_fp64_max_abs_diff"] = float(np.max(np.abs(grads[32] - grads[64])))
+    # central differences (eps = 1e-4) of the FP64 loss at seeded cells, on a smooth
+    # flow with velocity (the configured state is a stationary entropy mode whose
+    # gradient is zero up to rounding), over check_steps steps
+    prim2, cons2, faces2 = adjoint_state(n, velocity=0.2)
+    run = AdjointRun((n, n), (h, h), (1, 1), GAMMA, order=1, precision=64, checkpoint_every=every)
+    _, co, _ = run.forward(prim2, cons2, faces2, dt, check_steps)
+    gp, gc, _ = run.backward(None, _ke_cotangent(co, area), None)
+    g = _rho_gradient(gp, gc, prim2)
+    rng = np.random.default_rng(5)
+    cells = [tuple(int(x) for x in rng.integers(0, n, 2)) for _ in range(ncheck)]
+    errs = []
+    for (j, i) in cells:
+        vals = []
+        for sgn in (1, -1):
+            p3, c3 = prim2.copy(), cons2.copy()
+            p3[0, j, i] += sgn * 1e-4
+            c3[:, j, i] = 0
+            w = p3[:, j, i]
+            c3[0, j, i] = w[0]
+            c3[1:4, j, i] = w[1:4] * w[0]
+            c3[4:7, j, i] = w[4:7]
+            c3[7, j, i] = w[7] / (GAMMA - 1.0) + 0.5 * w[0] * (w[1:4] ** 2).sum() + 0.5 * (w[4:7] ** 2).sum()
+            _, cq, _ = run.forward(p3, c3, faces2, dt, check_steps)
+            vals.append(float(np.sum(0.5 * (cq[1] ** 2 + cq[2] ** 2 + cq[3] ** 2) / cq[0]) * area))
+        fd = (vals[0] - vals[1]) / 2e-4
+        errs.append((g[j, i], fd))
+    run.close()
+    gmax = float(np.max(np.abs(g)))
+    out["fd_check"] = {"state": "same rho, plus a smooth velocity field of rms 0.2", "precision": 64,
+                       "steps": check_steps, "cells": ncheck, "eps": 1e-4, "max_abs_grad": gmax,
+                       "max_err_over_max_grad": max(abs(a - b) for a, b in errs) / gmax,
+                       "median_rel_err": float(np.median([abs(a - b) / max(abs(b), 1e-300) for a, b in errs]))}
+    return out
+
+
 def cpu_baseline(n=64, cycles=2):
     """The FP64 NumPy oracle (single thread) on a bounded sample of the workload."""
     from oracle import mhd as M
@@ -235,6 +356,9 @@ def main():
     ap.add_argument('--no-fp64', action='store_true')
     ap.add_argument('--no-kh', action='store_true')
     ap.add_argument('--kh-cycles', type=int, default=200)
+    ap.add_argument('--no-adjoint', action='store_true')
+    ap.add_argument('--adj-steps', type=int, default=200)
+    ap.add_argument('--adj-n', type=int, default=256)
     ap.add_argument('--copy-probe', action='store_true',
                     help='only run a plain 1 GiB device copy three times (HBM counter calibration)')
     a = ap.parse_args()
@@ -288,6 +412,8 @@ def main():
         out["kh_batch"] = run_kh(cycles=a.kh_cycles)
         print('bench: KH batch %.3e cell-updates/s (sequential %.3e)' % (
             out["kh_batch"]["value"], out["kh_batch"]["sequential"]["value"]), file=sys.stderr, flush=True)
+    if not a.no_adjoint:
+        out["adjoint"] = run_adjoint(n=a.adj_n, steps=a.adj_steps)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         print('bench: cpu baseline %.3e cell-updates/s' % out["cpu_baseline"]["value"], file=sys.stderr,

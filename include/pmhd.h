@@ -150,6 +150,34 @@ int pmhd_batch_get_profile(pmhd_batch* b, int32_t max_entries, char* names, doub
                            int64_t* launches, int64_t* elements, int32_t* n_entries);
 const char* pmhd_batch_last_error(const pmhd_batch* b);
 
+/* ------------------------------------------------------------------------
+ * Adjoint of a run of fixed-dt steps, with checkpoint-and-recompute.  No
+ * counterpart in the reference (which has no gradients).  forward() runs nsteps
+ * cycles of the given dt from a state (the reference's loop body with dt fixed:
+ * no CFL choice, no clipping) and keeps a checkpoint every checkpoint_every
+ * steps; backward() maps the cotangent of the final state (prim, cons, faces;
+ * any pointer may be NULL for a zero cotangent) to the cotangent of the initial
+ * state of the last forward().  Arrays are host float64 in the reference layout.
+ * Cotangents are accumulated in float64 on the device.
+ * ------------------------------------------------------------------------ */
+typedef struct pmhd_adjoint pmhd_adjoint;
+
+int pmhd_adjoint_create(const pmhd_grid* grid, const pmhd_params* params, int32_t checkpoint_every,
+                        pmhd_adjoint** out);
+int pmhd_adjoint_destroy(pmhd_adjoint* a);
+int pmhd_adjoint_forward(pmhd_adjoint* a, const double* prim, const double* cons, const double* fbx,
+                         const double* fby, const double* fbz, double dt, int64_t nsteps,
+                         double* prim_out, double* cons_out, double* fbx_out, double* fby_out,
+                         double* fbz_out);
+int pmhd_adjoint_backward(pmhd_adjoint* a, const double* gprim, const double* gcons,
+                          const double* gfbx, const double* gfby, const double* gfbz,
+                          double* gprim0, double* gcons0, double* gfbx0, double* gfby0,
+                          double* gfbz0);
+int pmhd_adjoint_set_profiling(pmhd_adjoint* a, int32_t enable);
+int pmhd_adjoint_get_profile(pmhd_adjoint* a, int32_t max_entries, char* names, double* total_ms,
+                             int64_t* launches, int64_t* elements, int32_t* n_entries);
+const char* pmhd_adjoint_last_error(const pmhd_adjoint* a);
+
 #ifdef __cplusplus
 }
 #endif

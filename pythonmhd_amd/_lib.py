@@ -17,7 +17,9 @@ EXPORTS = ('pmhd_create', 'pmhd_destroy', 'pmhd_set_state', 'pmhd_get_state', 'p
            'pmhd_get_profile', 'pmhd_last_error',
            'pmhd_batch_create', 'pmhd_batch_destroy', 'pmhd_batch_set_state', 'pmhd_batch_get_state',
            'pmhd_batch_advance', 'pmhd_batch_autotune', 'pmhd_batch_get_tiles', 'pmhd_batch_synchronize',
-           'pmhd_batch_set_profiling', 'pmhd_batch_get_profile', 'pmhd_batch_last_error')
+           'pmhd_batch_set_profiling', 'pmhd_batch_get_profile', 'pmhd_batch_last_error',
+           'pmhd_adjoint_create', 'pmhd_adjoint_destroy', 'pmhd_adjoint_forward', 'pmhd_adjoint_backward',
+           'pmhd_adjoint_set_profiling', 'pmhd_adjoint_get_profile', 'pmhd_adjoint_last_error')
 
 
 class PmhdGrid(ctypes.Structure):
@@ -77,6 +79,14 @@ def _declare(lib):
     lib.pmhd_batch_get_profile.argtypes = lib.pmhd_get_profile.argtypes
     lib.pmhd_batch_last_error.argtypes = [H]
     lib.pmhd_batch_last_error.restype = ctypes.c_char_p
+    lib.pmhd_adjoint_create.argtypes = [ctypes.POINTER(PmhdGrid), ctypes.POINTER(PmhdParams), I32, ctypes.POINTER(H)]
+    lib.pmhd_adjoint_destroy.argtypes = [H]
+    lib.pmhd_adjoint_forward.argtypes = [H, P, P, P, P, P, D, I64, P, P, P, P, P]
+    lib.pmhd_adjoint_backward.argtypes = [H] + [P] * 10
+    lib.pmhd_adjoint_set_profiling.argtypes = [H, I32]
+    lib.pmhd_adjoint_get_profile.argtypes = lib.pmhd_get_profile.argtypes
+    lib.pmhd_adjoint_last_error.argtypes = [H]
+    lib.pmhd_adjoint_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
         if not name.endswith('last_error'):
             getattr(lib, name).restype = ctypes.c_int

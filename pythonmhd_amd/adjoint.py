@@ -1,0 +1,117 @@
+"""Gradients through runs of fixed-dt steps (pmhd_adjoint_* in include/pmhd.h).
+
+``AdjointRun`` owns a device context for one grid: ``forward`` runs ``nsteps``
+cycles of a fixed ``dt`` keeping a checkpoint every ``checkpoint_every`` steps,
+``backward`` maps the cotangent of the final state to the cotangent of the
+initial state of the last ``forward`` (recomputing each checkpoint segment, then
+each step's intermediates, and running the adjoint of every phase).
+
+``fixed_dt_steps`` exposes the same run to torch autograd: the initial state
+(prim, cons, face arrays) are tensors, the final state is returned as tensors, and
+``loss.backward()`` calls the adjoint.  Torch is only the tape here; the step and
+its adjoint run on the GPU through libpmhd.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import MHD, PmhdError, _ptr, face_shapes, load, make_grid, make_params, nvars
+
+
+class AdjointRun:
+    def __init__(self, shape, widths, bcs, gamma, order=1, entropy_fix=False, epsilon=0.5, precision=32,
+                 physics=MHD, checkpoint_every=20, min_dens=1e-12, min_pres=1e-12, min_energy=1e-12, lib=None):
+        self.lib = lib if lib is not None else load()
+        self.shape = tuple(int(s) for s in shape)
+        self.physics = int(physics)
+        g = make_grid(self.shape, widths, bcs)
+        p = make_params(gamma, 1.0, min_dens, min_pres, min_energy, order, entropy_fix, epsilon, precision,
+                        physics)
+        self.grid, self.params = g, p
+        h = ctypes.c_void_p()
+        rc = self.lib.pmhd_adjoint_create(ctypes.byref(g), ctypes.byref(p), int(checkpoint_every), ctypes.byref(h))
+        if rc != 0:
+            raise PmhdError('pmhd_adjoint_create failed (%d): %s' %
+                            (rc, self.lib.pmhd_adjoint_last_error(None).decode()))
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise PmhdError('%s failed (%d): %s' % (what, rc, self.lib.pmhd_adjoint_last_error(self.h).decode()))
+
+    def _state(self, prim, cons, faces):
+        nv = nvars(self.physics)
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        cons = np.ascontiguousarray(cons, dtype=np.float64)
+        if prim.shape != (nv,) + self.shape or cons.shape != (nv,) + self.shape:
+            raise ValueError('state shape %s does not match (%d,) + %s' % (prim.shape, nv, self.shape))
+        want = face_shapes(self.shape, self.physics)
+        fs = [np.ascontiguousarray(f, dtype=np.float64) for f in faces]
+        if len(fs) != len(want) or any(f.shape != w for f, w in zip(fs, want)):
+            raise ValueError('face arrays must have shapes %s' % (want,))
+        return prim, cons, fs
+
+    def forward(self, prim, cons, faces, dt, nsteps):
+        prim, cons, fs = self._state(prim, cons, faces)
+        po = np.empty_like(prim)
+        co = np.empty_like(cons)
+        fo = [np.empty_like(f) for f in fs]
+        fi = [_ptr(f) for f in fs] + [None] * (3 - len(fs))
+        fop = [_ptr(f) for f in fo] + [None] * (3 - len(fo))
+        self._check(self.lib.pmhd_adjoint_forward(self.h, _ptr(prim), _ptr(cons), *fi, float(dt), int(nsteps),
+                                                  _ptr(po), _ptr(co), *fop), 'pmhd_adjoint_forward')
+        return po, co, fo
+
+    def backward(self, gprim=None, gcons=None, gfaces=None):
+        nv = nvars(self.physics)
+        want = face_shapes(self.shape, self.physics)
+        gp = None if gprim is None else np.ascontiguousarray(gprim, dtype=np.float64)
+        gc = None if gcons is None else np.ascontiguousarray(gcons, dtype=np.float64)
+        gfs = [None] * len(want) if gfaces is None else \
+            [None if f is None else np.ascontiguousarray(f, dtype=np.float64) for f in gfaces]
+        p0 = np.empty((nv,) + self.shape)
+        c0 = np.empty((nv,) + self.shape)
+        f0 = [np.empty(w) for w in want]
+        gfi = [_ptr(f) for f in gfs] + [None] * (3 - len(gfs))
+        f0p = [_ptr(f) for f in f0] + [None] * (3 - len(f0))
+        self._check(self.lib.pmhd_adjoint_backward(self.h, _ptr(gp), _ptr(gc), *gfi, _ptr(p0), _ptr(c0), *f0p),
+                    'pmhd_adjoint_backward')
+        return p0, c0, f0
+
+    def set_profiling(self, on):
+        self._check(self.lib.pmhd_adjoint_set_profiling(self.h, 1 if on else 0), 'pmhd_adjoint_set_profiling')
+
+    def close(self):
+        if getattr(self, 'h', None) is not None and self.h.value:
+            self.lib.pmhd_adjoint_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fixed_dt_steps(run, dt, nsteps, prim, cons, *faces):
+    """Differentiable run of ``nsteps`` fixed-dt cycles: torch tensors in (initial
+    prim, cons, face arrays), torch tensors out (final state, same order)."""
+    import torch
+
+    class _Steps(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, prim_t, cons_t, *face_t):
+            p, c, f = run.forward(prim_t.detach().cpu().numpy(), cons_t.detach().cpu().numpy(),
+                                  [x.detach().cpu().numpy() for x in face_t], dt, nsteps)
+            dev, dtype = prim_t.device, prim_t.dtype
+            return tuple(torch.as_tensor(a, dtype=dtype, device=dev) for a in [p, c] + list(f))
+
+        @staticmethod
+        def backward(ctx, gp, gc, *gf):
+            def host(x):
+                return None if x is None else x.detach().cpu().numpy()
+            p0, c0, f0 = run.backward(host(gp), host(gc), [host(x) for x in gf])
+            dev, dtype = gp.device, gp.dtype
+            return tuple(torch.as_tensor(a, dtype=dtype, device=dev) for a in [p0, c0] + list(f0))
+
+    return _Steps.apply(prim, cons, *faces)

@@ -1,0 +1,538 @@
+// Adjoint of a run of fixed-dt steps (no counterpart in the reference), with
+// checkpoint-and-recompute.
+//
+// forward(state0, dt, nsteps): runs the steps, keeping the state at every
+//   checkpoint_every-th step.
+// backward(cotangent of the final state): for each checkpoint segment, last
+//   first, recomputes the segment's states from its checkpoint; then for each step
+//   of the segment, last first, recomputes that step's intermediates into buffers
+//   that are never reused within a step, and runs the adjoint of every phase in
+//   reverse order.  A phase's adjoint evaluates the phase's own kernel body on dual
+//   numbers (pmhd_dual.h): per output element, one recording pass finds the input
+//   slots, one seeded pass per slot adds that slot's share of the vector-Jacobian
+//   product to the slot's cotangent (FP64 atomics), and the element's output
+//   cotangents are then cleared (the values they belong to are dead before the
+//   phase).  Returns the cotangent of the initial state (prim, cons, faces).
+//
+// The loss's dependence on dt is not differentiated: dt is a fixed input.
+#pragma once
+#include "pmhd_api.h"
+#include "pmhd_dual.h"
+
+struct pmhd_adjoint {
+  virtual ~pmhd_adjoint() {}
+  std::string err;
+  virtual int forward(const double* prim, const double* cons, const double* const fin[3], double dt,
+                      int64_t nsteps, double* prim_out, double* cons_out, double* const fout[3]) = 0;
+  virtual int backward(const double* gprim, const double* gcons, const double* const gf[3], double* gprim0,
+                       double* gcons0, double* const gf0[3]) = 0;
+  virtual int set_profiling(int on) = 0;
+  virtual int get_profile(int maxn, char* names, double* ms, int64_t* launches, int64_t* elems, int32_t* n) = 0;
+  int fail(int code, const std::string& m) {
+    err = m;
+    return code;
+  }
+};
+
+namespace pmhd {
+
+// the element function of a phase (the update also yields CFL speeds, unused here)
+template <class Ph, class T>
+PMHD_FI void call_phase(const Ctx<T>& c, I3 p) {
+  if constexpr (Ph::id == K_UPDATE) {
+    double o[3];
+    Ph()(c, p, o);
+  } else {
+    Ph()(c, p);
+  }
+}
+
+template <class V>
+PMHD_FI AdPtr<V> ad_ptr(const V* val, double* adj, AdState* st) {
+  return AdPtr<V>{val, adj, st, 0};
+}
+
+// the dual context of a phase: values from c, cotangents from a
+template <class V>
+PMHD_FI void make_dual_ctx(const Ctx<V>& c, const Ctx<double>& a, AdState* st, Ctx<ADual<V>>& d) {
+  using D = ADual<V>;
+  d.ndim = c.ndim;
+  d.active = c.active;
+  d.physics = c.physics;
+  d.ny = c.ny;
+  d.nx = c.nx;
+  d.nz = c.nz;
+  d.bc_zrec = c.bc_zrec;
+  d.order = c.order;
+  d.efix = c.efix;
+  d.gamma = D(double(c.gamma));
+  d.eps = D(double(c.eps));
+  d.min_dens = D(double(c.min_dens));
+  d.min_pres = D(double(c.min_pres));
+  d.min_energy = D(double(c.min_energy));
+  d.dt = D(double(c.dt));
+  d.hdt = D(double(c.hdt));
+  d.bx1d = D(double(c.bx1d));
+  for (int q = 0; q < 3; ++q) {
+    d.bc[q] = c.bc[q];
+    d.dtd[q] = D(double(c.dtd[q]));
+    d.hdtd[q] = D(double(c.hdtd[q]));
+    d.invd[q] = D(double(c.invd[q]));
+    d.width[q] = D(double(c.width[q]));
+    d.wd[q] = c.wd[q];
+    d.emfq[q] = D(double(c.emfq[q]));
+    d.fb[q] = ad_ptr<V>(c.fb[q], const_cast<double*>(a.fb[q]), st);
+    d.fb2[q] = ad_ptr<V>(c.fb2[q], a.fb2[q], st);
+    d.nfb[q] = ad_ptr<V>(c.nfb[q], a.nfb[q], st);
+    d.UL[q] = ad_ptr<V>(c.UL[q], a.UL[q], st);
+    d.UR[q] = ad_ptr<V>(c.UR[q], a.UR[q], st);
+    d.F[q] = ad_ptr<V>(c.F[q], a.F[q], st);
+    d.Fc[q] = ad_ptr<V>(c.Fc[q], a.Fc[q], st);
+    d.E[q] = ad_ptr<V>(c.E[q], a.E[q], st);
+  }
+  d.ldh[0] = c.ldh[0];
+  d.ldh[1] = c.ldh[1];
+  d.U = ad_ptr<V>(c.U, const_cast<double*>(a.U), st);
+  d.W = ad_ptr<V>(c.W, const_cast<double*>(a.W), st);
+  d.U2 = ad_ptr<V>(c.U2, a.U2, st);
+  d.W2 = ad_ptr<V>(c.W2, a.W2, st);
+  d.WL = ad_ptr<V>(c.WL, a.WL, st);
+  d.WR = ad_ptr<V>(c.WR, a.WR, st);
+  d.S = ad_ptr<V>(c.S, a.S, st);
+  d.Ec = ad_ptr<V>(c.Ec, a.Ec, st);
+}
+
+// adjoint of one output element of phase Ph; returns false when the element has
+// more input or output slots than the state holds
+template <class Ph, class V, class AtomicAdd>
+PMHD_FI bool adjoint_element(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, AtomicAdd add) {
+  Ctx<ADual<V>> d;
+  make_dual_ctx(c, a, &st, d);
+  st.mode = 0;
+  st.nin = 0;
+  st.nout = 0;
+  st.overflow = 0;
+  st.seed = 0;
+  st.acc = 0.0;
+  call_phase<Ph>(d, p);
+  if (st.overflow) return false;
+  ad_prune(st);
+  st.mode = 1;
+  for (int k = 0; k < st.nin; ++k) {
+    st.seed = k;
+    st.acc = 0.0;
+    call_phase<Ph>(d, p);
+    if (st.acc != 0.0) add(st.in[k].adj + st.in[k].i, st.acc);
+  }
+  for (int k = 0; k < st.nout; ++k) st.out[k].adj[st.out[k].i] = 0.0;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// buffers of one recomputed step: each phase output has its own array, so
+// every phase's inputs are still intact when its adjoint runs
+// ---------------------------------------------------------------------------
+template <class T>
+struct StepBufs {
+  T* WL[3] = {};
+  T* WR[3] = {};
+  T* S[3] = {};
+  T* UL[3] = {};
+  T* UR[3] = {};
+  T* F[3] = {};
+  T* Fc[3] = {};
+  T* nfb[3] = {};
+  T* Ec_p = nullptr;   // cell E of the predictor (CellE)
+  T* Ec_h = nullptr;   // cell E of the half step (HalfE)
+  T* E_p[3] = {};      // corner EMFs of the predictor fluxes
+  T* E_c[3] = {};      // corner EMFs of the corrector fluxes
+  T* U2 = nullptr;
+  T* W2 = nullptr;
+  T* fb2[3] = {};
+};
+
+template <class T, class U, class A>
+void alloc_step(const Ctx<T>& c, StepBufs<U>& s, A&& alloc) {
+  const int nd = c.ndim;
+  const int64_t N = c.ncell();
+  const int64_t cv = cell_vars(c.physics), fv = face_vars(c.physics);
+  const bool mhd = c.physics == 0;
+  for (int d = 0; d < 3; ++d) {
+    const bool used = d < nd;
+    s.WL[d] = alloc(used ? fv * c.nface(d) : 1);
+    s.WR[d] = alloc(used ? fv * c.nface(d) : 1);
+    s.S[d] = alloc(used && c.order > 0 ? fv * c.nslope(d) : 1);
+    s.UL[d] = alloc(used ? fv * c.nface(d) : 1);
+    s.UR[d] = alloc(used ? fv * c.nface(d) : 1);
+    s.F[d] = alloc(used ? fv * c.nface(d) : 1);
+    s.Fc[d] = alloc(used && nd >= 2 ? fv * c.nface(d) : 1);
+    s.nfb[d] = alloc(mhd && used && nd >= 2 ? c.nface(d) : 1);
+    s.E_p[d] = alloc(mhd && (nd == 3 || (nd == 2 && d == 2)) ? c.nedge() : 1);
+    s.E_c[d] = alloc(mhd && (nd == 3 || (nd == 2 && d == 2)) ? c.nedge() : 1);
+    s.fb2[d] = alloc(mhd && used && nd >= 2 ? c.nface(d) : 1);
+  }
+  s.Ec_p = alloc(mhd ? (nd == 3 ? 3 * N : N) : 1);
+  s.Ec_h = alloc(mhd ? (nd == 3 ? 3 * N : N) : 1);
+  s.U2 = alloc(cv * N);
+  s.W2 = alloc(cv * N);
+}
+
+// the arrays a launch of id sees (values or cotangents of the same layout)
+template <class T>
+void bind_step(Ctx<T>& c, const StepBufs<T>& s, int id) {
+  int kind, dir;
+  phase_shape(id, kind, dir);
+  for (int d = 0; d < 3; ++d) {
+    c.UL[d] = s.UL[d];
+    c.UR[d] = s.UR[d];
+    c.F[d] = s.F[d];
+    c.Fc[d] = s.Fc[d];
+    c.nfb[d] = s.nfb[d];
+    c.fb2[d] = s.fb2[d];
+  }
+  c.U2 = s.U2;
+  c.W2 = s.W2;
+  c.WL = s.WL[dir];
+  c.WR = s.WR[dir];
+  c.S = s.S[dir];
+  const bool half = id == K_HALF_E || id == K_EMF2 || (id >= K_CT_FULL && id < K_CT_FULL + 3) || id == K_UPDATE;
+  c.Ec = half ? s.Ec_h : s.Ec_p;
+  for (int d = 0; d < 3; ++d) c.E[d] = half ? s.E_c[d] : s.E_p[d];
+}
+
+// runs a cycle forward while writing down the adjoint launches it implies
+template <class T, class B>
+struct TapeExec {
+  B* be = nullptr;
+  const StepBufs<T>* vals = nullptr;
+  const StepBufs<double>* adjs = nullptr;
+  Ctx<double> abase;                       // state cotangent pointers of this step
+  std::vector<std::function<void()>> tape;
+
+  template <class Ph>
+  void record(const Ctx<T>& c, Ctx<T>& cv, Ctx<double>& av) {
+    cv = c;
+    bind_step(cv, *vals, Ph::id);
+    av = abase;
+    bind_step(av, *adjs, Ph::id);
+  }
+  template <class Ph>
+  void run(const Ctx<T>& c) {
+    Ctx<T> cv;
+    Ctx<double> av;
+    record<Ph>(c, cv, av);
+    be->exec().template run<Ph>(cv);
+    B* b = be;
+    tape.push_back([b, cv, av] { b->template adjoint_phase<Ph>(cv, av); });
+  }
+  template <class Ph>
+  void run_max(const Ctx<T>& c) {
+    Ctx<T> cv;
+    Ctx<double> av;
+    record<Ph>(c, cv, av);
+    be->exec().template run_max<Ph>(cv);
+    B* b = be;
+    tape.push_back([b, cv, av] { b->template adjoint_phase<Ph>(cv, av); });
+  }
+};
+
+template <class T, class B>
+struct AdjointHandle : public pmhd_adjoint {
+  B be;
+  pmhd_grid g;
+  pmhd_params p;
+  Ctx<T> c;                 // forward context (single-grid layout)
+  StateBufs<T> sb;
+  int every = 1;
+  std::vector<void*> owned;
+  // checkpoints and the states of the segment being reversed: (U, W, fb) each
+  struct Snap {
+    T* U;
+    T* W;
+    T* fb[3];
+  };
+  std::vector<Snap> ckpt;
+  std::vector<Snap> seg;
+  StepBufs<T> vals;
+  StepBufs<double> adjs;
+  double* adj_state[2][5] = {};   // cotangents of (U, W, fb0, fb1, fb2), two sets
+  double dt = 0.0;
+  int64_t nsteps = 0;
+  bool have_forward = false;
+
+  ~AdjointHandle() override {
+    for (void* ptr : owned) be.release(ptr);
+  }
+  void* dalloc_bytes(size_t bytes) {
+    void* ptr = be.alloc(bytes > 0 ? bytes : 8);
+    if (!ptr) throw std::bad_alloc();
+    owned.push_back(ptr);
+    return ptr;
+  }
+  T* dalloc(int64_t n) { return static_cast<T*>(dalloc_bytes(size_t(n > 0 ? n : 1) * sizeof(T))); }
+  double* dalloc_d(int64_t n) {
+    double* ptr = static_cast<double*>(dalloc_bytes(size_t(n > 0 ? n : 1) * sizeof(double)));
+    be.zero(ptr, size_t(n > 0 ? n : 1) * sizeof(double));
+    return ptr;
+  }
+  int64_t nv() const { return cell_vars(p.physics); }
+  int64_t nface(int d) const { return p.physics == 0 && d < g.ndim && g.ndim >= 2 ? c.nface(d) : 0; }
+
+  Snap new_snap() {
+    Snap s;
+    s.U = dalloc(nv() * c.ncell());
+    s.W = dalloc(nv() * c.ncell());
+    for (int d = 0; d < 3; ++d) s.fb[d] = dalloc(nface(d) > 0 ? nface(d) : 1);
+    return s;
+  }
+  void copy_snap(T* U, T* W, T* const fb[3], const T* U0, const T* W0, const T* const fb0[3]) {
+    be.d2d(U, U0, sizeof(T) * size_t(nv() * c.ncell()));
+    be.d2d(W, W0, sizeof(T) * size_t(nv() * c.ncell()));
+    for (int d = 0; d < 3; ++d)
+      if (nface(d) > 0) be.d2d(fb[d], fb0[d], sizeof(T) * size_t(nface(d)));
+  }
+
+  int init(const pmhd_grid& g_, const pmhd_params& p_, int every_) {
+    g = g_;
+    p = p_;
+    every = every_ > 0 ? every_ : 1;
+    init_ctx(c, g, p);
+    try {
+      alloc_grid(c, sb, p.reconstruct_order, [&](int64_t n) { return dalloc(n); });
+      alloc_step(c, vals, [&](int64_t n) { return dalloc(n); });
+      alloc_step(c, adjs, [&](int64_t n) { return dalloc_d(n); });
+      for (int s = 0; s < 2; ++s) {
+        adj_state[s][0] = dalloc_d(nv() * c.ncell());
+        adj_state[s][1] = dalloc_d(nv() * c.ncell());
+        for (int d = 0; d < 3; ++d) adj_state[s][2 + d] = dalloc_d(nface(d) > 0 ? nface(d) : 1);
+      }
+      for (int k = 0; k < every; ++k) seg.push_back(new_snap());
+    } catch (const std::bad_alloc&) {
+      return fail(PMHD_ENOMEM, "device allocation failed");
+    }
+    bind_bufs(c, sb, 0);
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+
+  void set_dt_scalars() {
+    set_step_scalars(c, dt);
+    const long double hl = 0.5L * (static_cast<long double>(dt) / static_cast<long double>(g.dy));
+    c.ldh[0] = static_cast<double>(hl);
+    c.ldh[1] = static_cast<double>(hl - static_cast<long double>(c.ldh[0]));
+  }
+
+  // one forward step of the single-grid layout; the state ends in buffer set cur
+  int cur = 0;
+  void step_forward() {
+    bind_bufs(c, sb, cur);
+    run_cycle(be.exec(), c);
+    cur = 1 - cur;
+    bind_bufs(c, sb, cur);
+  }
+
+  int forward(const double* prim, const double* cons, const double* const fin[3], double dt_, int64_t n,
+              double* prim_out, double* cons_out, double* const fout[3]) override {
+    if (!(dt_ > 0) || n < 1) return fail(PMHD_EINVAL, "dt must be positive and nsteps at least 1");
+    dt = dt_;
+    nsteps = n;
+    cur = 0;
+    bind_bufs(c, sb, 0);
+    int rc = load_state(be, c, g, prim, cons, fin, sb.U[0], sb.W[0], sb.FB[0], err);
+    if (rc) return rc;
+    set_dt_scalars();
+    const int64_t nck = (n + every - 1) / every;
+    try {
+      while (int64_t(ckpt.size()) < nck) ckpt.push_back(new_snap());
+    } catch (const std::bad_alloc&) {
+      return fail(PMHD_ENOMEM, "device allocation failed (checkpoints)");
+    }
+    for (int64_t s = 0; s < n; ++s) {
+      if (s % every == 0) {
+        const Snap& k = ckpt[size_t(s / every)];
+        copy_snap(k.U, k.W, k.fb, sb.U[cur], sb.W[cur], sb.FB[cur]);
+      }
+      step_forward();
+    }
+    double* fo[3] = {fout[0], fout[1], fout[2]};
+    store_state(be, c, g, sb.U[cur], sb.W[cur], sb.FB[cur], prim_out, cons_out, fo);
+    be.sync();
+    have_forward = true;
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+
+  void upload_adj(double* dst, const double* src, int64_t n) {
+    if (src) be.h2d(dst, src, sizeof(double) * size_t(n));
+    else be.zero(dst, sizeof(double) * size_t(n));
+  }
+
+  int backward(const double* gprim, const double* gcons, const double* const gf[3], double* gprim0,
+               double* gcons0, double* const gf0[3]) override {
+    if (!have_forward) return fail(PMHD_EINVAL, "backward needs a preceding forward");
+    const int64_t N = c.ncell();
+    int out = 0;   // adj_state[out]: cotangent of the current step's output state
+    upload_adj(adj_state[out][0], gcons, nv() * N);
+    upload_adj(adj_state[out][1], gprim, nv() * N);
+    for (int d = 0; d < 3; ++d)
+      if (nface(d) > 0) upload_adj(adj_state[out][2 + d], gf[d], nface(d));
+    const int64_t nck = (nsteps + every - 1) / every;
+    for (int64_t sgi = nck - 1; sgi >= 0; --sgi) {
+      const int64_t s0 = sgi * every;
+      const int64_t s1 = std::min<int64_t>(s0 + every, nsteps);
+      // states of the segment, from its checkpoint
+      const Snap& k = ckpt[size_t(sgi)];
+      cur = 0;
+      copy_snap(sb.U[0], sb.W[0], sb.FB[0], k.U, k.W, k.fb);
+      for (int64_t s = s0; s < s1; ++s) {
+        const Snap& q = seg[size_t(s - s0)];
+        copy_snap(q.U, q.W, q.fb, sb.U[cur], sb.W[cur], sb.FB[cur]);
+        if (s + 1 < s1) step_forward();
+      }
+      for (int64_t s = s1 - 1; s >= s0; --s) {
+        const Snap& q = seg[size_t(s - s0)];
+        const int in = 1 - out;
+        for (int a = 0; a < 5; ++a) {
+          const int64_t len = a < 2 ? nv() * N : nface(a - 2);
+          if (len > 0) be.zero(adj_state[in][a], sizeof(double) * size_t(len));
+        }
+        // the step's context: inputs from the stored state
+        Ctx<T> cs = c;
+        cs.U = q.U;
+        cs.W = q.W;
+        for (int d = 0; d < 3; ++d) cs.fb[d] = q.fb[d];
+        TapeExec<T, B> tx;
+        tx.be = &be;
+        tx.vals = &vals;
+        tx.adjs = &adjs;
+        std::memset(&tx.abase, 0, sizeof(tx.abase));
+        tx.abase.U = adj_state[in][0];
+        tx.abase.W = adj_state[in][1];
+        for (int d = 0; d < 3; ++d) tx.abase.fb[d] = adj_state[in][2 + d];
+        // the output state's cotangents live where this step writes U2, W2, fb2
+        be.d2d(adjs.U2, adj_state[out][0], sizeof(double) * size_t(nv() * N));
+        be.d2d(adjs.W2, adj_state[out][1], sizeof(double) * size_t(nv() * N));
+        for (int d = 0; d < 3; ++d)
+          if (nface(d) > 0) be.d2d(adjs.fb2[d], adj_state[out][2 + d], sizeof(double) * size_t(nface(d)));
+        run_cycle(tx, cs);
+        for (size_t t = tx.tape.size(); t-- > 0;) tx.tape[t]();
+        if (be.adjoint_overflow()) return fail(PMHD_EINVAL, "adjoint element stencil larger than supported");
+        out = in;
+      }
+    }
+    // cotangent of the initial state
+    std::vector<double> tmp;
+    auto down = [&](double* dst, const double* src, int64_t n) {
+      if (dst) be.d2h(dst, src, sizeof(double) * size_t(n));
+    };
+    down(gcons0, adj_state[out][0], nv() * N);
+    down(gprim0, adj_state[out][1], nv() * N);
+    for (int d = 0; d < 3; ++d)
+      if (nface(d) > 0) down(gf0[d], adj_state[out][2 + d], nface(d));
+    be.sync();
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+
+  int set_profiling(int on) override {
+    be.set_profiling(on != 0);
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+  int get_profile(int maxn, char* names, double* ms, int64_t* launches, int64_t* elems, int32_t* n) override {
+    *n = be.profile(maxn, names, ms, launches, elems);
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+};
+
+template <class B>
+int create_adjoint(const pmhd_grid* g, const pmhd_params* p, int every, pmhd_adjoint** out, std::string& err) {
+  if (!g || !p || !out) {
+    err = "null argument";
+    return PMHD_EINVAL;
+  }
+  *out = nullptr;
+  int rc = check_grid(g, err);
+  if (rc) return rc;
+  rc = check_launch_extents(g, err);
+  if (rc) return rc;
+  rc = check_params(p, err);
+  if (rc) return rc;
+  if (every < 1) {
+    err = "checkpoint_every must be at least 1";
+    return PMHD_EINVAL;
+  }
+  const pmhd_grid gg = normal_grid(*g);
+  pmhd_adjoint* h = nullptr;
+  if (p->precision == 64) {
+    auto* hh = new (std::nothrow) AdjointHandle<double, B>();
+    if (!hh) { err = "out of host memory"; return PMHD_ENOMEM; }
+    rc = hh->init(gg, *p, every);
+    h = hh;
+  } else {
+    auto* hh = new (std::nothrow) AdjointHandle<float, B>();
+    if (!hh) { err = "out of host memory"; return PMHD_ENOMEM; }
+    rc = hh->init(gg, *p, every);
+    h = hh;
+  }
+  if (rc != PMHD_OK) {
+    err = h->err;
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return PMHD_OK;
+}
+
+}  // namespace pmhd
+
+#define PMHD_DEFINE_ADJOINT_ABI(BACKEND)                                                            \
+  static std::string g_pmhd_adj_error;                                                              \
+  extern "C" int pmhd_adjoint_create(const pmhd_grid* g, const pmhd_params* p,                      \
+                                     int32_t checkpoint_every, pmhd_adjoint** out) {                \
+    try {                                                                                           \
+      return pmhd::create_adjoint<BACKEND>(g, p, checkpoint_every, out, g_pmhd_adj_error);          \
+    } catch (...) {                                                                                 \
+      g_pmhd_adj_error = "unexpected exception in pmhd_adjoint_create";                             \
+      return PMHD_EINVAL;                                                                           \
+    }                                                                                               \
+  }                                                                                                 \
+  extern "C" int pmhd_adjoint_destroy(pmhd_adjoint* a) {                                            \
+    delete a;                                                                                       \
+    return PMHD_OK;                                                                                 \
+  }                                                                                                 \
+  extern "C" int pmhd_adjoint_forward(pmhd_adjoint* a, const double* prim, const double* cons,      \
+                                      const double* fbx, const double* fby, const double* fbz,      \
+                                      double dt, int64_t nsteps, double* prim_out,                  \
+                                      double* cons_out, double* fbx_out, double* fby_out,           \
+                                      double* fbz_out) {                                            \
+    if (!a) return PMHD_EINVAL;                                                                     \
+    const double* fin[3] = {fbx, fby, fbz};                                                         \
+    double* fo[3] = {fbx_out, fby_out, fbz_out};                                                    \
+    try { return a->forward(prim, cons, fin, dt, nsteps, prim_out, cons_out, fo); }                 \
+    catch (...) { return a->fail(PMHD_EINVAL, "exception in pmhd_adjoint_forward"); }               \
+  }                                                                                                 \
+  extern "C" int pmhd_adjoint_backward(pmhd_adjoint* a, const double* gprim, const double* gcons,   \
+                                       const double* gfbx, const double* gfby, const double* gfbz,  \
+                                       double* gprim0, double* gcons0, double* gfbx0,               \
+                                       double* gfby0, double* gfbz0) {                              \
+    if (!a) return PMHD_EINVAL;                                                                     \
+    const double* gf[3] = {gfbx, gfby, gfbz};                                                       \
+    double* g0[3] = {gfbx0, gfby0, gfbz0};                                                          \
+    try { return a->backward(gprim, gcons, gf, gprim0, gcons0, g0); }                               \
+    catch (...) { return a->fail(PMHD_EINVAL, "exception in pmhd_adjoint_backward"); }              \
+  }                                                                                                 \
+  extern "C" int pmhd_adjoint_set_profiling(pmhd_adjoint* a, int32_t enable) {                      \
+    if (!a) return PMHD_EINVAL;                                                                     \
+    return a->set_profiling(enable);                                                                \
+  }                                                                                                 \
+  extern "C" int pmhd_adjoint_get_profile(pmhd_adjoint* a, int32_t max_entries, char* names,        \
+                                          double* total_ms, int64_t* launches, int64_t* elements,   \
+                                          int32_t* n_entries) {                                     \
+    if (!a || !names || !total_ms || !launches || !elements || !n_entries) return PMHD_EINVAL;      \
+    return a->get_profile(max_entries, names, total_ms, launches, elements, n_entries);             \
+  }                                                                                                 \
+  extern "C" const char* pmhd_adjoint_last_error(const pmhd_adjoint* a) {                           \
+    return a ? a->err.c_str() : g_pmhd_adj_error.c_str();                                           \
+  }

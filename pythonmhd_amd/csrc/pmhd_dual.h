@@ -49,10 +49,12 @@ struct ADual {
   PMHD_FI ADual& operator-=(const ADual& b) { return *this = *this - b; }
 };
 
+// at 0 the derivative is taken as 0 (the step's square roots vanish only at its
+// degenerate points, e.g. |B_t| = 0, where it switches to a fixed branch)
 template <class V>
 PMHD_FI ADual<V> sqrt_(const ADual<V>& a) {
   const V s = sqrt_(a.v);
-  return ADual<V>(s, a.t / (V(2) * s), 0);
+  return ADual<V>(s, s > V(0) ? a.t / (V(2) * s) : V(0), 0);
 }
 template <class V>
 PMHD_FI ADual<V> fabs_(const ADual<V>& a) {

@@ -9,7 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "pmhd_api.h"
+#include "pmhd_adjoint.h"
 
 namespace pmhd {
 
@@ -234,6 +234,24 @@ template <class T>
 __global__ void k_box_activate(Ctx<T>* ctxs, int nbox) {
   const int b = int(blockIdx.x * blockDim.x + threadIdx.x);
   if (b < nbox) ctxs[b].active = 1;
+}
+
+// ---------------------------------------------------------------------------
+// adjoint of one phase: one thread per output element (pmhd_adjoint.h)
+// ---------------------------------------------------------------------------
+struct DevAtomicAdd {
+  __device__ void operator()(double* p, double v) const { atomicAdd(p, v); }
+};
+
+template <class Ph, class T>
+__global__ void __launch_bounds__(64) k_adjoint(const Ctx<T> c, const Ctx<double> a, int n1, int n2, int* flag) {
+  const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+  const int i = int(blockIdx.y * blockDim.y + threadIdx.y);
+  const int j = int(blockIdx.z);
+  if (k < n2 && i < n1) {
+    AdState st;
+    if (!adjoint_element<Ph>(c, a, I3{j, i, k}, st, DevAtomicAdd())) atomicOr(flag, 1);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -553,6 +571,7 @@ struct HipBackend {
     if (ex.red) hipFree(ex.red);
     if (ex.part) hipFree(ex.part);
     if (ex.parts) hipFree(ex.parts);
+    if (adj_flag) hipFree(adj_flag);
     if (ex.s) hipStreamDestroy(ex.s);
   }
   HipExec& exec() { return ex; }
@@ -580,7 +599,35 @@ struct HipBackend {
   }
   void zero(void* dst, size_t bytes) {
     ex.note(hipMemsetAsync(dst, 0, bytes, ex.s));
+  }
+  void d2d(void* dst, const void* src, size_t bytes) {
+    ex.note(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ex.s));
+  }
+  int* adj_flag = nullptr;
+  template <class Ph, class T>
+  void adjoint_phase(const Ctx<T>& c, const Ctx<double>& a) {
+    if (!adj_flag) {
+      ex.note(hipMalloc(reinterpret_cast<void**>(&adj_flag), sizeof(int)));
+      ex.note(hipMemsetAsync(adj_flag, 0, sizeof(int), ex.s));
+    }
+    int64_t n[3];
+    c.extent(Ph::kind, Ph::dir, n);
+    if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return;
+    unsigned bx = 1;
+    while (bx < 64 && int64_t(bx) < n[2]) bx <<= 1;
+    const unsigned by = 64 / bx;
+    dim3 grid(unsigned((n[2] + bx - 1) / bx), unsigned((n[1] + by - 1) / by), unsigned(n[0]));
+    ex.timed(K_COUNT - 1, n[0] * n[1] * n[2], [&] {
+      hipLaunchKernelGGL((k_adjoint<Ph, T>), grid, dim3(bx, by, 1), 0, ex.s, c, a, int(n[1]), int(n[2]), adj_flag);
+    });
+    ex.note(hipGetLastError());
+  }
+  bool adjoint_overflow() {
+    if (!adj_flag) return false;
+    int h = 0;
+    ex.note(hipMemcpyAsync(&h, adj_flag, sizeof(int), hipMemcpyDeviceToHost, ex.s));
     ex.note(hipStreamSynchronize(ex.s));
+    return h != 0;
   }
   void sync() { ex.note(hipStreamSynchronize(ex.s)); }
   void set_profiling(bool on) {
@@ -612,3 +659,4 @@ struct HipBackend {
 }  // namespace pmhd
 
 PMHD_DEFINE_C_ABI(pmhd::HipBackend)
+PMHD_DEFINE_ADJOINT_ABI(pmhd::HipBackend)

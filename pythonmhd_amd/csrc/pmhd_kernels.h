@@ -614,7 +614,7 @@ PMHD_FI void ctu_apply_3d(const Ctx<T>& c, int d, int tc, I3 pcell, int64_t hi_o
   T h = c.hdtd[tc];
   const int vs[5] = {0, 1, 2, 3, 6};
   // x faces, y fluxes: the reference's 80-bit factor (FP64 instantiation only)
-  const bool ld = sizeof(T) == sizeof(double) && d == 0 && tc == 1;
+  const bool ld = is_f64<T>() && d == 0 && tc == 1;
 #pragma unroll
   for (int q = 0; q < 5; ++q) {
     int v = vs[q];

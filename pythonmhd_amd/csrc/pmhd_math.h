@@ -36,6 +36,11 @@ struct Island {
 };
 template <class T>
 using island_t = typename Island<T>::type;
+// true for plain double (not for a dual number of the same size)
+template <class T>
+PMHD_FI constexpr bool is_f64() { return false; }
+template <>
+PMHD_FI constexpr bool is_f64<double>() { return true; }
 
 // NumPy element semantics: maximum/minimum propagate NaN and return the second
 // argument on ties (so the sign of zero follows NumPy); sign(0) is +0.

@@ -9,7 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "pmhd_api.h"
+#include "pmhd_adjoint.h"
 
 namespace pmhd {
 
@@ -220,6 +220,19 @@ struct HostBackend {
   void h2d(void* d, const void* s, size_t b) { std::memcpy(d, s, b); }
   void d2h(void* d, const void* s, size_t b) { std::memcpy(d, s, b); }
   void zero(void* d, size_t b) { std::memset(d, 0, b); }
+  void d2d(void* d, const void* s, size_t b) { std::memcpy(d, s, b); }
+  bool overflow = false;
+  template <class Ph, class T>
+  void adjoint_phase(const Ctx<T>& c, const Ctx<double>& a) {
+    int64_t n[3];
+    c.extent(Ph::kind, Ph::dir, n);
+    AdState st;
+    for (int64_t j = 0; j < n[0]; ++j)
+      for (int64_t i = 0; i < n[1]; ++i)
+        for (int64_t k = 0; k < n[2]; ++k)
+          if (!adjoint_element<Ph>(c, a, I3{j, i, k}, st, [](double* p, double v) { *p += v; })) overflow = true;
+  }
+  bool adjoint_overflow() const { return overflow; }
   void sync() {}
   const char* error() { return nullptr; }
   void set_profiling(bool) {}
@@ -229,3 +242,4 @@ struct HostBackend {
 }  // namespace pmhd
 
 PMHD_DEFINE_C_ABI(pmhd::HostBackend)
+PMHD_DEFINE_ADJOINT_ABI(pmhd::HostBackend)

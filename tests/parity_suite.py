@@ -342,3 +342,156 @@ def test_hydro_batch_matches_single(lib):
         same(cyc[k], cc)
         s.close()
     b.close()
+
+
+# ---------------------------------------------------------------------------
+# adjoint of fixed-dt runs: gradients against central differences of the FP64
+# oracle (an independent FP64 reference gradient), checkpoint invariance, FP32
+# ---------------------------------------------------------------------------
+def _kinetic_loss(c, nv=8):
+    return float(np.sum(0.5 * (c[1] ** 2 + c[2] ** 2 + c[3] ** 2) / c[0]))
+
+
+def _kinetic_cotangent(c):
+    g = np.zeros_like(c)
+    g[0] = -0.5 * (c[1] ** 2 + c[2] ** 2 + c[3] ** 2) / c[0] ** 2
+    g[1:4] = c[1:4] / c[0]
+    return g
+
+
+def _generic_state(rng, shape):
+    """A smooth state with every variable varying along every axis (no uniform
+    direction), so no limiter sits exactly at a tie where the step is not
+    differentiable."""
+    nd = len(shape)
+    g = np.meshgrid(*[(np.arange(n) + 0.5) / n for n in shape], indexing='ij')
+
+    def field(amp, base):
+        k = rng.integers(1, 3, nd)
+        ph = rng.uniform(0, 2 * np.pi)
+        return base + amp * np.sin(2 * np.pi * sum(ki * gi for ki, gi in zip(k, g)) + ph)
+    prim = np.stack([field(0.2, 1.0), field(0.3, 0.0), field(0.3, 0.0), field(0.3, 0.0),
+                     field(0.3, 0.5), field(0.3, 0.2), field(0.3, -0.3), field(0.2, 1.0)])
+    faces = []
+    for d in range(nd if nd >= 2 else 0):
+        ax = {0: 1, 1: 0, 2: 2}[d] if nd == 3 else {0: 1, 1: 0}[d]
+        fs = list(shape)
+        fs[ax] += 1
+        gf = np.meshgrid(*[(np.arange(n) + (0.0 if a == ax else 0.5)) / shape[a] for a, n in enumerate(fs)],
+                         indexing='ij')
+        k = rng.integers(1, 3, nd)
+        faces.append(0.4 + 0.3 * np.sin(2 * np.pi * sum(ki * gi for ki, gi in zip(k, gf)) + rng.uniform(0, 6)))
+    return prim, faces
+
+
+def _oracle_steps(nd, prim, cons, faces, dt, n, widths, bcs, P):
+    p, c, f = prim, cons, list(faces)
+    for _ in range(n):
+        if nd == 2:
+            p, c, f0, f1 = M.step_2d(p, c, f[0], f[1], dt, widths[0], widths[1], bcs[0], bcs[1], P)
+            f = [f0, f1]
+        else:
+            p, c, f0, f1, f2 = M.step_3d(p, c, f[0], f[1], f[2], dt, widths[0], widths[1], widths[2],
+                                         bcs[0], bcs[1], bcs[2], P)
+            f = [f0, f1, f2]
+    return p, c, f
+
+
+@pytest.mark.parametrize('nd,order,bcs', [(2, 1, (1, 1)), (2, 2, (0, 1)), (3, 1, (1, 1, 1))])
+def test_adjoint_against_oracle_differences(lib, nd, order, bcs):
+    """Gradient of the total kinetic energy after a few fixed-dt steps with respect
+    to the whole initial state (prim, cons, faces): the adjoint's directional
+    derivative along random directions and its entries at chosen cells equal
+    central differences of the FP64 oracle (smooth flow, away from limiter kinks)."""
+    from pythonmhd_amd.adjoint import AdjointRun
+    shape = (10, 12) if nd == 2 else (5, 6, 4)
+    rng = np.random.default_rng(40 + nd + order)
+    prim, faces = _generic_state(rng, shape)
+    cons = M.prim_to_cons_all(prim, GAMMA)
+    widths = widths_of(shape)
+    dt, n = (0.004, 3) if nd == 2 else (0.005, 2)
+    P = M.Params(GAMMA, 0.8, order=order, entropy_fix=False)
+    print('case: adjoint', nd, order, bcs, flush=True)
+    run = AdjointRun(shape, widths, bcs, GAMMA, order=order, precision=64, checkpoint_every=2, lib=lib)
+    _, co, _ = run.forward(prim, cons, faces, dt, n)
+    gp, gc, gf = run.backward(None, _kinetic_cotangent(co), None)
+    run.close()
+
+    def loss(p, c, f):
+        return _kinetic_loss(_oracle_steps(nd, p, c, f, dt, n, widths, bcs, P)[1])
+
+    # piecewise-smooth step (limiter and upwind switches): small steps keep the
+    # central difference on one smooth piece
+    eps = 1e-7
+    for trial in range(2):
+        vp = rng.standard_normal(prim.shape)
+        vc = rng.standard_normal(cons.shape)
+        vf = [rng.standard_normal(f.shape) for f in faces]
+        fd = (loss(prim + eps * vp, cons + eps * vc, [f + eps * v for f, v in zip(faces, vf)])
+              - loss(prim - eps * vp, cons - eps * vc, [f - eps * v for f, v in zip(faces, vf)])) / (2 * eps)
+        ad = float(np.sum(gp * vp) + np.sum(gc * vc) + sum(np.sum(g * v) for g, v in zip(gf, vf)))
+        assert abs(ad - fd) <= 1e-6 * abs(fd) + 1e-8, (ad, fd)
+    eps = 1e-6
+    for (arr, v) in (('cons', 1), ('cons', 0), ('prim', 7), ('cons', 7)):
+        idx = tuple(int(rng.integers(0, s)) for s in shape)
+        def pert(s):
+            p2, c2 = prim.copy(), cons.copy()
+            (c2 if arr == 'cons' else p2)[(v,) + idx] += s
+            return loss(p2, c2, faces)
+        fd = (pert(eps) - pert(-eps)) / (2 * eps)
+        ad = (gc if arr == 'cons' else gp)[(v,) + idx]
+        assert abs(ad - fd) <= 2e-6 * abs(fd) + 1e-9, (arr, v, idx, ad, fd)
+
+
+def test_adjoint_checkpoint_spacing_and_fp32(lib):
+    """The gradient does not depend on the checkpoint spacing (to FP64 rounding of
+    the cotangent sums), and the FP32 adjoint is within 1e-3 of the FP64 one."""
+    from pythonmhd_amd.adjoint import AdjointRun
+    shape = (12, 12)
+    rng = np.random.default_rng(8)
+    prim, faces = _generic_state(rng, shape)
+    cons = M.prim_to_cons_all(prim, GAMMA)
+    widths = widths_of(shape)
+    grads = []
+    for every, prec in ((1, 64), (3, 64), (7, 64), (3, 32)):
+        run = AdjointRun(shape, widths, (1, 1), GAMMA, order=1, precision=prec, checkpoint_every=every, lib=lib)
+        _, co, _ = run.forward(prim, cons, faces, 0.004, 7)
+        grads.append(run.backward(None, _kinetic_cotangent(co), None))
+        run.close()
+    ref = grads[0]
+    for g in grads[1:3]:
+        for a, b in zip(g[:2], ref[:2]):
+            assert np.allclose(a, b, rtol=1e-12, atol=1e-14)
+    scale = np.max(np.abs(ref[1]))
+    assert np.max(np.abs(grads[3][1] - ref[1])) <= 1e-3 * scale
+
+
+def test_adjoint_hydro_1d(lib):
+    from pythonmhd_amd._lib import HYDRO
+    from pythonmhd_amd.adjoint import AdjointRun
+    n = 32
+    x = (np.arange(n) + 0.5) / n
+    # phases off the grid's symmetry points: no two neighbouring cells tie exactly
+    prim = np.stack([1 + 0.2 * np.sin(2 * np.pi * x + 0.4), 0.3 * np.cos(2 * np.pi * x + 0.9),
+                     0.1 * np.sin(4 * np.pi * x + 0.2), 0.2 * np.cos(4 * np.pi * x + 1.0),
+                     1 + 0.1 * np.cos(2 * np.pi * x + 2.1)])
+    cons = H.prim_to_cons(prim, 1.4)
+    dt, steps = 0.004, 4
+    P = H.Params(1.4, 0.5, order=1)
+    run = AdjointRun((n,), (1.0 / n,), (1,), 1.4, order=1, precision=64, physics=HYDRO, checkpoint_every=3,
+                     lib=lib)
+    _, co, _ = run.forward(prim, cons, (), dt, steps)
+    gp, gc, _ = run.backward(None, _kinetic_cotangent(co), None)
+    run.close()
+
+    def loss(p, c):
+        for _ in range(steps):
+            p, c = H.step(p, c, dt, (1.0 / n,), (1,), P)
+        return _kinetic_loss(c)
+
+    rng = np.random.default_rng(2)
+    vp, vc = rng.standard_normal(prim.shape), rng.standard_normal(cons.shape)
+    eps = 1e-7
+    fd = (loss(prim + eps * vp, cons + eps * vc) - loss(prim - eps * vp, cons - eps * vc)) / (2 * eps)
+    ad = float(np.sum(gp * vp) + np.sum(gc * vc))
+    assert abs(ad - fd) <= 1e-6 * abs(fd) + 1e-8, (ad, fd)
