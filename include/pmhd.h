@@ -80,6 +80,15 @@ int pmhd_step(pmhd_handle* h, double dt);
 int pmhd_advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int64_t max_cycles,
                  double* t_out, int64_t* cycle_out, double* t_second_last);
 
+/* Faces per thread of the fused predictor sweep along x, y and z (no counterpart
+ * in the reference; a tuning knob).  With a run length s > 0 one launch per
+ * direction reconstructs the cells (PLM slopes, or the PPM slopes of a separate
+ * launch), traces the edge states and forms the predictor's interface states and
+ * Roe fluxes for s consecutive faces per thread; 0 selects separate slope, edge
+ * and predictor launches.  Results are identical for every choice.  The default
+ * is picked from the grid size and the device.  Lengths must lie in [0, 4096]. */
+int pmhd_set_sweep(pmhd_handle* h, int32_t seg_x, int32_t seg_y, int32_t seg_z);
+
 /* Wait for all queued work of the handle. */
 int pmhd_synchronize(pmhd_handle* h);
 

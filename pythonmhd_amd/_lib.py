@@ -13,7 +13,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_LIB = os.path.join(_HERE, 'libpmhd.so')
 
 EXPORTS = ('pmhd_create', 'pmhd_destroy', 'pmhd_set_state', 'pmhd_get_state', 'pmhd_compute_dt',
-           'pmhd_step', 'pmhd_advance', 'pmhd_synchronize', 'pmhd_diagnostics', 'pmhd_set_profiling',
+           'pmhd_step', 'pmhd_set_sweep', 'pmhd_advance', 'pmhd_synchronize', 'pmhd_diagnostics',
+           'pmhd_set_profiling',
            'pmhd_get_profile', 'pmhd_last_error',
            'pmhd_batch_create', 'pmhd_batch_destroy', 'pmhd_batch_set_state', 'pmhd_batch_get_state',
            'pmhd_batch_advance', 'pmhd_batch_autotune', 'pmhd_batch_get_tiles', 'pmhd_batch_synchronize',
@@ -54,6 +55,7 @@ def _declare(lib):
     lib.pmhd_get_state.argtypes = [H, ctypes.c_int32, P, P, P, P, P]
     lib.pmhd_compute_dt.argtypes = [H, ctypes.POINTER(ctypes.c_double)]
     lib.pmhd_step.argtypes = [H, ctypes.c_double]
+    lib.pmhd_set_sweep.argtypes = [H, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     lib.pmhd_advance.argtypes = [H, ctypes.c_double, ctypes.c_double, ctypes.c_int64, ctypes.c_int64,
                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                                  ctypes.POINTER(ctypes.c_double)]
@@ -222,6 +224,12 @@ class Solver:
 
     def step(self, dt):
         self._check(self.lib.pmhd_step(self.h, float(dt)), 'pmhd_step')
+
+    def set_sweep(self, seg):
+        """Faces per thread of the fused predictor sweep along (x, y, z); 0 selects
+        separate slope, edge and predictor launches (same results)."""
+        seg = list(seg) + [0] * (3 - len(seg))
+        self._check(self.lib.pmhd_set_sweep(self.h, int(seg[0]), int(seg[1]), int(seg[2])), 'pmhd_set_sweep')
 
     def advance(self, t0, t_end, cycle0, max_cycles):
         t = ctypes.c_double()

@@ -1,9 +1,16 @@
 """Build libpmhd.so (HIP, gfx950) in-tree, and the host test build of the same
 kernel bodies used by tests/ (tests/hostbuild/libpmhd_host.so).
 
+The library has four translation units, compiled in parallel and linked into one
+shared object: the FP64 step (IEEE arithmetic in the reference's operation order,
+no contraction), the FP32-bulk step (contracted multiply-adds, hardware FP32
+division and square root; its double-precision islands stay IEEE), the adjoint
+(IEEE) and the C ABI (host code).
+
     python -m pythonmhd_amd.build            # both
-    python -m pythonmhd_amd.build --gpu-only
+    python -m pythonmhd_amd.build --gpu-only [--only f32,abi]
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -12,18 +19,27 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, 'pythonmhd_amd', 'csrc')
 INC = os.path.join(ROOT, 'include')
 GPU_LIB = os.path.join(ROOT, 'pythonmhd_amd', 'libpmhd.so')
-HOST_SRC = os.path.join(ROOT, 'tests', 'hostbuild', 'pmhd_host.cpp')
-HOST_LIB = os.path.join(ROOT, 'tests', 'hostbuild', 'libpmhd_host.so')
+GPU_OBJ = os.path.join(ROOT, 'pythonmhd_amd', '_build')
+HOST_DIR = os.path.join(ROOT, 'tests', 'hostbuild')
+HOST_LIB = os.path.join(HOST_DIR, 'libpmhd_host.so')
+HOST_OBJ = os.path.join(HOST_DIR, '_build')
 ARCH = os.environ.get('PYTORCH_ROCM_ARCH', 'gfx950')
 
-HIPCC_FLAGS = [
-    '--offload-arch=' + ARCH, '-O3', '-std=c++17', '-fPIC', '-shared',
-    # keep the reference's operation order: no contraction into FMA
-    '-ffp-contract=off',
+HIPCC_COMMON = [
+    '--offload-arch=' + ARCH, '-O3', '-std=c++17', '-fPIC',
     # no packed FP32 instructions
     '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
     '-Wno-unused-result', '-Wno-unused-value',
 ]
+# keep the reference's operation order: no contraction into FMA, IEEE division/sqrt
+IEEE = ['-ffp-contract=off']
+FAST32 = ['-ffp-contract=fast', '-fno-hip-fp32-correctly-rounded-divide-sqrt', '-fno-signed-zeros',
+          '-fgpu-flush-denormals-to-zero']
+
+GPU_UNITS = [('pmhd_hip_f64.hip', IEEE), ('pmhd_hip_f32.hip', FAST32), ('pmhd_hip_adj.hip', IEEE)]
+HOST_UNITS = [(os.path.join(HOST_DIR, 'pmhd_host_f64.cpp'), []), (os.path.join(HOST_DIR, 'pmhd_host_f32.cpp'), []),
+              (os.path.join(HOST_DIR, 'pmhd_host_adj.cpp'), []), (os.path.join(CSRC, 'pmhd_abi.cpp'), [])]
+GXX_COMMON = ['-O2', '-std=c++17', '-fPIC', '-ffp-contract=off', '-Wno-unknown-pragmas']
 
 
 def _run(cmd):
@@ -31,26 +47,62 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build_gpu():
+def _parallel(cmds):
+    with concurrent.futures.ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
+
+
+def _obj(dirname, src):
+    return os.path.join(dirname, os.path.splitext(os.path.basename(src))[0] + '.o')
+
+
+def build_gpu(only=None):
+    """Compile every unit (or only the units whose file names contain one of the
+    strings in `only`, reusing the other objects) and link libpmhd.so."""
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+    os.makedirs(GPU_OBJ, exist_ok=True)
+    inc = ['-I' + INC, '-I' + CSRC]
+    cmds, objs = [], []
+    for src, flags in GPU_UNITS:
+        o = _obj(GPU_OBJ, src)
+        cmds.append([hipcc] + HIPCC_COMMON + flags + inc + ['-c', os.path.join(CSRC, src), '-o', o])
+        objs.append(o)
+    abi = _obj(GPU_OBJ, 'pmhd_abi.cpp')
+    cmds.append(['g++'] + GXX_COMMON + inc + ['-c', os.path.join(CSRC, 'pmhd_abi.cpp'), '-o', abi])
+    objs.append(abi)
+    if only:
+        cmds = [c for c, o in zip(cmds, objs) if any(u in os.path.basename(o) for u in only) or not os.path.exists(o)]
+    _parallel(cmds)
     tmp = GPU_LIB + '.tmp'
-    _run([hipcc] + HIPCC_FLAGS + ['-I' + INC, '-I' + CSRC, os.path.join(CSRC, 'pmhd_hip.hip'), '-o', tmp])
+    _run([hipcc, '--offload-arch=' + ARCH, '-shared', '-fPIC', '--hip-link'] + objs + ['-o', tmp])
     os.replace(tmp, GPU_LIB)
     return GPU_LIB
 
 
 def build_host():
+    os.makedirs(HOST_OBJ, exist_ok=True)
+    inc = ['-I' + INC, '-I' + CSRC]
+    cmds, objs = [], []
+    for src, flags in HOST_UNITS:
+        o = _obj(HOST_OBJ, src)
+        cmds.append(['g++'] + GXX_COMMON + ['-fopenmp'] + flags + inc + ['-c', src, '-o', o])
+        objs.append(o)
+    _parallel(cmds)
     tmp = HOST_LIB + '.tmp'
-    _run(['g++', '-O2', '-std=c++17', '-fPIC', '-shared', '-ffp-contract=off', '-fopenmp',
-          '-Wno-unknown-pragmas', '-I' + INC, '-I' + CSRC, HOST_SRC, '-o', tmp])
+    _run(['g++', '-shared', '-fPIC', '-fopenmp'] + objs + ['-o', tmp])
     os.replace(tmp, HOST_LIB)
     return HOST_LIB
 
 
 def main(argv):
-    build_gpu()
-    if '--gpu-only' not in argv:
-        build_host()
+    only = argv[argv.index('--only') + 1].split(',') if '--only' in argv else None
+    if '--gpu-only' in argv:
+        build_gpu(only)
+    else:
+        with concurrent.futures.ThreadPoolExecutor(max_workers=2) as ex:
+            for f in [ex.submit(build_gpu), ex.submit(build_host)]:
+                f.result()
 
 
 if __name__ == '__main__':

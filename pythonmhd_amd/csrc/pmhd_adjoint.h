@@ -22,6 +22,7 @@
 struct pmhd_adjoint {
   virtual ~pmhd_adjoint() {}
   std::string err;
+  virtual int init(const pmhd_grid& g, const pmhd_params& p, int checkpoint_every) = 0;
   virtual int forward(const double* prim, const double* cons, const double* const fin[3], double dt,
                       int64_t nsteps, double* prim_out, double* cons_out, double* const fout[3]) = 0;
   virtual int backward(const double* gprim, const double* gcons, const double* const gf[3], double* gprim0,
@@ -34,7 +35,11 @@ struct pmhd_adjoint {
   }
 };
 
-namespace pmhd {
+// constructors of the adjoint handles (defined in the adjoint's translation unit)
+pmhd_adjoint* pmhd_new_adjoint_f64();
+pmhd_adjoint* pmhd_new_adjoint_f32();
+
+PMHD_NS_BEGIN
 
 // the element function of a phase (the update also yields CFL speeds, unused here)
 template <class Ph, class T>
@@ -75,6 +80,7 @@ PMHD_FI void make_dual_ctx(const Ctx<V>& c, const Ctx<double>& a, AdState* st, C
   d.bx1d = D(double(c.bx1d));
   for (int q = 0; q < 3; ++q) {
     d.bc[q] = c.bc[q];
+    d.seg[q] = c.seg[q];
     d.dtd[q] = D(double(c.dtd[q]));
     d.hdtd[q] = D(double(c.hdtd[q]));
     d.invd[q] = D(double(c.invd[q]));
@@ -216,14 +222,22 @@ struct TapeExec {
     av = abase;
     bind_step(av, *adjs, Ph::id);
   }
+  bool fused = false;   // a fused launch was met (it has no per-element adjoint)
   template <class Ph>
   void run(const Ctx<T>& c) {
-    Ctx<T> cv;
-    Ctx<double> av;
-    record<Ph>(c, cv, av);
-    be->exec().template run<Ph>(cv);
-    B* b = be;
-    tape.push_back([b, cv, av] { b->template adjoint_phase<Ph>(cv, av); });
+    if constexpr (Ph::kind == EXT_SEG) {
+      // the adjoint's contexts keep seg = 0: separate slope, edge and predictor
+      // launches, whose elements are small enough to differentiate one by one
+      (void)c;
+      fused = true;
+    } else {
+      Ctx<T> cv;
+      Ctx<double> av;
+      record<Ph>(c, cv, av);
+      be->exec().template run<Ph>(cv);
+      B* b = be;
+      tape.push_back([b, cv, av] { b->template adjoint_phase<Ph>(cv, av); });
+    }
   }
   template <class Ph>
   void run_max(const Ctx<T>& c) {
@@ -292,7 +306,7 @@ struct AdjointHandle : public pmhd_adjoint {
       if (nface(d) > 0) be.d2d(fb[d], fb0[d], sizeof(T) * size_t(nface(d)));
   }
 
-  int init(const pmhd_grid& g_, const pmhd_params& p_, int every_) {
+  int init(const pmhd_grid& g_, const pmhd_params& p_, int every_) override {
     g = g_;
     p = p_;
     every = every_ > 0 ? every_ : 1;
@@ -415,6 +429,7 @@ struct AdjointHandle : public pmhd_adjoint {
         for (int d = 0; d < 3; ++d)
           if (nface(d) > 0) be.d2d(adjs.fb2[d], adj_state[out][2 + d], sizeof(double) * size_t(nface(d)));
         run_cycle(tx, cs);
+        if (tx.fused) return fail(PMHD_EINVAL, "the adjoint needs separate predictor launches (seg = 0)");
         for (size_t t = tx.tape.size(); t-- > 0;) tx.tape[t]();
         if (be.adjoint_overflow()) return fail(PMHD_EINVAL, "adjoint element stencil larger than supported");
         out = in;
@@ -446,8 +461,8 @@ struct AdjointHandle : public pmhd_adjoint {
   }
 };
 
-template <class B>
-int create_adjoint(const pmhd_grid* g, const pmhd_params* p, int every, pmhd_adjoint** out, std::string& err) {
+inline int create_adjoint(const pmhd_grid* g, const pmhd_params* p, int every, pmhd_adjoint** out,
+                          std::string& err) {
   if (!g || !p || !out) {
     err = "null argument";
     return PMHD_EINVAL;
@@ -464,18 +479,9 @@ int create_adjoint(const pmhd_grid* g, const pmhd_params* p, int every, pmhd_adj
     return PMHD_EINVAL;
   }
   const pmhd_grid gg = normal_grid(*g);
-  pmhd_adjoint* h = nullptr;
-  if (p->precision == 64) {
-    auto* hh = new (std::nothrow) AdjointHandle<double, B>();
-    if (!hh) { err = "out of host memory"; return PMHD_ENOMEM; }
-    rc = hh->init(gg, *p, every);
-    h = hh;
-  } else {
-    auto* hh = new (std::nothrow) AdjointHandle<float, B>();
-    if (!hh) { err = "out of host memory"; return PMHD_ENOMEM; }
-    rc = hh->init(gg, *p, every);
-    h = hh;
-  }
+  pmhd_adjoint* h = p->precision == 64 ? pmhd_new_adjoint_f64() : pmhd_new_adjoint_f32();
+  if (!h) { err = "out of host memory"; return PMHD_ENOMEM; }
+  rc = h->init(gg, *p, every);
   if (rc != PMHD_OK) {
     err = h->err;
     delete h;
@@ -485,14 +491,18 @@ int create_adjoint(const pmhd_grid* g, const pmhd_params* p, int every, pmhd_adj
   return PMHD_OK;
 }
 
-}  // namespace pmhd
+PMHD_NS_END
 
-#define PMHD_DEFINE_ADJOINT_ABI(BACKEND)                                                            \
+#define PMHD_DEFINE_ADJOINT_FACTORIES(BACKEND)                                                      \
+  pmhd_adjoint* pmhd_new_adjoint_f64() { return new (std::nothrow) pmhd::AdjointHandle<double, BACKEND>(); } \
+  pmhd_adjoint* pmhd_new_adjoint_f32() { return new (std::nothrow) pmhd::AdjointHandle<float, BACKEND>(); }
+
+#define PMHD_DEFINE_ADJOINT_ABI()                                                                   \
   static std::string g_pmhd_adj_error;                                                              \
   extern "C" int pmhd_adjoint_create(const pmhd_grid* g, const pmhd_params* p,                      \
                                      int32_t checkpoint_every, pmhd_adjoint** out) {                \
     try {                                                                                           \
-      return pmhd::create_adjoint<BACKEND>(g, p, checkpoint_every, out, g_pmhd_adj_error);          \
+      return pmhd::create_adjoint(g, p, checkpoint_every, out, g_pmhd_adj_error);                   \
     } catch (...) {                                                                                 \
       g_pmhd_adj_error = "unexpected exception in pmhd_adjoint_create";                             \
       return PMHD_EINVAL;                                                                           \

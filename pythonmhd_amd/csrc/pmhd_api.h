@@ -22,6 +22,7 @@ struct pmhd_handle {
   std::string err;
   pmhd_grid g;
   pmhd_params p;
+  virtual int init(const pmhd_grid&, const pmhd_params&) = 0;
   virtual int set_state(const double*, const double*, const double*, const double*, const double*) = 0;
   virtual int get_state(int, double*, double*, double*, double*, double*) = 0;
   virtual int compute_dt(double*) = 0;
@@ -31,16 +32,27 @@ struct pmhd_handle {
   virtual int set_profiling(int on) = 0;
   virtual int get_profile(int maxn, char* names, double* ms, int64_t* launches, int64_t* elems,
                           int32_t* n) = 0;
+  virtual int set_sweep(const int32_t seg[3]) = 0;
   int fail(int code, const std::string& m) {
     err = m;
     return code;
   }
 };
 
+// Constructors of the handles of one precision (FP64 or FP32 bulk), each defined
+// in the translation unit compiled for that precision; the result still needs
+// init().  nullptr when out of host memory.
+pmhd_handle* pmhd_new_handle_f64();
+pmhd_handle* pmhd_new_handle_f32();
+struct pmhd_batch;
+pmhd_batch* pmhd_new_batch_f64();
+pmhd_batch* pmhd_new_batch_f32();
+
 struct pmhd_batch {
   virtual ~pmhd_batch() {}
   std::string err;
   int nbox = 0;
+  virtual int init(int n, const pmhd_grid* grids, const pmhd_params& p) = 0;
   virtual int set_state(int box, const double*, const double*, const double*, const double*, const double*) = 0;
   virtual int get_state(int box, int which, double*, double*, double*, double*, double*) = 0;
   virtual int advance(double t0, double t_end, int64_t cycle0, int64_t max_cycles, double* t_out,
@@ -58,7 +70,7 @@ struct pmhd_batch {
   }
 };
 
-namespace pmhd {
+PMHD_NS_BEGIN
 
 // ---------------------------------------------------------------------------
 // batch launch tables: per launch id, every box's index box and the tile shape
@@ -172,6 +184,17 @@ void init_ctx(Ctx<T>& c, const pmhd_grid& g, const pmhd_params& p) {
     c.wd[d] = w[d];
     c.invd[d] = T(1.0 / w[d]);
   }
+}
+
+// Faces per thread of the fused predictor sweep along each direction (Ctx::seg):
+// the largest power of two, at most 32, that still leaves about 2048 threads per
+// compute unit for nfaces[d] faces, and at least 2 (a run of one face reconstructs
+// two cells).  Long runs amortise the cell reconstructed before each run.
+inline int sweep_segment(int64_t nfaces, int cus) {
+  const int64_t target = int64_t(cus > 0 ? cus : 256) * 2048;
+  int s = 2;
+  while (s < 32 && nfaces / (2 * s) >= target) s *= 2;
+  return s;
 }
 
 // the two state buffers of a grid (the update writes the other one)
@@ -313,7 +336,7 @@ struct Handle : public pmhd_handle {
     for (void* ptr : owned) be.release(ptr);
   }
 
-  int init(const pmhd_grid& g_, const pmhd_params& p_) {
+  int init(const pmhd_grid& g_, const pmhd_params& p_) override {
     g = g_;
     p = p_;
     init_ctx(c, g, p);
@@ -325,6 +348,13 @@ struct Handle : public pmhd_handle {
     bind_bufs(c, sb, cur);
     const char* e = be.error();
     if (e) return fail(PMHD_EHIP, e);
+    return PMHD_OK;
+  }
+
+  int set_sweep(const int32_t seg[3]) override {
+    for (int d = 0; d < 3; ++d)
+      if (seg[d] < 0 || seg[d] > 4096) return fail(PMHD_EINVAL, "sweep run length must be in [0, 4096]");
+    for (int d = 0; d < 3; ++d) c.seg[d] = d < g.ndim ? seg[d] : 0;
     return PMHD_OK;
   }
 
@@ -440,7 +470,7 @@ struct BatchHandle : public pmhd_batch {
     return ptr;
   }
 
-  int init(int n, const pmhd_grid* gs, const pmhd_params& p_) {
+  int init(int n, const pmhd_grid* gs, const pmhd_params& p_) override {
     nbox = n;
     p = p_;
     grids.assign(gs, gs + n);
@@ -694,8 +724,7 @@ inline pmhd_grid normal_grid(const pmhd_grid& g) {
   return gg;
 }
 
-template <class B>
-int create_handle(const pmhd_grid* g, const pmhd_params* p, pmhd_handle** out, std::string& err) {
+inline int create_handle(const pmhd_grid* g, const pmhd_params* p, pmhd_handle** out, std::string& err) {
   if (!g || !p || !out) {
     err = "null argument";
     return PMHD_EINVAL;
@@ -708,18 +737,9 @@ int create_handle(const pmhd_grid* g, const pmhd_params* p, pmhd_handle** out, s
   rc = check_params(p, err);
   if (rc) return rc;
   const pmhd_grid gg = normal_grid(*g);
-  pmhd_handle* h = nullptr;
-  if (p->precision == 64) {
-    auto* hh = new (std::nothrow) Handle<double, B>();
-    if (!hh) { err = "out of host memory"; return PMHD_ENOMEM; }
-    rc = hh->init(gg, *p);
-    h = hh;
-  } else {
-    auto* hh = new (std::nothrow) Handle<float, B>();
-    if (!hh) { err = "out of host memory"; return PMHD_ENOMEM; }
-    rc = hh->init(gg, *p);
-    h = hh;
-  }
+  pmhd_handle* h = p->precision == 64 ? pmhd_new_handle_f64() : pmhd_new_handle_f32();
+  if (!h) { err = "out of host memory"; return PMHD_ENOMEM; }
+  rc = h->init(gg, *p);
   if (rc != PMHD_OK) {
     err = h->err;
     delete h;
@@ -729,8 +749,7 @@ int create_handle(const pmhd_grid* g, const pmhd_params* p, pmhd_handle** out, s
   return PMHD_OK;
 }
 
-template <class B>
-int create_batch(int nbox, const pmhd_grid* gs, const pmhd_params* p, pmhd_batch** out, std::string& err) {
+inline int create_batch(int nbox, const pmhd_grid* gs, const pmhd_params* p, pmhd_batch** out, std::string& err) {
   if (!gs || !p || !out) {
     err = "null argument";
     return PMHD_EINVAL;
@@ -746,18 +765,9 @@ int create_batch(int nbox, const pmhd_grid* gs, const pmhd_params* p, pmhd_batch
     if (gs[b].ndim != gs[0].ndim) { err = "all boxes of a batch must have the same ndim"; return PMHD_EINVAL; }
     gg[size_t(b)] = normal_grid(gs[b]);
   }
-  pmhd_batch* h = nullptr;
-  if (p->precision == 64) {
-    auto* hh = new (std::nothrow) BatchHandle<double, B>();
-    if (!hh) { err = "out of host memory"; return PMHD_ENOMEM; }
-    rc = hh->init(nbox, gg.data(), *p);
-    h = hh;
-  } else {
-    auto* hh = new (std::nothrow) BatchHandle<float, B>();
-    if (!hh) { err = "out of host memory"; return PMHD_ENOMEM; }
-    rc = hh->init(nbox, gg.data(), *p);
-    h = hh;
-  }
+  pmhd_batch* h = p->precision == 64 ? pmhd_new_batch_f64() : pmhd_new_batch_f32();
+  if (!h) { err = "out of host memory"; return PMHD_ENOMEM; }
+  rc = h->init(nbox, gg.data(), *p);
   if (rc != PMHD_OK) {
     err = h->err;
     delete h;
@@ -788,14 +798,19 @@ inline int advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int6
   return PMHD_OK;
 }
 
-}  // namespace pmhd
+PMHD_NS_END
 
-// C ABI bodies; the including translation unit names its backend.
-#define PMHD_DEFINE_C_ABI(BACKEND)                                                          \
+// The constructors of one precision, in the translation unit compiled for it.
+#define PMHD_DEFINE_FACTORIES(SUFFIX, T, BACKEND)                                           \
+  pmhd_handle* pmhd_new_handle_##SUFFIX() { return new (std::nothrow) pmhd::Handle<T, BACKEND>(); } \
+  pmhd_batch* pmhd_new_batch_##SUFFIX() { return new (std::nothrow) pmhd::BatchHandle<T, BACKEND>(); }
+
+// C ABI bodies (one translation unit; the handles come from the constructors above).
+#define PMHD_DEFINE_C_ABI()                                                                 \
   static std::string g_pmhd_create_error;                                                   \
   extern "C" int pmhd_create(const pmhd_grid* g, const pmhd_params* p, pmhd_handle** out) { \
     try {                                                                                   \
-      return pmhd::create_handle<BACKEND>(g, p, out, g_pmhd_create_error);                  \
+      return pmhd::create_handle(g, p, out, g_pmhd_create_error);                           \
     } catch (...) {                                                                         \
       g_pmhd_create_error = "unexpected exception in pmhd_create";                          \
       return PMHD_EINVAL;                                                                   \
@@ -824,6 +839,12 @@ inline int advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int6
   extern "C" int pmhd_step(pmhd_handle* h, double dt) {                                     \
     if (!h) return PMHD_EINVAL;                                                             \
     return h->step(dt);                                                                     \
+  }                                                                                         \
+  extern "C" int pmhd_set_sweep(pmhd_handle* h, int32_t seg_x, int32_t seg_y, int32_t seg_z) { \
+    if (!h) return PMHD_EINVAL;                                                             \
+    const int32_t seg[3] = {seg_x, seg_y, seg_z};                                           \
+    try { return h->set_sweep(seg); }                                                       \
+    catch (...) { return h->fail(PMHD_EINVAL, "exception in pmhd_set_sweep"); }             \
   }                                                                                         \
   extern "C" int pmhd_advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0,      \
                               int64_t max_cycles, double* t_out, int64_t* cycle_out,        \
@@ -856,7 +877,7 @@ inline int advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int6
   extern "C" int pmhd_batch_create(int32_t nbox, const pmhd_grid* grids,                    \
                                    const pmhd_params* p, pmhd_batch** out) {                \
     try {                                                                                   \
-      return pmhd::create_batch<BACKEND>(nbox, grids, p, out, g_pmhd_create_error);         \
+      return pmhd::create_batch(nbox, grids, p, out, g_pmhd_create_error);                  \
     } catch (...) {                                                                         \
       g_pmhd_create_error = "unexpected exception in pmhd_batch_create";                    \
       return PMHD_EINVAL;                                                                   \

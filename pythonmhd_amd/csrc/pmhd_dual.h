@@ -17,7 +17,7 @@
 
 #include "pmhd_math.h"
 
-namespace pmhd {
+PMHD_NS_BEGIN
 
 template <class V>
 struct ADual {
@@ -160,4 +160,4 @@ PMHD_FI void ad_prune(AdState& s) {
   s.nin = n;
 }
 
-}  // namespace pmhd
+PMHD_NS_END

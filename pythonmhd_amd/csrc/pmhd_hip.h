@@ -2,6 +2,9 @@
 // shuffle reductions for the CFL maximum, one HIP stream per handle.  A single
 // grid launches each phase over its own index box; a batch launches each phase
 // once over the tiles of every box (a persistent grid walks the tile list).
+// Included by the library's translation units (pmhd_hip_f64.hip, pmhd_hip_f32.hip,
+// pmhd_hip_adj.hip), each of which instantiates the handles of one kind.
+#pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -11,7 +14,7 @@
 
 #include "pmhd_adjoint.h"
 
-namespace pmhd {
+PMHD_NS_BEGIN
 
 // ---------------------------------------------------------------------------
 // single-grid kernels
@@ -23,6 +26,17 @@ __global__ void __launch_bounds__(256) k_phase(const Ctx<T> c, int n1, int n2) {
   const int i = int(blockIdx.y * blockDim.y + threadIdx.y);
   const int j = int(blockIdx.z);
   if (k < n2 && i < n1) Ph()(c, I3{j, i, k});
+}
+
+// one thread per element of an (n0, n1, n2) box numbered flat with n2 fastest:
+// launches over runs of faces, whose fastest axis may be short (the runs along z)
+template <class Ph, class T>
+__global__ void __launch_bounds__(256) k_phase_flat(const Ctx<T> c, unsigned n1, unsigned n2, unsigned total) {
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const unsigned k = t % n2;
+  const unsigned r = t / n2;
+  Ph()(c, I3{int64_t(r / n1), int64_t(r % n1), int64_t(k)});
 }
 
 __device__ inline unsigned long long umax64(unsigned long long a, unsigned long long b) {
@@ -329,6 +343,8 @@ struct HipExec {
     pend.push_back(Rec{id, n, a, b});
   }
 
+  int flat_block = 256;   // threads per block of the flat launches (64..256)
+
   static void grid_of(const int64_t n[3], dim3& grid, dim3& block) {
     unsigned bx = 1;
     while (bx < 64 && int64_t(bx) < n[2]) bx <<= 1;
@@ -342,9 +358,19 @@ struct HipExec {
     int64_t n[3];
     c.extent(Ph::kind, Ph::dir, n);
     if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return;
+    const int64_t total = n[0] * n[1] * n[2];
+    if (Ph::kind == EXT_SEG && total < (int64_t(1) << 31)) {
+      const unsigned nb = unsigned((total + flat_block - 1) / flat_block);
+      timed(Ph::id, total, [&] {
+        hipLaunchKernelGGL((k_phase_flat<Ph, T>), dim3(nb), dim3(unsigned(flat_block)), 0, s, c, unsigned(n[1]),
+                           unsigned(n[2]), unsigned(total));
+      });
+      note(hipGetLastError());
+      return;
+    }
     dim3 grid, block;
     grid_of(n, grid, block);
-    timed(Ph::id, n[0] * n[1] * n[2], [&] {
+    timed(Ph::id, total, [&] {
       hipLaunchKernelGGL((k_phase<Ph, T>), grid, block, 0, s, c, int(n[1]), int(n[2]));
     });
     note(hipGetLastError());
@@ -617,7 +643,7 @@ struct HipBackend {
     while (bx < 64 && int64_t(bx) < n[2]) bx <<= 1;
     const unsigned by = 64 / bx;
     dim3 grid(unsigned((n[2] + bx - 1) / bx), unsigned((n[1] + by - 1) / by), unsigned(n[0]));
-    ex.timed(K_COUNT - 1, n[0] * n[1] * n[2], [&] {
+    ex.timed(K_ADJOINT, n[0] * n[1] * n[2], [&] {
       hipLaunchKernelGGL((k_adjoint<Ph, T>), grid, dim3(bx, by, 1), 0, ex.s, c, a, int(n[1]), int(n[2]), adj_flag);
     });
     ex.note(hipGetLastError());
@@ -656,7 +682,4 @@ struct HipBackend {
   }
 };
 
-}  // namespace pmhd
-
-PMHD_DEFINE_C_ABI(pmhd::HipBackend)
-PMHD_DEFINE_ADJOINT_ABI(pmhd::HipBackend)
+PMHD_NS_END

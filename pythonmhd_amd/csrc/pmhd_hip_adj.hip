@@ -1,0 +1,6 @@
+// Adjoint handles of libpmhd.so (FP64 and FP32), with IEEE semantics throughout:
+// each phase's adjoint evaluates the phase's kernel body on dual numbers.
+#define PMHD_TU_NS adj
+#include "pmhd_hip.h"
+
+PMHD_DEFINE_ADJOINT_FACTORIES(pmhd::HipBackend)

@@ -12,7 +12,7 @@
 #pragma once
 #include "pmhd_step.h"
 
-namespace pmhd {
+PMHD_NS_BEGIN
 namespace hydro {
 
 constexpr int NV = 5;
@@ -594,4 +594,4 @@ void run_cycle(X& ex, const Ctx<T>& c) {
   else cycle(ex, c);
 }
 
-}  // namespace pmhd
+PMHD_NS_END

@@ -10,24 +10,26 @@
 #pragma once
 #include "pmhd_math.h"
 
-namespace pmhd {
+PMHD_NS_BEGIN
 
 enum { OUTFLOW = 0, PERIODIC = 1 };
 // element kinds a launch runs over
-enum ExtKind { EXT_CELL = 0, EXT_FACE = 1, EXT_SLOPE = 2, EXT_EDGE = 3 };
+// EXT_SEG: runs of consecutive faces along the launch direction (one run per thread)
+enum ExtKind { EXT_CELL = 0, EXT_FACE = 1, EXT_SLOPE = 2, EXT_EDGE = 3, EXT_SEG = 4 };
 
 // launch identifiers (for per-kernel timing); "+ d" for the per-direction ones
 enum KernelId {
   K_SLOPES = 0, K_EDGES = 3, K_COPY = 6, K_PREDICT = 9, K_CELL_E = 12, K_EMF = 13, K_EMF2 = 14,
   K_CT_HALF = 17, K_HALF_E = 20, K_CORRECT = 21, K_CT_FULL = 24, K_UPDATE = 27, K_CFL = 28,
-  K_DIAG = 29, K_COUNT = 30
+  K_DIAG = 29, K_SWEEP = 30, K_ADJOINT = 33, K_COUNT = 34
 };
 inline const char* kernel_name(int id) {
   static const char* names[K_COUNT] = {
       "slopes_x", "slopes_y", "slopes_z", "edges_x", "edges_y", "edges_z", "copy_x", "copy_y", "copy_z",
       "predict_x", "predict_y", "predict_z", "cell_e", "emf", "emf_corr", "unused_15", "unused_16",
       "ct_half_x", "ct_half_y", "ct_half_z", "half_e", "ctu_correct_x", "ctu_correct_y", "ctu_correct_z",
-      "ct_full_x", "ct_full_y", "ct_full_z", "update", "cfl", "diag"};
+      "ct_full_x", "ct_full_y", "ct_full_z", "update", "cfl", "diag", "sweep_x", "sweep_y", "sweep_z",
+      "adjoint"};
   return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 // element kind (ExtKind) and direction of each launch id
@@ -44,6 +46,7 @@ inline void phase_shape(int id, int& kind, int& dir) {
     kind = EXT_FACE;
     dir = id < K_CT_FULL ? id - K_CORRECT : id - K_CT_FULL;
   }
+  else if (id >= K_SWEEP && id < K_SWEEP + 3) { kind = EXT_SEG; dir = id - K_SWEEP; }
 }
 
 // pointer types of a context's arrays (the adjoint's dual contexts use accessors)
@@ -67,6 +70,8 @@ struct Ctx {
   int bc_zrec;       // boundary condition the z reconstruction/corrections use
   int order;         // 0 copy, 1 PLM, 2 PPM
   int efix;
+  int seg[3];        // faces per thread of the fused predictor sweep along x, y, z (0: separate
+                     // slope / edge / predictor launches)
   T gamma, eps, min_dens, min_pres, min_energy;
   // per-cycle scalars (all formed on the host in double)
   T dt, hdt;
@@ -125,6 +130,10 @@ struct Ctx {
       n[0] = sny(d); n[1] = snx(d); n[2] = snz(d);
     } else if (kind == EXT_EDGE) {
       n[0] = eny(); n[1] = enx(); n[2] = enz();
+    } else if (kind == EXT_SEG) {
+      n[0] = fny(d); n[1] = fnx(d); n[2] = fnz(d);
+      const int64_t s = seg[d] > 0 ? seg[d] : 1;
+      n[d == 0 ? 1 : (d == 1 ? 0 : 2)] = (n_along(d) + s) / s;   // runs over n + 1 faces
     } else {
       n[0] = ny; n[1] = nx; n[2] = nz;
     }
@@ -150,10 +159,15 @@ PMHD_FI I3 unflat(int64_t idx, int64_t ny, int64_t nx, int64_t nz) {
 }
 
 // ghost maps along one axis of n cells
-PMHD_FI int64_t map2(int64_t e, int64_t n, int bc) {  // reconstruction (2 ghosts)
+// reconstruction (2 ghosts): callers pass -2n <= e < 3n, so two conditional wraps
+// give e mod n without a division
+PMHD_FI int64_t map2(int64_t e, int64_t n, int bc) {
   if (bc == OUTFLOW) return e < 0 ? 0 : (e >= n ? n - 1 : e);
-  int64_t m = e % n;
-  return m < 0 ? m + n : m;
+  if (e < 0) e += n;
+  if (e < 0) e += n;
+  if (e >= n) e -= n;
+  if (e >= n) e -= n;
+  return e;
 }
 PMHD_FI int64_t map1(int64_t e, int64_t n, int bc) {  // one ghost each side
   if (e < 0) return bc == OUTFLOW ? 0 : n - 1;
@@ -262,9 +276,13 @@ PMHD_FI void k_slopes(const Ctx<T>& c, int d, int bc, I3 q) {
   store7(c.S, c.nslope(d), idx, ps);
 }
 
-// edges and characteristic tracing of each cell; writes the two faces it owns
+// traced edge states of cell p (coordinate e along d, 0 <= e < n) in directional
+// order: dla is the right state of face e, dra the left state of face e+1.  The
+// slopes of cells e-1, e, e+1 come from the slope array (PPM); with from_s false
+// (PLM) the cell's own limited slope is formed here from the same eigen-system as
+// the tracing, which is what the slope launch computes for that cell.
 template <class T>
-PMHD_FI void k_edges(const Ctx<T>& c, int d, int bc, I3 p) {
+PMHD_FI void cell_edge_states(const Ctx<T>& c, int d, int bc, I3 p, bool from_s, T dla[7], T dra[7]) {
   const int64_t N = c.ncell();
   const int64_t idx = c.cid(p.j, p.i, p.k);
   const int64_t n = c.n_along(d);
@@ -278,20 +296,34 @@ PMHD_FI void k_edges(const Ctx<T>& c, int d, int bc, I3 p) {
   T bn = c.W[(4 + d) * N + idx];
   PrimEig<T> E;
   eigen_prim(wc, bn, c.gamma, E);
-  const int64_t NS = c.nslope(d);
-  auto sidx = [&](int64_t ee) {
-    I3 s = with(p, d, ee + 1);
-    return (s.j * c.snx(d) + s.i) * c.snz(d) + s.k;
-  };
-  T sl[7], sc[7], sr[7];
-  load7(c.S, NS, sidx(e - 1), sl);
-  load7(c.S, NS, sidx(e), sc);
-  load7(c.S, NS, sidx(e + 1), sr);
   T la[7], ra[7];
-  trace_cell(E, wl, wc, wr, sl, sc, sr, c.dtd[d], c.order, la, ra);
-  T dla[7], dra[7];
+  if (from_s) {
+    const int64_t NS = c.nslope(d);
+    auto sidx = [&](int64_t ee) {
+      I3 s = with(p, d, ee + 1);
+      return (s.j * c.snx(d) + s.i) * c.snz(d) + s.k;
+    };
+    T sl[7], sc[7], sr[7];
+    load7(c.S, NS, sidx(e - 1), sl);
+    load7(c.S, NS, sidx(e), sc);
+    load7(c.S, NS, sidx(e + 1), sr);
+    trace_cell(E, wl, wc, wr, sl, sc, sr, c.dtd[d], c.order, la, ra);
+  } else {
+    T sc[7];
+    limited_slope(E, wl, wc, wr, sc);
+    trace_cell(E, wl, wc, wr, sc, sc, sc, c.dtd[d], c.order, la, ra);
+  }
   from_rot(la, d, dla);
   from_rot(ra, d, dra);
+}
+
+// edges and characteristic tracing of each cell; writes the two faces it owns
+template <class T>
+PMHD_FI void k_edges(const Ctx<T>& c, int d, int bc, I3 p) {
+  const int64_t n = c.n_along(d);
+  const int64_t e = co(p, d);
+  T dla[7], dra[7];
+  cell_edge_states(c, d, bc, p, true, dla, dra);
   const int64_t NF = c.nface(d);
   I3 f0 = with(p, d, e);
   I3 f1 = with(p, d, e + 1);
@@ -369,15 +401,14 @@ PMHD_FI void recon_b_term_3d(const Ctx<T>& c, int d, int bc, I3 p, int64_t e, T 
   out[1] = c.hdt * (v2 * term2);
 }
 
+// predictor of face f from its reconstructed states wl, wr (directional order;
+// modified in place)
 template <class T>
-PMHD_FI void k_predict(const Ctx<T>& c, int d, I3 f) {
+PMHD_FI void predict_face(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7]) {
   const int64_t N = c.ncell();
   const int64_t NF = c.nface(d);
   const int64_t idx = c.fid(d, f.j, f.i, f.k);
   const int64_t e = co(f, d);
-  T wl[7], wr[7];
-  load7(c.WL, NF, idx, wl);
-  load7(c.WR, NF, idx, wr);
   apply_floors(wl, c.min_dens, c.min_pres);
   apply_floors(wr, c.min_dens, c.min_pres);
   T bn;
@@ -429,6 +460,68 @@ PMHD_FI void k_predict(const Ctx<T>& c, int d, I3 f) {
   roe_flux(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf);
   from_rot(rf, d, df);
   store7(c.F[d], NF, idx, df);
+}
+
+template <class T>
+PMHD_FI void k_predict(const Ctx<T>& c, int d, I3 f) {
+  const int64_t NF = c.nface(d);
+  const int64_t idx = c.fid(d, f.j, f.i, f.k);
+  T wl[7], wr[7];
+  load7(c.WL, NF, idx, wl);
+  load7(c.WR, NF, idx, wr);
+  predict_face(c, d, f, wl, wr);
+}
+
+// ===========================================================================
+// fused predictor sweep: reconstruction, tracing and the predictor of a run of
+// seg[d] consecutive faces along d.  q's coordinate along d is the run index;
+// the run walks its faces in order and keeps the left state of the next face
+// (the right edge state of the cell before it) in registers, so each face costs
+// one cell reconstruction, plus one for the cell before the run.  The face
+// states are those the edge launch would store: face 0 takes the left edge of
+// cell 0 on both sides with outflow and the right edge of cell n-1 on its left
+// when periodic; face n takes the right edge of cell n-1 on both sides with
+// outflow and the left edge of cell 0 on its right when periodic.  order 2
+// reads the slopes of the slope launch; order 1 forms them here.
+// ===========================================================================
+template <class T>
+PMHD_FI void k_sweep(const Ctx<T>& c, int d, int bc, I3 q) {
+  const int64_t n = c.n_along(d);
+  const int64_t len = c.seg[d] > 0 ? c.seg[d] : 1;
+  const int64_t f0 = co(q, d) * len;
+  const int64_t f1 = f0 + len < n + 1 ? f0 + len : n + 1;
+  const bool from_s = c.order == 2;
+  T prev[7];
+  {
+    T la[7], ra[7];
+    if (f0 == 0 && bc == OUTFLOW) {
+      cell_edge_states(c, d, bc, with(q, d, 0), from_s, la, ra);
+#pragma unroll
+      for (int v = 0; v < 7; ++v) prev[v] = la[v];
+    } else {
+      cell_edge_states(c, d, bc, with(q, d, f0 == 0 ? n - 1 : f0 - 1), from_s, la, ra);
+#pragma unroll
+      for (int v = 0; v < 7; ++v) prev[v] = ra[v];
+    }
+  }
+  for (int64_t f = f0; f < f1; ++f) {
+    T wl[7], wr[7];
+#pragma unroll
+    for (int v = 0; v < 7; ++v) wl[v] = prev[v];
+    if (f < n || bc != OUTFLOW) {
+      T la[7], ra[7];
+      cell_edge_states(c, d, bc, with(q, d, f < n ? f : 0), from_s, la, ra);
+#pragma unroll
+      for (int v = 0; v < 7; ++v) {
+        wr[v] = la[v];
+        prev[v] = ra[v];
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 7; ++v) wr[v] = prev[v];
+    }
+    predict_face(c, d, with(q, d, f), wl, wr);
+  }
 }
 
 // ===========================================================================
@@ -902,4 +995,4 @@ PMHD_FI void k_diag(const Ctx<T>& c, int64_t idx, double sums[9], double maxes[2
   maxes[1] = bmax;
 }
 
-}  // namespace pmhd
+PMHD_NS_END

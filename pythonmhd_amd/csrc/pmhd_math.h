@@ -23,7 +23,17 @@
 #endif
 #define PMHD_FI PMHD_HD inline __attribute__((always_inline))
 
-namespace pmhd {
+// Each translation unit of the library (FP64 step, FP32 step, adjoint, C ABI) is
+// compiled with its own floating-point flags; its copy of the code lives in its
+// own inline namespace so that no inline function of one unit is merged with the
+// same-named function of another at link time.
+#ifndef PMHD_TU_NS
+#define PMHD_TU_NS unit
+#endif
+#define PMHD_NS_BEGIN namespace pmhd { inline namespace PMHD_TU_NS {
+#define PMHD_NS_END } }
+
+PMHD_NS_BEGIN
 
 constexpr double kSmall = 0.000000000001;        // Source/PythonMHD_Constants.py:7
 constexpr double kFourThirds = 1.333333333333333;  // Source/PythonMHD_Constants.py:8
@@ -170,26 +180,28 @@ PMHD_FI void eigen_prim(const T w[7], T bn, T gamma, PrimEig<T>& E) {
   T gp = gamma * p;
   T inv = T(1) / rho;
   T sqd = sqrt_(rho);
-  // wave speeds and normalisation in N (double) precision
-  N n_inv = N(1) / N(rho);
-  N n_rbtsq = N(btsq) * n_inv;
-  N n_cs2 = N(gp) * n_inv;
-  N n_ca2 = N(bn) * N(bn) * n_inv;
-  N ssum = n_ca2 + n_rbtsq + n_cs2;
-  N sdiff = n_ca2 + n_rbtsq - n_cs2;
-  N ssqrt = sqrt_(sdiff * sdiff + N(4) * n_cs2 * n_rbtsq);
-  N n_cf2 = N(0.5) * (ssum + ssqrt);
-  N n_csl2 = n_cs2 * n_ca2 / n_cf2;
-  N af, as;
-  if (ssqrt == N(0)) {
-    af = N(1); as = N(0);
-  } else if ((n_cs2 - n_csl2) <= N(0)) {
-    af = N(0); as = N(1);
-  } else if ((n_cf2 - n_cs2) <= N(0)) {
-    af = N(1); as = N(0);
-  } else {
-    af = sqrt_((n_cs2 - n_csl2) / ssqrt);
-    as = sqrt_((n_cf2 - n_cs2) / ssqrt);
+  N n_cs2, n_ca2, n_cf2, n_csl2, af, as;
+  {
+    // wave speeds and normalisation in N (double) precision
+    N n_inv = N(1) / N(rho);
+    N n_rbtsq = N(btsq) * n_inv;
+    n_cs2 = N(gp) * n_inv;
+    n_ca2 = N(bn) * N(bn) * n_inv;
+    N ssum = n_ca2 + n_rbtsq + n_cs2;
+    N sdiff = n_ca2 + n_rbtsq - n_cs2;
+    N ssqrt = sqrt_(sdiff * sdiff + N(4) * n_cs2 * n_rbtsq);
+    n_cf2 = N(0.5) * (ssum + ssqrt);
+    n_csl2 = n_cs2 * n_ca2 / n_cf2;
+    if (ssqrt == N(0)) {
+      af = N(1); as = N(0);
+    } else if ((n_cs2 - n_csl2) <= N(0)) {
+      af = N(0); as = N(1);
+    } else if ((n_cf2 - n_cs2) <= N(0)) {
+      af = N(1); as = N(0);
+    } else {
+      af = sqrt_((n_cs2 - n_csl2) / ssqrt);
+      as = sqrt_((n_cf2 - n_cs2) / ssqrt);
+    }
   }
   const T cs2 = T(n_cs2);
   const T cs = sqrt_(cs2);
@@ -723,4 +735,4 @@ PMHD_FI T upwind(T m, T pos, T neg, T zero) {
   return m > T(0) ? pos : (m < T(0) ? neg : (m == T(0) ? zero : T(0)));
 }
 
-}  // namespace pmhd
+PMHD_NS_END

@@ -13,7 +13,7 @@
 #pragma once
 #include "pmhd_kernels.h"
 
-namespace pmhd {
+PMHD_NS_BEGIN
 
 // ---------------------------------------------------------------------------
 // phases
@@ -49,6 +49,12 @@ struct Predict {
   PMHD_PHASE_TRAITS(K_PREDICT + D, EXT_FACE, D)
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_predict(c, D, p); }
+};
+template <int D>
+struct Sweep {  // fused slopes (PLM) + edges + predictor over runs of faces
+  PMHD_PHASE_TRAITS(K_SWEEP + D, EXT_SEG, D)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 q) const { k_sweep(c, D, c.bc[D], q); }
 };
 struct CellE {
   PMHD_PHASE_TRAITS(K_CELL_E, EXT_CELL, 0)
@@ -124,9 +130,15 @@ void diagnostics(X& ex, const Ctx<T>& c, double sums[9], double maxes[2]) {
             sums, maxes);
 }
 
-// interface states and predictor fluxes of direction D
+// interface states and predictor fluxes of direction D: with c.seg[D] > 0 the
+// reconstruction and the predictor are one launch (PPM keeps its slope launch)
 template <int D, class T, class X>
 void predictor_sweep(X& ex, const Ctx<T>& c) {
+  if (c.order > 0 && c.seg[D] > 0) {
+    if (c.order == 2) ex.template run<Slopes<D>>(c);
+    ex.template run<Sweep<D>>(c);
+    return;
+  }
   if (c.order > 0) {
     ex.template run<Slopes<D>>(c);
     ex.template run<Edges<D>>(c);
@@ -318,4 +330,4 @@ PMHD_FI void box_end(Ctx<T>& c, BoxClock& k) {
   k.done += 1;
 }
 
-}  // namespace pmhd
+PMHD_NS_END

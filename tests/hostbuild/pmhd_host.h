@@ -2,6 +2,7 @@
 // INFRASTRUCTURE ONLY.  It compiles pythonmhd_amd/csrc/*.h with g++ so the
 // FP64 arithmetic of the HIP kernels can be checked against the oracle in a
 // container without a GPU.  The product package never loads this library.
+#pragma once
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -11,7 +12,7 @@
 
 #include "pmhd_adjoint.h"
 
-namespace pmhd {
+PMHD_NS_BEGIN
 
 template <class Ph, class T>
 void host_each(const Ctx<T>& c) {
@@ -239,7 +240,5 @@ struct HostBackend {
   int profile(int, char*, double*, int64_t*, int64_t*) { return 0; }
 };
 
-}  // namespace pmhd
+PMHD_NS_END
 
-PMHD_DEFINE_C_ABI(pmhd::HostBackend)
-PMHD_DEFINE_ADJOINT_ABI(pmhd::HostBackend)

@@ -1,0 +1,5 @@
+// Host test build: the adjoint handles.
+#define PMHD_TU_NS adj
+#include "pmhd_host.h"
+
+PMHD_DEFINE_ADJOINT_FACTORIES(pmhd::HostBackend)
