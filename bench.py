@@ -32,6 +32,8 @@ ALG_WORDS = {
     'edges': 8 + 7 + 14,    # prim + slope in, two interface states out (per cell)
     'copy': 7 + 14,
     'predict': 20 + 21,     # two states + face/cell B terms in; U_L, U_R, flux out (per face)
+    'sweep': 13 + 21,       # fused slopes+edges+predictor: prim (8), faces (3), momenta (2) of one
+                            # cell in; U_L, U_R, flux out (per face)
     'cell_e': 7 + 3,
     'emf': 12 + 3,          # three face-flux components per face array + cell E in (per edge point)
     'ctu_correct': 34 + 1 + 7,  # states, transverse fluxes, EMF, faces, cell B/v, half-step B in; flux out
@@ -89,11 +91,13 @@ def div_b(faces, h):
     return float(np.max(np.abs(d)) / bmax)
 
 
-def run_gpu(prim, cons, faces, n, precision, steps, warmup, profile_steps=3):
+def run_gpu(prim, cons, faces, n, precision, steps, warmup, profile_steps=3, sweep=None):
     from pythonmhd_amd._lib import Solver
     h = 1.0 / n
     s = Solver((n, n, n), (h, h, h), (1, 1, 1), GAMMA, 0.4, order=1, entropy_fix=False,
                precision=precision)
+    if sweep is not None:
+        s.set_sweep(sweep)
     s.set_state(prim, cons, faces)
     t, cyc, _ = s.advance(0.0, 1e30, 0, warmup)
     s.synchronize()
@@ -108,7 +112,7 @@ def run_gpu(prim, cons, faces, n, precision, steps, warmup, profile_steps=3):
         s.synchronize()
         prof = s.profile()
         s.set_profiling(False)
-    return s, el, prof
+    return s, el, prof, cyc + profile_steps
 
 
 def kh_boxes(nbox=256, seed=7):
@@ -331,7 +335,22 @@ def _cpu_model():
     return 'unknown'
 
 
-def roofline(prof, word_bytes):
+TRAFFIC = os.path.join(ROOT, 'profiles', 'hbm_traffic_fp32_turbulence256.json')
+
+
+def measured_traffic(kernel, n, word_bytes):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (calibrated against a plain device copy), for the workload
+    they were measured on (FP32, 256^3); None otherwise."""
+    if n != 256 or word_bytes != 4 or not os.path.exists(TRAFFIC):
+        return None
+    with open(TRAFFIC) as f:
+        t = json.load(f)
+    row = t.get('launches', {}).get(kernel)
+    return None if row is None else row['fetch_bytes'] + row['write_bytes']
+
+
+def roofline(prof, word_bytes, n=256):
     if not prof:
         return None
     best = max(prof.items(), key=lambda kv: kv[1][0])
@@ -342,7 +361,8 @@ def roofline(prof, word_bytes):
     bytes_per_launch = words * word_bytes * ne / nl
     ach = bytes_per_launch / per_launch_s / 1e9
     return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": round(ach / (HBM_PEAK / 1e9), 4), "traffic": None, "kernel": name,
+            "frac": round(ach / (HBM_PEAK / 1e9), 4), "traffic": measured_traffic(name, n, word_bytes),
+            "kernel": name,
             "avg_launch_ms": round(ms / nl, 4), "alg_bytes_per_launch": int(bytes_per_launch)}
 
 
@@ -354,6 +374,9 @@ def main():
     ap.add_argument('--n', type=int, default=256)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
+    ap.add_argument('--err-cycles', type=int, default=100,
+                    help='cycles after which the FP32 and FP64 states are compared')
+    ap.add_argument('--sweep', default='', help='fused sweep run lengths x,y,z (0: separate launches)')
     ap.add_argument('--no-kh', action='store_true')
     ap.add_argument('--kh-cycles', type=int, default=200)
     ap.add_argument('--no-adjoint', action='store_true')
@@ -379,7 +402,8 @@ def main():
     n, K, W = a.n, a.steps, a.warmup
     prim, cons, faces = turbulence_ic(n)
     print('bench: %d^3 turbulence IC ready' % n, file=sys.stderr, flush=True)
-    s32, el32, prof32 = run_gpu(prim, cons, faces, n, 32, K, W)
+    sweep = [int(x) for x in a.sweep.split(',')] if a.sweep else None
+    s32, el32, prof32, cyc32 = run_gpu(prim, cons, faces, n, 32, K, W, sweep=sweep)
     cells = n ** 3
     v32 = cells * K / el32
     print('bench: fp32 %.3f ms/step, %.3e cell-updates/s' % (el32 / K * 1e3, v32), file=sys.stderr, flush=True)
@@ -389,23 +413,34 @@ def main():
            "config": {"workload": "3D ideal-MHD turbulence box %d^3 periodic, gamma=5/3, PLM+Roe+CTU+CT, "
                                   "CFL 0.4, FP32 bulk with FP64 islands" % n,
                       "cells": cells, "reconstruct_order": 1, "parallelism": "single GPU"},
-           "roofline": roofline(prof32, 4),
+           "roofline": roofline(prof32, 4, n),
            "hbm_roof_fraction_step": v32 * 64 / HBM_PEAK,
            "kernel_ms_per_step": {k: round(v[0] / 3, 4) for k, v in sorted(prof32.items())}}
+    if sweep is not None:
+        out["config"]["sweep_run_lengths"] = sweep
     if not a.no_fp64:
+        # both precisions continue (untimed) to the configured cycle count before
+        # their states are compared
+        ncmp = max(a.err_cycles, cyc32)
+        t32, _, _ = s32.advance(0.0, 1e30, cyc32, ncmp)
         p32, c32, f32 = s32.get_state(0)
         s32.close()
-        s64, el64, prof64 = run_gpu(prim, cons, faces, n, 64, K, W, profile_steps=3)
+        s64, el64, prof64, cyc64 = run_gpu(prim, cons, faces, n, 64, K, W, profile_steps=3)
         v64 = cells * K / el64
         print('bench: fp64 %.3f ms/step, %.3e cell-updates/s' % (el64 / K * 1e3, v64), file=sys.stderr,
               flush=True)
+        s64.advance(0.0, 1e30, cyc64, ncmp)
         p64, c64, f64 = s64.get_state(0)
         s64.close()
-        out["fp64"] = {"value": v64, "ms_per_step": el64 / K * 1e3, "roofline": roofline(prof64, 8)}
+        out["fp64"] = {"value": v64, "ms_per_step": el64 / K * 1e3, "roofline": roofline(prof64, 8, n),
+                       "kernel_ms_per_step": {k: round(v[0] / 3, 4) for k, v in sorted(prof64.items())}}
+        out["fp32_over_fp64_throughput"] = v32 / v64
         out["l1_rho_fp32_vs_fp64"] = float(np.mean(np.abs(c32[0] - c64[0])))
         out["divb_fp32"] = div_b(f32, 1.0 / n)
         out["divb_fp64"] = div_b(f64, 1.0 / n)
-        out["fp32_vs_fp64_steps"] = W + K + 3
+        out["fp32_vs_fp64_cycles"] = ncmp
+        print('bench: after %d cycles L1(rho) fp32 vs fp64 %.3e, div B fp32 %.3e, fp64 %.3e' % (
+            ncmp, out["l1_rho_fp32_vs_fp64"], out["divb_fp32"], out["divb_fp64"]), file=sys.stderr, flush=True)
     else:
         s32.close()
     if not a.no_kh:

@@ -86,7 +86,8 @@ int pmhd_advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int64_
  * launch), traces the edge states and forms the predictor's interface states and
  * Roe fluxes for s consecutive faces per thread; 0 selects separate slope, edge
  * and predictor launches.  Results are identical for every choice.  The default
- * is picked from the grid size and the device.  Lengths must lie in [0, 4096]. */
+ * is picked from the grid size and the device (2 to 16 along x and y, 2 along
+ * z).  Lengths must lie in [0, 4096]. */
 int pmhd_set_sweep(pmhd_handle* h, int32_t seg_x, int32_t seg_y, int32_t seg_z);
 
 /* Wait for all queued work of the handle. */

@@ -252,6 +252,8 @@ struct TapeExec {
 
 template <class T, class B>
 struct AdjointHandle : public pmhd_adjoint {
+  // the adjoint keeps faces in the cell type (its unit does not define PMHD_FACES_F64)
+  static_assert(sizeof(face_t<T>) == sizeof(T), "adjoint contexts store faces in the cell type");
   B be;
   pmhd_grid g;
   pmhd_params p;

@@ -186,24 +186,35 @@ void init_ctx(Ctx<T>& c, const pmhd_grid& g, const pmhd_params& p) {
   }
 }
 
-// Faces per thread of the fused predictor sweep along each direction (Ctx::seg):
-// the largest power of two, at most 32, that still leaves about 2048 threads per
-// compute unit for nfaces[d] faces, and at least 2 (a run of one face reconstructs
-// two cells).  Long runs amortise the cell reconstructed before each run.
-inline int sweep_segment(int64_t nfaces, int cus) {
+// Faces per thread of the fused predictor sweep along direction d (Ctx::seg): the
+// largest power of two that still leaves about 2048 threads per compute unit for
+// nfaces faces, at least 2 (a run of one face reconstructs two cells) and at most
+// 16 along x and y, 2 along z.  Long runs amortise the cell reconstructed before
+// each run; along z, the contiguous axis, neighbouring threads own neighbouring
+// runs, so short runs keep their loads close together.
+inline int sweep_segment(int d, int64_t nfaces, int cus) {
   const int64_t target = int64_t(cus > 0 ? cus : 256) * 2048;
+  const int cap = d == 2 ? 2 : 16;
   int s = 2;
-  while (s < 32 && nfaces / (2 * s) >= target) s *= 2;
+  while (s < cap && nfaces / (2 * s) >= target) s *= 2;
   return s;
 }
 
 // the two state buffers of a grid (the update writes the other one)
 template <class T>
 struct StateBufs {
+  using F = face_t<T>;
   T* U[2] = {nullptr, nullptr};
   T* W[2] = {nullptr, nullptr};
-  T* FB[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  F* FB[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
 };
+
+// room for n face values of a context of T, from an allocator of T values
+template <class T, class A>
+face_t<T>* alloc_faces(A&& alloc, int64_t n) {
+  static_assert(sizeof(face_t<T>) % sizeof(T) == 0, "face type must be a multiple of the cell type");
+  return reinterpret_cast<face_t<T>*>(alloc(n * int64_t(sizeof(face_t<T>) / sizeof(T))));
+}
 
 // cell variables and interface-state variables of a physics
 inline int cell_vars(int physics) { return physics == 1 ? 5 : 8; }
@@ -224,7 +235,7 @@ void alloc_grid(Ctx<T>& c, StateBufs<T>& s, int order, A&& alloc) {
   for (int b = 0; b < 2; ++b) {
     s.U[b] = alloc(cv * N);
     s.W[b] = alloc(cv * N);
-    for (int d = 0; d < 3; ++d) s.FB[b][d] = alloc(mhd && d < nd && nd >= 2 ? c.nface(d) : 1);
+    for (int d = 0; d < 3; ++d) s.FB[b][d] = alloc_faces<T>(alloc, mhd && d < nd && nd >= 2 ? c.nface(d) : 1);
   }
   c.WL = alloc(fv * maxf);
   c.WR = alloc(fv * maxf);
@@ -235,7 +246,7 @@ void alloc_grid(Ctx<T>& c, StateBufs<T>& s, int order, A&& alloc) {
     c.UR[d] = alloc(used ? fv * c.nface(d) : 1);
     c.F[d] = alloc(used ? fv * c.nface(d) : 1);
     c.Fc[d] = alloc(used && nd >= 2 ? fv * c.nface(d) : 1);
-    c.nfb[d] = alloc(mhd && used && nd >= 2 ? c.nface(d) : 1);
+    c.nfb[d] = alloc_faces<T>(alloc, mhd && used && nd >= 2 ? c.nface(d) : 1);
     c.E[d] = alloc(mhd && (nd == 3 || (nd == 2 && d == 2)) ? c.nedge() : 1);
   }
   c.Ec = alloc(mhd ? (nd == 3 ? 3 * N : N) : 1);
@@ -280,7 +291,7 @@ void download_vals(B& be, double* dst, const T* src, int64_t n) {
 // load / store the reference-layout state of one grid into buffer set `s`
 template <class T, class B>
 int load_state(B& be, Ctx<T>& c, const pmhd_grid& g, const double* prim, const double* cons,
-               const double* const fin[3], T* U, T* W, T* const FB[3], std::string& err) {
+               const double* const fin[3], T* U, T* W, face_t<T>* const FB[3], std::string& err) {
   if (!prim || !cons) {
     err = "prim and cons are required";
     return PMHD_EINVAL;
@@ -301,7 +312,8 @@ int load_state(B& be, Ctx<T>& c, const pmhd_grid& g, const double* prim, const d
 }
 
 template <class T, class B>
-void store_state(B& be, const Ctx<T>& c, const pmhd_grid& g, const T* U, const T* W, const T* const FB[3],
+void store_state(B& be, const Ctx<T>& c, const pmhd_grid& g, const T* U, const T* W,
+                 const face_t<T>* const FB[3],
                  double* prim, double* cons, double* const fo[3]) {
   const int64_t N = c.ncell();
   if (cons) download_vals(be, cons, U, cell_vars(c.physics) * N);
@@ -346,6 +358,7 @@ struct Handle : public pmhd_handle {
       return fail(PMHD_ENOMEM, "device allocation failed");
     }
     bind_bufs(c, sb, cur);
+    for (int d = 0; d < g.ndim; ++d) c.seg[d] = sweep_segment(d, c.nface(d), be.compute_units());
     const char* e = be.error();
     if (e) return fail(PMHD_EHIP, e);
     return PMHD_OK;
@@ -565,7 +578,8 @@ struct BatchHandle : public pmhd_batch {
     if (check_box(box)) return PMHD_EINVAL;
     const Ctx<T>& c = hctx[size_t(box)];
     const bool second = which == 1 && hclk[size_t(box)].done > 0;
-    const T* fb[3] = {second ? c.fb2[0] : c.fb[0], second ? c.fb2[1] : c.fb[1], second ? c.fb2[2] : c.fb[2]};
+    const face_t<T>* fb[3] = {second ? c.fb2[0] : c.fb[0], second ? c.fb2[1] : c.fb[1],
+                              second ? c.fb2[2] : c.fb[2]};
     double* fo[3] = {fbx, fby, fbz};
     be.sync();
     store_state(be, c, grids[size_t(box)], second ? c.U2 : c.U, second ? c.W2 : c.W, fb, prim, cons, fo);

@@ -60,6 +60,22 @@ using CPtr = typename Mem<T>::cp;
 template <class T>
 using Ptr = typename Mem<T>::p;
 
+// storage type of a context's face-centred field: the FP32-bulk step keeps its
+// faces in double (its translation unit defines PMHD_FACES_F64), so that the
+// constrained-transport update preserves div B to double rounding
+template <class T>
+struct FaceOf {
+  using type = T;
+};
+#ifdef PMHD_FACES_F64
+template <>
+struct FaceOf<float> {
+  using type = double;
+};
+#endif
+template <class T>
+using face_t = typename FaceOf<T>::type;
+
 template <class T>
 struct Ctx {
   int ndim;
@@ -83,14 +99,14 @@ struct Ctx {
   T emfq[3];         // 0.5*(0.5*(dt/dx)), ...
   T bx1d;            // uniform B_x of the 1D step
   double ldh[2];     // 0.5*dt/dy as the reference's 80-bit value, hi + lo (FP64 only)
-  // state
+  // state (faces in the face type face_t<T>)
   CPtr<T> U;
   CPtr<T> W;
   Ptr<T> U2;
   Ptr<T> W2;
-  CPtr<T> fb[3];
-  Ptr<T> fb2[3];
-  Ptr<T> nfb[3];
+  CPtr<face_t<T>> fb[3];
+  Ptr<face_t<T>> fb2[3];
+  Ptr<face_t<T>> nfb[3];
   // work
   Ptr<T> WL;
   Ptr<T> WR;
@@ -245,10 +261,10 @@ PMHD_FI void k_cfl(const Ctx<T>& c, const PU& U, const PF* fb, I3 p, T out[3]) {
   for (int v = 0; v < 8; ++v) u[v] = U[v * N + idx];
   T bf[3] = {T(0), T(0), T(0)};
   if (c.ndim >= 2) {
-    bf[0] = fb[0][c.fid(0, p.j, p.i, p.k)];
-    bf[1] = fb[1][c.fid(1, p.j, p.i, p.k)];
+    bf[0] = T(fb[0][c.fid(0, p.j, p.i, p.k)]);
+    bf[1] = T(fb[1][c.fid(1, p.j, p.i, p.k)]);
   }
-  if (c.ndim == 3) bf[2] = fb[2][c.fid(2, p.j, p.i, p.k)];
+  if (c.ndim == 3) bf[2] = T(fb[2][c.fid(2, p.j, p.i, p.k)]);
   cell_speeds(u, bf, c.ndim, c.gamma, out);
 }
 
@@ -374,7 +390,7 @@ template <class T, class P>
 PMHD_FI T face_val(const Ctx<T>& c, const P& fb, int a, I3 p, int64_t fa) {
   // value of face array of axis a at face coordinate fa along a, other coords of p
   I3 q = with(p, a, fa);
-  return fb[c.fid(a, q.j, q.i, q.k)];
+  return T(fb[c.fid(a, q.j, q.i, q.k)]);
 }
 
 // source term of the 3D reconstruction on the (ghost-mapped) cell e along d
@@ -422,7 +438,7 @@ PMHD_FI void predict_face(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7]) {
     wl[5] = wl[5] + cl[1];
     wr[4] = wr[4] + cr[0];
     wr[5] = wr[5] + cr[1];
-    bn = c.fb[d][idx];
+    bn = T(c.fb[d][idx]);
     ke = d;
   } else if (c.ndim == 2) {
     // getByReconSourceTerms_2D / getBxReconSourceTerms_2D
@@ -435,7 +451,7 @@ PMHD_FI void predict_face(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7]) {
     T rv = c.W[vslot * N + c.cid(pr.j, pr.i, pr.k)];
     int64_t flo = e - 1 < 0 ? (bc == OUTFLOW ? 0 : n - 1) : e - 1;
     int64_t fhi = e + 1 > n ? (bc == OUTFLOW ? n : 1) : e + 1;
-    T fm = c.fb[d][idx];
+    T fm = T(c.fb[d][idx]);
     T lf = face_val(c, c.fb[d], d, f, flo);
     T rf = face_val(c, c.fb[d], d, f, fhi);
     T c0 = c.hdt * lv * ((fm - lf) * c.invd[d]);
@@ -560,9 +576,9 @@ PMHD_FI void k_half_e(const Ctx<T>& c, I3 p) {
              - c.hdtd[1] * (c.F[1][v * NY + fy1] - c.F[1][v * NY + fy0])
              - c.hdtd[2] * (c.F[2][v * NZ + fz1] - c.F[2][v * NZ + fz0]);
     }
-    T bx = T(0.5) * (c.nfb[0][fx0] + c.nfb[0][fx1]);
-    T by = T(0.5) * (c.nfb[1][fy0] + c.nfb[1][fy1]);
-    T bz = T(0.5) * (c.nfb[2][fz0] + c.nfb[2][fz1]);
+    T bx = T(0.5) * (T(c.nfb[0][fx0]) + T(c.nfb[0][fx1]));
+    T by = T(0.5) * (T(c.nfb[1][fy0]) + T(c.nfb[1][fy1]));
+    T bz = T(0.5) * (T(c.nfb[2][fz0]) + T(c.nfb[2][fz1]));
     c.Ec[idx] = (by * h[3] - bz * h[2]) / h[0];
     c.Ec[N + idx] = (bz * h[1] - bx * h[3]) / h[0];
     c.Ec[2 * N + idx] = (bx * h[2] - by * h[1]) / h[0];
@@ -573,8 +589,8 @@ PMHD_FI void k_half_e(const Ctx<T>& c, I3 p) {
       T a = c.U[v * N + idx] - c.hdtd[0] * (c.F[0][v * NX + fx1] - c.F[0][v * NX + fx0]);
       h[v] = a - c.hdtd[1] * (c.F[1][v * NY + fy1] - c.F[1][v * NY + fy0]);
     }
-    T bx = (c.nfb[0][fx0] + c.nfb[0][fx1]) / T(2);
-    T by = (c.nfb[1][fy0] + c.nfb[1][fy1]) / T(2);
+    T bx = (T(c.nfb[0][fx0]) + T(c.nfb[0][fx1])) / T(2);
+    T by = (T(c.nfb[1][fy0]) + T(c.nfb[1][fy1])) / T(2);
     T vx = h[1] / h[0];
     T vy = h[2] / h[0];
     c.Ec[idx] = vy * bx - vx * by;
@@ -839,29 +855,32 @@ PMHD_FI void ctu_states(const Ctx<T>& c, int d, I3 f, T ul[7], T ur[7]) {
 // ===========================================================================
 template <class T, class PI, class PO>
 PMHD_FI void k_ct(const Ctx<T>& c, int d, const PI& fin, const PO& fout, const T h[3], I3 f) {
+  // in the face type F (double for the FP32-bulk step): the curl differences of the
+  // edge EMFs and the update then round at double precision
+  using F = face_t<T>;
   const int64_t idx = c.fid(d, f.j, f.i, f.k);
   if (c.ndim == 2) {
     const auto E = c.E[2];
     if (d == 0) {
-      fout[idx] = fin[idx] - h[1] * (E[c.eid(f.j + 1, f.i, 0)] - E[c.eid(f.j, f.i, 0)]);
+      fout[idx] = fin[idx] - F(h[1]) * (F(E[c.eid(f.j + 1, f.i, 0)]) - F(E[c.eid(f.j, f.i, 0)]));
     } else {
-      fout[idx] = fin[idx] + h[0] * (E[c.eid(f.j, f.i + 1, 0)] - E[c.eid(f.j, f.i, 0)]);
+      fout[idx] = fin[idx] + F(h[0]) * (F(E[c.eid(f.j, f.i + 1, 0)]) - F(E[c.eid(f.j, f.i, 0)]));
     }
     return;
   }
   const auto Ex = c.E[0];
   const auto Ey = c.E[1];
   const auto Ez = c.E[2];
-  T a, b;
+  F a, b;
   if (d == 0) {
-    a = h[2] * (Ey[c.eid(f.j, f.i, f.k + 1)] - Ey[c.eid(f.j, f.i, f.k)]);
-    b = h[1] * (Ez[c.eid(f.j + 1, f.i, f.k)] - Ez[c.eid(f.j, f.i, f.k)]);
+    a = F(h[2]) * (F(Ey[c.eid(f.j, f.i, f.k + 1)]) - F(Ey[c.eid(f.j, f.i, f.k)]));
+    b = F(h[1]) * (F(Ez[c.eid(f.j + 1, f.i, f.k)]) - F(Ez[c.eid(f.j, f.i, f.k)]));
   } else if (d == 1) {
-    a = h[0] * (Ez[c.eid(f.j, f.i + 1, f.k)] - Ez[c.eid(f.j, f.i, f.k)]);
-    b = h[2] * (Ex[c.eid(f.j, f.i, f.k + 1)] - Ex[c.eid(f.j, f.i, f.k)]);
+    a = F(h[0]) * (F(Ez[c.eid(f.j, f.i + 1, f.k)]) - F(Ez[c.eid(f.j, f.i, f.k)]));
+    b = F(h[2]) * (F(Ex[c.eid(f.j, f.i, f.k + 1)]) - F(Ex[c.eid(f.j, f.i, f.k)]));
   } else {
-    a = h[1] * (Ex[c.eid(f.j + 1, f.i, f.k)] - Ex[c.eid(f.j, f.i, f.k)]);
-    b = h[0] * (Ey[c.eid(f.j, f.i + 1, f.k)] - Ey[c.eid(f.j, f.i, f.k)]);
+    a = F(h[1]) * (F(Ex[c.eid(f.j + 1, f.i, f.k)]) - F(Ex[c.eid(f.j, f.i, f.k)]));
+    b = F(h[0]) * (F(Ey[c.eid(f.j, f.i + 1, f.k)]) - F(Ey[c.eid(f.j, f.i, f.k)]));
   }
   fout[idx] = fin[idx] + (a - b);
 }
@@ -876,7 +895,7 @@ PMHD_FI void k_correct(const Ctx<T>& c, int d, I3 f) {
   const int64_t idx = c.fid(d, f.j, f.i, f.k);
   T ul[7], ur[7], wl[7], wr[7];
   ctu_states(c, d, f, ul, ur);
-  T bn = c.nfb[d][idx];
+  T bn = T(c.nfb[d][idx]);
   int ke = c.ndim == 3 ? d : 0;
   cons_to_prim(ul, bn, c.gamma, ke, wl);
   cons_to_prim(ur, bn, c.gamma, ke, wr);
@@ -920,8 +939,8 @@ PMHD_FI void k_update(const Ctx<T>& c, I3 p) {
       u[v] = u[v] - c.dtd[0] * (c.Fc[0][v * NX + fx1] - c.Fc[0][v * NX + fx0]);
       u[v] = u[v] - c.dtd[1] * (c.Fc[1][v * NY + fy1] - c.Fc[1][v * NY + fy0]);
     }
-    bx = (c.fb2[0][fx0] + c.fb2[0][fx1]) / T(2);
-    T by = (c.fb2[1][fy0] + c.fb2[1][fy1]) / T(2);
+    bx = (T(c.fb2[0][fx0]) + T(c.fb2[0][fx1])) / T(2);
+    T by = (T(c.fb2[1][fy0]) + T(c.fb2[1][fy1])) / T(2);
     u[5] = nbz;
     u[4] = by;
     cons_to_prim(u, bx, c.gamma, 0, w);
@@ -935,9 +954,9 @@ PMHD_FI void k_update(const Ctx<T>& c, I3 p) {
       u[v] = u[v] - c.dtd[1] * (c.Fc[1][v * NY + fy1] - c.Fc[1][v * NY + fy0]);
       u[v] = u[v] - c.dtd[2] * (c.Fc[2][v * NZ + fz1] - c.Fc[2][v * NZ + fz0]);
     }
-    bx = T(0.5) * (c.fb2[0][fx0] + c.fb2[0][fx1]);
-    T by = T(0.5) * (c.fb2[1][fy0] + c.fb2[1][fy1]);
-    T bz = T(0.5) * (c.fb2[2][fz0] + c.fb2[2][fz1]);
+    bx = T(0.5) * (T(c.fb2[0][fx0]) + T(c.fb2[0][fx1]));
+    T by = T(0.5) * (T(c.fb2[1][fy0]) + T(c.fb2[1][fy1]));
+    T bz = T(0.5) * (T(c.fb2[2][fz0]) + T(c.fb2[2][fz1]));
     u[4] = by;
     u[5] = bz;
     cons_to_prim(u, bx, c.gamma, 0, w);

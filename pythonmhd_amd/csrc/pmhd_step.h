@@ -252,9 +252,10 @@ PMHD_FI void swap_buffers(Ctx<T>& c) {
   c.W2 = const_cast<T*>(c.W);
   c.U = u;
   c.W = w;
+  using F = face_t<T>;
   for (int d = 0; d < 3; ++d) {
-    T* f = c.fb2[d];
-    c.fb2[d] = const_cast<T*>(c.fb[d]);
+    F* f = c.fb2[d];
+    c.fb2[d] = const_cast<F*>(c.fb[d]);
     c.fb[d] = f;
   }
 }

@@ -1,5 +1,6 @@
-// Host test build: the FP32-bulk step.
+// Host test build: the FP32-bulk step (faces in double, as in the HIP unit).
 #define PMHD_TU_NS f32
+#define PMHD_FACES_F64
 #include "pmhd_host.h"
 
 PMHD_DEFINE_FACTORIES(f32, float, pmhd::HostBackend)
