@@ -59,6 +59,12 @@ template <class T> PMHD_FI T npmin(T a, T b) { return (a < b || a != a) ? a : b;
 template <class T> PMHD_FI T npsign(T x) {
   return x > T(0) ? T(1) : (x < T(0) ? T(-1) : (x == T(0) ? T(0) : x));
 }
+#ifdef PMHD_FAST_ORDER
+// the FP32-bulk unit: single-instruction maxima and minima (they differ from NumPy's
+// only for NaN operands and the sign of a zero result)
+PMHD_FI float npmax(float a, float b) { return fmaxf(a, b); }
+PMHD_FI float npmin(float a, float b) { return fminf(a, b); }
+#endif
 template <class T> PMHD_FI T sq(T x) { return x * x; }
 PMHD_FI double sqrt_(double x) { return sqrt(x); }
 PMHD_FI float sqrt_(float x) { return sqrtf(x); }
@@ -325,9 +331,18 @@ PMHD_FI void limited_slope(const PrimEig<T>& E, const T wl[7], const T wc[7], co
     wg[v] = (dl[v] * dr[v] > T(0)) ? (T(2) * dl[v] * dr[v]) / (dl[v] + dr[v]) : T(0);
   }
   T cc[7], lc[7], rc[7], wc_[7];
+#ifdef PMHD_FAST_ORDER
+  // dc = dl + dr, so L.dc = L.dl + L.dr (one projection fewer; rounds differently)
+  (void)dc;
+  eig_project(E, dl, lc);
+  eig_project(E, dr, rc);
+#pragma unroll
+  for (int v = 0; v < 7; ++v) cc[v] = lc[v] + rc[v];
+#else
   eig_project(E, dc, cc);
   eig_project(E, dl, lc);
   eig_project(E, dr, rc);
+#endif
   eig_project(E, wg, wc_);
   T s[7];
 #pragma unroll
@@ -394,6 +409,30 @@ PMHD_FI void trace_cell(const PrimEig<T>& E, const T wl[7], const T wc[7], const
     ra[v] = ra[v] - c3 * (c1[v] - (T(1) - T(kFourThirds) * c3) * c2[v]);
   }
   const T hdt = T(0.5) * dtdx;
+#ifdef PMHD_FAST_ORDER
+  // L_i . (v3 (c1 + c2) + v4 c2) = v3 (L_i.c1 + L_i.c2) + v4 L_i.c2: project c1 and c2
+  // once and weight the projections per wave, without branches (rounds differently
+  // from the per-wave projections below)
+  T p1[7], p2[7];
+  eig_project(E, c1, p1);
+  if (order == 2) {
+    eig_project(E, c2, p2);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 7; ++i) p2[i] = T(0);
+  }
+  const T vl = hdt * E.lam[0], vr = hdt * E.lam[6];
+#pragma unroll
+  for (int i = 0; i < 7; ++i) {
+    const T v2 = hdt * E.lam[i];
+    const T v3l = vl - v2, v4l = T(kFourThirds) * (vl * vl - v2 * v2);
+    const T v3r = vr - v2, v4r = T(kFourThirds) * (vr * vr - v2 * v2);
+    const T fl = E.lam[i] <= T(0) ? v3l * (p1[i] + p2[i]) + v4l * p2[i] : T(0);
+    const T fr = E.lam[i] >= T(0) ? v3r * (p1[i] - p2[i]) + v4r * p2[i] : T(0);
+    eig_add_col(E, i, fl, la);
+    eig_add_col(E, i, fr, ra);
+  }
+#else
 #pragma unroll
   for (int i = 0; i < 7; ++i) {
     if (E.lam[i] <= T(0)) {
@@ -422,6 +461,7 @@ PMHD_FI void trace_cell(const PrimEig<T>& E, const T wl[7], const T wc[7], const
       eig_add_col(E, i, fac, ra);
     }
   }
+#endif
 }
 
 // ---------------------------------------------------------------------------

@@ -1,6 +1,7 @@
 // Host test build: the FP32-bulk step (faces in double, as in the HIP unit).
 #define PMHD_TU_NS f32
 #define PMHD_FACES_F64
+#define PMHD_FAST_ORDER
 #include "pmhd_host.h"
 
 PMHD_DEFINE_FACTORIES(f32, float, pmhd::HostBackend)
