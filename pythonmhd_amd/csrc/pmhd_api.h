@@ -688,6 +688,14 @@ inline int check_grid(const pmhd_grid* g, std::string& err) {
   if (g->nx > (1 << 24) || g->ny > (1 << 24) || g->nz > (1 << 24)) {
     err = "number of cells along an axis must not exceed 2^24"; return PMHD_EINVAL;
   }
+  {
+    // every variable plane of a state, work or edge array stays below 4 GiB: the
+    // kernels form byte offsets within a plane in 32 bits
+    const int64_t ext = (g->nx + 2) * (g->ndim >= 2 ? g->ny + 2 : 1) * (g->ndim == 3 ? g->nz + 2 : 1);
+    if (ext > (int64_t(1) << 29)) {
+      err = "grid too large: (nx+2)(ny+2)(nz+2) must not exceed 2^29 cells"; return PMHD_EINVAL;
+    }
+  }
   if (!(g->dx > 0) || (g->ndim >= 2 && !(g->dy > 0)) || (g->ndim == 3 && !(g->dz > 0))) {
     err = "cell widths must be positive"; return PMHD_EINVAL;
   }

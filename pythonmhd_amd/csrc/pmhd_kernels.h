@@ -8,6 +8,8 @@
 // 2D uses nz = 1, 1D uses ny = nz = 1.  Interface states and fluxes are stored in
 // the reference's directional 7-vector order (Source/EvolveFunctions_MHD.py:775-807).
 #pragma once
+#include <type_traits>
+
 #include "pmhd_math.h"
 
 PMHD_NS_BEGIN
@@ -215,10 +217,44 @@ PMHD_FI int rot_to_dir(int d, int r) {
   }
 }
 
+// Element i of the array that starts at base (one variable plane of a state or work
+// array).  On the GPU the byte offset is formed in 32 bits -- every plane is below
+// 4 GiB (checked when a grid is created) -- so a load or store takes the plane's
+// uniform base address and a 32-bit offset, with no 64-bit address arithmetic per
+// element.  Accessor types (the adjoint's dual arrays) index directly.
+template <class T>
+PMHD_FI T ld(const T* base, int64_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + uint32_t(uint64_t(i) * sizeof(T)));
+#else
+  return base[i];
+#endif
+}
+template <class T>
+PMHD_FI T ld(T* base, int64_t i) {
+  return ld(static_cast<const T*>(base), i);
+}
+template <class P, class = typename std::enable_if<std::is_class<P>::value>::type>
+PMHD_FI auto ld(const P& base, int64_t i) -> decltype(base[i]) {
+  return base[i];
+}
+template <class T, class V>
+PMHD_FI void st(T* base, int64_t i, const V& v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + uint32_t(uint64_t(i) * sizeof(T))) = T(v);
+#else
+  base[i] = T(v);
+#endif
+}
+template <class P, class V, class = typename std::enable_if<std::is_class<P>::value>::type>
+PMHD_FI void st(const P& base, int64_t i, const V& v) {
+  base[i] = v;
+}
+
 template <class P, class T>
 PMHD_FI void load_rot8(const P& A, int64_t N, int64_t c, int d, T w[7]) {
 #pragma unroll
-  for (int r = 0; r < 7; ++r) w[r] = A[rot8(d, r) * N + c];
+  for (int r = 0; r < 7; ++r) w[r] = ld(A + rot8(d, r) * N, c);
 }
 template <class T>
 PMHD_FI void to_rot(const T dir[7], int d, T rot[7]) {
@@ -233,12 +269,12 @@ PMHD_FI void from_rot(const T rot[7], int d, T dir[7]) {
 template <class P, class T>
 PMHD_FI void load7(const P& A, int64_t NF, int64_t f, T v[7]) {
 #pragma unroll
-  for (int r = 0; r < 7; ++r) v[r] = A[r * NF + f];
+  for (int r = 0; r < 7; ++r) v[r] = ld(A + r * NF, f);
 }
 template <class P, class T>
 PMHD_FI void store7(const P& A, int64_t NF, int64_t f, const T v[7]) {
 #pragma unroll
-  for (int r = 0; r < 7; ++r) A[r * NF + f] = v[r];
+  for (int r = 0; r < 7; ++r) st(A + r * NF, f, v[r]);
 }
 
 // ===========================================================================
@@ -252,19 +288,19 @@ PMHD_FI void k_cfl(const Ctx<T>& c, const PU& U, const PF* fb, I3 p, T out[3]) {
   if (c.physics == 1) {
     T h[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) h[v] = U[v * N + idx];
+    for (int v = 0; v < 5; ++v) h[v] = ld(U + v * N, idx);
     hydro::cell_speeds(h, c.gamma, out);
     return;
   }
   T u[8];
 #pragma unroll
-  for (int v = 0; v < 8; ++v) u[v] = U[v * N + idx];
+  for (int v = 0; v < 8; ++v) u[v] = ld(U + v * N, idx);
   T bf[3] = {T(0), T(0), T(0)};
   if (c.ndim >= 2) {
-    bf[0] = T(fb[0][c.fid(0, p.j, p.i, p.k)]);
-    bf[1] = T(fb[1][c.fid(1, p.j, p.i, p.k)]);
+    bf[0] = T(ld(fb[0], c.fid(0, p.j, p.i, p.k)));
+    bf[1] = T(ld(fb[1], c.fid(1, p.j, p.i, p.k)));
   }
-  if (c.ndim == 3) bf[2] = T(fb[2][c.fid(2, p.j, p.i, p.k)]);
+  if (c.ndim == 3) bf[2] = T(ld(fb[2], c.fid(2, p.j, p.i, p.k)));
   cell_speeds(u, bf, c.ndim, c.gamma, out);
 }
 
@@ -284,7 +320,7 @@ PMHD_FI void k_slopes(const Ctx<T>& c, int d, int bc, I3 q) {
   load_rot8(c.W, N, c.cid(pl.j, pl.i, pl.k), d, wl);
   load_rot8(c.W, N, c.cid(pc.j, pc.i, pc.k), d, wc);
   load_rot8(c.W, N, c.cid(pr.j, pr.i, pr.k), d, wr);
-  T bn = c.W[(4 + d) * N + c.cid(pc.j, pc.i, pc.k)];
+  T bn = ld(c.W + (4 + d) * N, c.cid(pc.j, pc.i, pc.k));
   PrimEig<T> E;
   eigen_prim(wc, bn, c.gamma, E);
   T ps[7];
@@ -309,7 +345,7 @@ PMHD_FI void cell_edge_states(const Ctx<T>& c, int d, int bc, I3 p, bool from_s,
   load_rot8(c.W, N, c.cid(pl.j, pl.i, pl.k), d, wl);
   load_rot8(c.W, N, idx, d, wc);
   load_rot8(c.W, N, c.cid(pr.j, pr.i, pr.k), d, wr);
-  T bn = c.W[(4 + d) * N + idx];
+  T bn = ld(c.W + (4 + d) * N, idx);
   PrimEig<T> E;
   eigen_prim(wc, bn, c.gamma, E);
   T la[7], ra[7];
@@ -390,7 +426,7 @@ template <class T, class P>
 PMHD_FI T face_val(const Ctx<T>& c, const P& fb, int a, I3 p, int64_t fa) {
   // value of face array of axis a at face coordinate fa along a, other coords of p
   I3 q = with(p, a, fa);
-  return T(fb[c.fid(a, q.j, q.i, q.k)]);
+  return T(ld(fb, c.fid(a, q.j, q.i, q.k)));
 }
 
 // source term of the 3D reconstruction on the (ghost-mapped) cell e along d
@@ -410,9 +446,9 @@ PMHD_FI void recon_b_term_3d(const Ctx<T>& c, int d, int bc, I3 p, int64_t e, T 
   T term1 = minmod_recon(dbn, db1, d == 1);
   T term2 = minmod_recon(dbn, db2, false);
   int64_t cm = c.cid(pm.j, pm.i, pm.k);
-  T rho = c.W[cm];
-  T v1 = c.U[(1 + t1) * N + cm] / rho;
-  T v2 = c.U[(1 + t2) * N + cm] / rho;
+  T rho = ld(c.W, cm);
+  T v1 = ld(c.U + (1 + t1) * N, cm) / rho;
+  T v2 = ld(c.U + (1 + t2) * N, cm) / rho;
   out[0] = c.hdt * (v1 * term1);
   out[1] = c.hdt * (v2 * term2);
 }
@@ -438,7 +474,7 @@ PMHD_FI void predict_face(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7]) {
     wl[5] = wl[5] + cl[1];
     wr[4] = wr[4] + cr[0];
     wr[5] = wr[5] + cr[1];
-    bn = T(c.fb[d][idx]);
+    bn = T(ld(c.fb[d], idx));
     ke = d;
   } else if (c.ndim == 2) {
     // getByReconSourceTerms_2D / getBxReconSourceTerms_2D
@@ -447,11 +483,11 @@ PMHD_FI void predict_face(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7]) {
     I3 pl = with(f, d, map1(e - 1, n, bc));
     I3 pr = with(f, d, map1(e, n, bc));
     int vslot = d == 0 ? 2 : 1;  // x faces use v_y, y faces use v_x
-    T lv = c.W[vslot * N + c.cid(pl.j, pl.i, pl.k)];
-    T rv = c.W[vslot * N + c.cid(pr.j, pr.i, pr.k)];
+    T lv = ld(c.W + vslot * N, c.cid(pl.j, pl.i, pl.k));
+    T rv = ld(c.W + vslot * N, c.cid(pr.j, pr.i, pr.k));
     int64_t flo = e - 1 < 0 ? (bc == OUTFLOW ? 0 : n - 1) : e - 1;
     int64_t fhi = e + 1 > n ? (bc == OUTFLOW ? n : 1) : e + 1;
-    T fm = T(c.fb[d][idx]);
+    T fm = T(ld(c.fb[d], idx));
     T lf = face_val(c, c.fb[d], d, f, flo);
     T rf = face_val(c, c.fb[d], d, f, fhi);
     T c0 = c.hdt * lv * ((fm - lf) * c.invd[d]);
@@ -548,13 +584,13 @@ PMHD_FI void k_cell_e(const Ctx<T>& c, I3 p) {
   const int64_t N = c.ncell();
   const int64_t idx = c.cid(p.j, p.i, p.k);
   if (c.ndim == 3) {
-    T rho = c.U[idx], mx = c.U[N + idx], my = c.U[2 * N + idx], mz = c.U[3 * N + idx];
-    T bx = c.W[4 * N + idx], by = c.W[5 * N + idx], bz = c.W[6 * N + idx];
-    c.Ec[idx] = (by * mz - bz * my) / rho;
-    c.Ec[N + idx] = (bz * mx - bx * mz) / rho;
-    c.Ec[2 * N + idx] = (bx * my - by * mx) / rho;
+    T rho = ld(c.U, idx), mx = ld(c.U + N, idx), my = ld(c.U + 2 * N, idx), mz = ld(c.U + 3 * N, idx);
+    T bx = ld(c.W + 4 * N, idx), by = ld(c.W + 5 * N, idx), bz = ld(c.W + 6 * N, idx);
+    st(c.Ec, idx, (by * mz - bz * my) / rho);
+    st(c.Ec + N, idx, (bz * mx - bx * mz) / rho);
+    st(c.Ec + 2 * N, idx, (bx * my - by * mx) / rho);
   } else {
-    c.Ec[idx] = c.W[2 * N + idx] * c.W[4 * N + idx] - c.W[N + idx] * c.W[5 * N + idx];
+    st(c.Ec, idx, ld(c.W + 2 * N, idx) * ld(c.W + 4 * N, idx) - ld(c.W + N, idx) * ld(c.W + 5 * N, idx));
   }
 }
 
@@ -572,28 +608,28 @@ PMHD_FI void k_half_e(const Ctx<T>& c, I3 p) {
     T h[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      h[v] = c.U[v * N + idx] - c.hdtd[0] * (c.F[0][v * NX + fx1] - c.F[0][v * NX + fx0])
-             - c.hdtd[1] * (c.F[1][v * NY + fy1] - c.F[1][v * NY + fy0])
-             - c.hdtd[2] * (c.F[2][v * NZ + fz1] - c.F[2][v * NZ + fz0]);
+      h[v] = ld(c.U + v * N, idx) - c.hdtd[0] * (ld(c.F[0] + v * NX, fx1) - ld(c.F[0] + v * NX, fx0))
+             - c.hdtd[1] * (ld(c.F[1] + v * NY, fy1) - ld(c.F[1] + v * NY, fy0))
+             - c.hdtd[2] * (ld(c.F[2] + v * NZ, fz1) - ld(c.F[2] + v * NZ, fz0));
     }
-    T bx = T(0.5) * (T(c.nfb[0][fx0]) + T(c.nfb[0][fx1]));
-    T by = T(0.5) * (T(c.nfb[1][fy0]) + T(c.nfb[1][fy1]));
-    T bz = T(0.5) * (T(c.nfb[2][fz0]) + T(c.nfb[2][fz1]));
-    c.Ec[idx] = (by * h[3] - bz * h[2]) / h[0];
-    c.Ec[N + idx] = (bz * h[1] - bx * h[3]) / h[0];
-    c.Ec[2 * N + idx] = (bx * h[2] - by * h[1]) / h[0];
+    T bx = T(0.5) * (T(ld(c.nfb[0], fx0)) + T(ld(c.nfb[0], fx1)));
+    T by = T(0.5) * (T(ld(c.nfb[1], fy0)) + T(ld(c.nfb[1], fy1)));
+    T bz = T(0.5) * (T(ld(c.nfb[2], fz0)) + T(ld(c.nfb[2], fz1)));
+    st(c.Ec, idx, (by * h[3] - bz * h[2]) / h[0]);
+    st(c.Ec + N, idx, (bz * h[1] - bx * h[3]) / h[0]);
+    st(c.Ec + 2 * N, idx, (bx * h[2] - by * h[1]) / h[0]);
   } else {
     T h[3];
 #pragma unroll
     for (int v = 0; v < 3; ++v) {
-      T a = c.U[v * N + idx] - c.hdtd[0] * (c.F[0][v * NX + fx1] - c.F[0][v * NX + fx0]);
-      h[v] = a - c.hdtd[1] * (c.F[1][v * NY + fy1] - c.F[1][v * NY + fy0]);
+      T a = ld(c.U + v * N, idx) - c.hdtd[0] * (ld(c.F[0] + v * NX, fx1) - ld(c.F[0] + v * NX, fx0));
+      h[v] = a - c.hdtd[1] * (ld(c.F[1] + v * NY, fy1) - ld(c.F[1] + v * NY, fy0));
     }
-    T bx = (T(c.nfb[0][fx0]) + T(c.nfb[0][fx1])) / T(2);
-    T by = (T(c.nfb[1][fy0]) + T(c.nfb[1][fy1])) / T(2);
+    T bx = (T(ld(c.nfb[0], fx0)) + T(ld(c.nfb[0], fx1))) / T(2);
+    T by = (T(ld(c.nfb[1], fy0)) + T(ld(c.nfb[1], fy1))) / T(2);
     T vx = h[1] / h[0];
     T vy = h[2] / h[0];
-    c.Ec[idx] = vy * bx - vx * by;
+    st(c.Ec, idx, vy * bx - vx * by);
   }
 }
 
@@ -602,7 +638,7 @@ PMHD_FI void k_half_e(const Ctx<T>& c, I3 p) {
 // ===========================================================================
 template <class T>
 PMHD_FI T fluxv(const Ctx<T>& c, const Ptr<T>* Fs, int d, int v, int64_t j, int64_t i, int64_t k) {
-  return Fs[d][v * c.nface(d) + c.fid(d, j, i, k)];
+  return ld(Fs[d] + v * c.nface(d), c.fid(d, j, i, k));
 }
 
 // Fs: the flux set (predictor c.F or corrector c.Fc) the EMFs are formed from
@@ -618,13 +654,13 @@ PMHD_FI void k_emf(const Ctx<T>& c, const Ptr<T>* Fs, I3 p) {
     T lt0 = fluxv(c, Fs, 1, 0, p.j, xl, 0), lt5 = fluxv(c, Fs, 1, 5, p.j, xl, 0);
     T lf0 = fluxv(c, Fs, 0, 0, yr, p.i, 0), lf4 = fluxv(c, Fs, 0, 4, yr, p.i, 0);
     T ul0 = fluxv(c, Fs, 0, 0, yl, p.i, 0), ul4 = fluxv(c, Fs, 0, 4, yl, p.i, 0);
-    T cE = c.Ec[c.cid(yr, xr, 0)], lE = c.Ec[c.cid(yr, xl, 0)];
-    T uE = c.Ec[c.cid(yl, xr, 0)], ulE = c.Ec[c.cid(yl, xl, 0)];
+    T cE = ld(c.Ec, c.cid(yr, xr, 0)), lE = ld(c.Ec, c.cid(yr, xl, 0));
+    T uE = ld(c.Ec, c.cid(yl, xr, 0)), ulE = ld(c.Ec, c.cid(yl, xl, 0));
     T x1 = upwind(lt0, lt5 - ul4 - ulE, lt5 - lf4 - lE, lt5 - T(0.5) * (lf4 + ul4 + lE + ulE));
     T x2 = upwind(t0, t5 - ul4 - uE, t5 - lf4 - cE, t5 - T(0.5) * (lf4 + ul4 + cE + uE));
     T y1 = upwind(ul0, -ul4 + lt5 - ulE, -ul4 + t5 - uE, -ul4 + T(0.5) * (lt5 + t5 - ulE - uE));
     T y2 = upwind(lf0, -lf4 + lt5 - lE, -lf4 + t5 - cE, -lf4 + T(0.5) * (lt5 + t5 - lE - cE));
-    c.E[2][idx] = T(0.25) * (x1 + x2 + y1 + y2);
+    st(c.E[2], idx, T(0.25) * (x1 + x2 + y1 + y2));
     return;
   }
   const int64_t zl = edge_lo(p.k, c.nz, bz), zr = edge_hi(p.k, c.nz, bz);
@@ -636,39 +672,39 @@ PMHD_FI void k_emf(const Ctx<T>& c, const Ptr<T>* Fs, I3 p) {
     T ub0 = fluxv(c, Fs, 2, 0, yl, xr, p.k), ub5 = fluxv(c, Fs, 2, 5, yl, xr, p.k);
     T t0 = fluxv(c, Fs, 1, 0, p.j, xr, zr), t4 = fluxv(c, Fs, 1, 4, p.j, xr, zr);
     T bt0 = fluxv(c, Fs, 1, 0, p.j, xr, zl), bt4 = fluxv(c, Fs, 1, 4, p.j, xr, zl);
-    T cE = Exc[c.cid(yr, xr, zr)], bE = Exc[c.cid(yr, xr, zl)];
-    T uE = Exc[c.cid(yl, xr, zr)], ubE = Exc[c.cid(yl, xr, zl)];
+    T cE = ld(Exc, c.cid(yr, xr, zr)), bE = ld(Exc, c.cid(yr, xr, zl));
+    T uE = ld(Exc, c.cid(yl, xr, zr)), ubE = ld(Exc, c.cid(yl, xr, zl));
     T z1 = upwind(bt0, ub5 - ubE, b5 - bE, T(0.5) * (ub5 + b5 - ubE - bE));
     T z2 = upwind(t0, ub5 - uE, b5 - cE, T(0.5) * (ub5 + b5 - uE - cE));
     T y1 = upwind(ub0, -bt4 - ubE, -t4 - uE, T(0.5) * (-bt4 - t4 - ubE - uE));
     T y2 = upwind(b0, -bt4 - bE, -t4 - cE, T(0.5) * (-bt4 - t4 - bE - cE));
-    c.E[0][idx] = T(0.25) * (b5 + ub5 - t4 - bt4 + y1 + y2 + z1 + z2);
+    st(c.E[0], idx, T(0.25) * (b5 + ub5 - t4 - bt4 + y1 + y2 + z1 + z2));
   }
   {  // E_y (getYCornerEMF_3D): faces x (F) and z (H)
     T b0 = fluxv(c, Fs, 2, 0, yr, xr, p.k), b4 = fluxv(c, Fs, 2, 4, yr, xr, p.k);
     T lb0 = fluxv(c, Fs, 2, 0, yr, xl, p.k), lb4 = fluxv(c, Fs, 2, 4, yr, xl, p.k);
     T lf0 = fluxv(c, Fs, 0, 0, yr, p.i, zr), lf5 = fluxv(c, Fs, 0, 5, yr, p.i, zr);
     T blf0 = fluxv(c, Fs, 0, 0, yr, p.i, zl), blf5 = fluxv(c, Fs, 0, 5, yr, p.i, zl);
-    T cE = Eyc[c.cid(yr, xr, zr)], bE = Eyc[c.cid(yr, xr, zl)];
-    T lE = Eyc[c.cid(yr, xl, zr)], blE = Eyc[c.cid(yr, xl, zl)];
+    T cE = ld(Eyc, c.cid(yr, xr, zr)), bE = ld(Eyc, c.cid(yr, xr, zl));
+    T lE = ld(Eyc, c.cid(yr, xl, zr)), blE = ld(Eyc, c.cid(yr, xl, zl));
     T z1 = upwind(blf0, -lb4 - blE, -b4 - bE, T(0.5) * (-lb4 - b4 - blE - bE));
     T z2 = upwind(lf0, -lb4 - lE, -b4 - cE, T(0.5) * (-lb4 - b4 - lE - cE));
     T x1 = upwind(lb0, blf5 - blE, lf5 - lE, T(0.5) * (blf5 + lf5 - blE - lE));
     T x2 = upwind(b0, blf5 - bE, lf5 - cE, T(0.5) * (blf5 + lf5 - bE - cE));
-    c.E[1][idx] = T(0.25) * (lf5 + blf5 - b4 - lb4 + x1 + x2 + z1 + z2);
+    st(c.E[1], idx, T(0.25) * (lf5 + blf5 - b4 - lb4 + x1 + x2 + z1 + z2));
   }
   {  // E_z (getZCornerEMF_3D): faces x (F) and y (G)
     T t0 = fluxv(c, Fs, 1, 0, p.j, xr, zr), t5 = fluxv(c, Fs, 1, 5, p.j, xr, zr);
     T lt0 = fluxv(c, Fs, 1, 0, p.j, xl, zr), lt5 = fluxv(c, Fs, 1, 5, p.j, xl, zr);
     T lf0 = fluxv(c, Fs, 0, 0, yr, p.i, zr), lf4 = fluxv(c, Fs, 0, 4, yr, p.i, zr);
     T ul0 = fluxv(c, Fs, 0, 0, yl, p.i, zr), ul4 = fluxv(c, Fs, 0, 4, yl, p.i, zr);
-    T cE = Ezc[c.cid(yr, xr, zr)], lE = Ezc[c.cid(yr, xl, zr)];
-    T uE = Ezc[c.cid(yl, xr, zr)], ulE = Ezc[c.cid(yl, xl, zr)];
+    T cE = ld(Ezc, c.cid(yr, xr, zr)), lE = ld(Ezc, c.cid(yr, xl, zr));
+    T uE = ld(Ezc, c.cid(yl, xr, zr)), ulE = ld(Ezc, c.cid(yl, xl, zr));
     T x1 = upwind(lt0, -ul4 - ulE, -lf4 - lE, T(-0.5) * (lf4 + ul4 + lE + ulE));
     T x2 = upwind(t0, -ul4 - uE, -lf4 - cE, T(-0.5) * (lf4 + ul4 + cE + uE));
     T y1 = upwind(ul0, lt5 - ulE, t5 - uE, T(0.5) * (lt5 + t5 - ulE - uE));
     T y2 = upwind(lf0, lt5 - lE, t5 - cE, T(0.5) * (lt5 + t5 - lE - cE));
-    c.E[2][idx] = T(0.25) * (lt5 + t5 - ul4 - lf4 + x1 + x2 + y1 + y2);
+    st(c.E[2], idx, T(0.25) * (lt5 + t5 - ul4 - lf4 + x1 + x2 + y1 + y2));
   }
 }
 
@@ -696,8 +732,8 @@ PMHD_FI void ctu_source_3d(const Ctx<T>& c, int d, int bc, I3 f, int64_t e, T sr
   T mm1 = minmod_ctu(db1, dbn);
   T mm2 = minmod_ctu(db2, dbn);
   int64_t cm = c.cid(pm.j, pm.i, pm.k);
-  T cb[3] = {c.W[4 * N + cm], c.W[5 * N + cm], c.W[6 * N + cm]};
-  T cv[3] = {c.W[1 * N + cm], c.W[2 * N + cm], c.W[3 * N + cm]};
+  T cb[3] = {ld(c.W + 4 * N, cm), ld(c.W + 5 * N, cm), ld(c.W + 6 * N, cm)};
+  T cv[3] = {ld(c.W + 1 * N, cm), ld(c.W + 2 * N, cm), ld(c.W + 3 * N, cm)};
   T v1 = cv[t1], v2 = cv[t2], b1 = cb[t1], b2 = cb[t2];
   src[0] = T(0);
   src[1] = c.hdt * cb[0] * dbn;
@@ -723,12 +759,12 @@ PMHD_FI void ctu_apply_3d(const Ctx<T>& c, int d, int tc, I3 pcell, int64_t hi_o
   T h = c.hdtd[tc];
   const int vs[5] = {0, 1, 2, 3, 6};
   // x faces, y fluxes: the reference's 80-bit factor (FP64 instantiation only)
-  const bool ld = is_f64<T>() && d == 0 && tc == 1;
+  const bool ext80 = is_f64<T>() && d == 0 && tc == 1;
 #pragma unroll
   for (int q = 0; q < 5; ++q) {
     int v = vs[q];
-    T x = c.F[tc][v * NF + fhi] - c.F[tc][v * NF + flo];
-    if (ld)
+    T x = ld(c.F[tc] + v * NF, fhi) - ld(c.F[tc] + v * NF, flo);
+    if (ext80)
       u[v] = T(ld_fms(double(u[v]), c.ldh[0], c.ldh[1], double(x), false));
     else
       u[v] = u[v] - h * x;
@@ -740,12 +776,12 @@ PMHD_FI void ctu_apply_3d(const Ctx<T>& c, int d, int tc, I3 pcell, int64_t hi_o
   I3 b01 = with(b, t2, co(b, t2) + 1);
   I3 b11 = with(b10, t2, co(b, t2) + 1);
   const auto E = c.E[d];
-  T e00 = E[c.eid(b.j, b.i, b.k)], e10 = E[c.eid(b10.j, b10.i, b10.k)];
-  T e01 = E[c.eid(b01.j, b01.i, b01.k)], e11 = E[c.eid(b11.j, b11.i, b11.k)];
+  T e00 = ld(E, c.eid(b.j, b.i, b.k)), e10 = ld(E, c.eid(b10.j, b10.i, b10.k));
+  T e01 = ld(E, c.eid(b01.j, b01.i, b01.k)), e11 = ld(E, c.eid(b11.j, b11.i, b11.k));
   T q = c.emfq[tc];
   if (tc == t1) {
     T y = e10 - e00 + e11 - e01;
-    if (ld)
+    if (ext80)
       u[5] = T(ld_fms(double(u[5]), 0.5 * c.ldh[0], 0.5 * c.ldh[1], double(y), true));
     else
       u[5] = u[5] + q * y;
@@ -816,7 +852,7 @@ PMHD_FI void ctu_states(const Ctx<T>& c, int d, I3 f, T ul[7], T ur[7]) {
       I3 pc = with(f, d, cm);
       int64_t ci = c.cid(pc.j, pc.i, pc.k);
       T db = face_val(c, c.fb[d], d, f, fhi) - face_val(c, c.fb[d], d, f, flo);
-      T cbx = c.W[4 * N + ci], cby = c.W[5 * N + ci], cbz = c.W[6 * N + ci], cvz = c.W[3 * N + ci];
+      T cbx = ld(c.W + 4 * N, ci), cby = ld(c.W + 5 * N, ci), cbz = ld(c.W + 6 * N, ci), cvz = ld(c.W + 3 * N, ci);
       T inv = c.invd[d];
       s[0] = T(0);
       s[1] = cbx * db * inv;
@@ -835,7 +871,7 @@ PMHD_FI void ctu_states(const Ctx<T>& c, int d, I3 f, T ul[7], T ur[7]) {
       int64_t flo = c.fid(tc, pc.j, pc.i, pc.k), fhi = c.fid(tc, hi.j, hi.i, hi.k);
       T g[7];
 #pragma unroll
-      for (int v = 0; v < 7; ++v) g[v] = c.F[tc][v * NT + fhi] - c.F[tc][v * NT + flo];
+      for (int v = 0; v < 7; ++v) g[v] = ld(c.F[tc] + v * NT, fhi) - ld(c.F[tc] + v * NT, flo);
       // slot remap: the transverse flux's in-plane B component moves to the
       // out-of-plane B_z slot of this direction, its own B slot is zeroed
       T dr[7] = {g[0], g[1], g[2], g[3], T(0), T(0), g[6]};
@@ -862,9 +898,9 @@ PMHD_FI void k_ct(const Ctx<T>& c, int d, const PI& fin, const PO& fout, const T
   if (c.ndim == 2) {
     const auto E = c.E[2];
     if (d == 0) {
-      fout[idx] = fin[idx] - F(h[1]) * (F(E[c.eid(f.j + 1, f.i, 0)]) - F(E[c.eid(f.j, f.i, 0)]));
+      st(fout, idx, ld(fin, idx) - F(h[1]) * (F(ld(E, c.eid(f.j + 1, f.i, 0))) - F(ld(E, c.eid(f.j, f.i, 0)))));
     } else {
-      fout[idx] = fin[idx] + F(h[0]) * (F(E[c.eid(f.j, f.i + 1, 0)]) - F(E[c.eid(f.j, f.i, 0)]));
+      st(fout, idx, ld(fin, idx) + F(h[0]) * (F(ld(E, c.eid(f.j, f.i + 1, 0))) - F(ld(E, c.eid(f.j, f.i, 0)))));
     }
     return;
   }
@@ -873,16 +909,16 @@ PMHD_FI void k_ct(const Ctx<T>& c, int d, const PI& fin, const PO& fout, const T
   const auto Ez = c.E[2];
   F a, b;
   if (d == 0) {
-    a = F(h[2]) * (F(Ey[c.eid(f.j, f.i, f.k + 1)]) - F(Ey[c.eid(f.j, f.i, f.k)]));
-    b = F(h[1]) * (F(Ez[c.eid(f.j + 1, f.i, f.k)]) - F(Ez[c.eid(f.j, f.i, f.k)]));
+    a = F(h[2]) * (F(ld(Ey, c.eid(f.j, f.i, f.k + 1))) - F(ld(Ey, c.eid(f.j, f.i, f.k))));
+    b = F(h[1]) * (F(ld(Ez, c.eid(f.j + 1, f.i, f.k))) - F(ld(Ez, c.eid(f.j, f.i, f.k))));
   } else if (d == 1) {
-    a = F(h[0]) * (F(Ez[c.eid(f.j, f.i + 1, f.k)]) - F(Ez[c.eid(f.j, f.i, f.k)]));
-    b = F(h[2]) * (F(Ex[c.eid(f.j, f.i, f.k + 1)]) - F(Ex[c.eid(f.j, f.i, f.k)]));
+    a = F(h[0]) * (F(ld(Ez, c.eid(f.j, f.i + 1, f.k))) - F(ld(Ez, c.eid(f.j, f.i, f.k))));
+    b = F(h[2]) * (F(ld(Ex, c.eid(f.j, f.i, f.k + 1))) - F(ld(Ex, c.eid(f.j, f.i, f.k))));
   } else {
-    a = F(h[1]) * (F(Ex[c.eid(f.j + 1, f.i, f.k)]) - F(Ex[c.eid(f.j, f.i, f.k)]));
-    b = F(h[0]) * (F(Ey[c.eid(f.j, f.i + 1, f.k)]) - F(Ey[c.eid(f.j, f.i, f.k)]));
+    a = F(h[1]) * (F(ld(Ex, c.eid(f.j + 1, f.i, f.k))) - F(ld(Ex, c.eid(f.j, f.i, f.k))));
+    b = F(h[0]) * (F(ld(Ey, c.eid(f.j, f.i + 1, f.k))) - F(ld(Ey, c.eid(f.j, f.i, f.k))));
   }
-  fout[idx] = fin[idx] + (a - b);
+  st(fout, idx, ld(fin, idx) + (a - b));
 }
 
 // ===========================================================================
@@ -895,7 +931,7 @@ PMHD_FI void k_correct(const Ctx<T>& c, int d, I3 f) {
   const int64_t idx = c.fid(d, f.j, f.i, f.k);
   T ul[7], ur[7], wl[7], wr[7];
   ctu_states(c, d, f, ul, ur);
-  T bn = T(c.nfb[d][idx]);
+  T bn = T(ld(c.nfb[d], idx));
   int ke = c.ndim == 3 ? d : 0;
   cons_to_prim(ul, bn, c.gamma, ke, wl);
   cons_to_prim(ur, bn, c.gamma, ke, wr);
@@ -920,27 +956,27 @@ PMHD_FI void k_update(const Ctx<T>& c, I3 p) {
   const int dv[7] = {0, 1, 2, 3, 5, 6, 7};
   T u[7];
 #pragma unroll
-  for (int v = 0; v < 7; ++v) u[v] = c.U[dv[v] * N + idx];
+  for (int v = 0; v < 7; ++v) u[v] = ld(c.U + dv[v] * N, idx);
   const int64_t NX = c.nface(0);
   int64_t fx0 = c.fid(0, p.j, p.i, p.k), fx1 = c.fid(0, p.j, p.i + 1, p.k);
   T bx, w[7];
   if (c.ndim == 1) {
 #pragma unroll
-    for (int v = 0; v < 7; ++v) u[v] = u[v] - c.dtd[0] * (c.F[0][v * NX + fx1] - c.F[0][v * NX + fx0]);
+    for (int v = 0; v < 7; ++v) u[v] = u[v] - c.dtd[0] * (ld(c.F[0] + v * NX, fx1) - ld(c.F[0] + v * NX, fx0));
     bx = c.bx1d;
     cons_to_prim(u, bx, c.gamma, 0, w);
   } else if (c.ndim == 2) {
     const int64_t NY = c.nface(1);
     int64_t fy0 = c.fid(1, p.j, p.i, p.k), fy1 = c.fid(1, p.j + 1, p.i, p.k);
-    T nbz = u[5] - c.dtd[0] * (c.Fc[0][5 * NX + fx1] - c.Fc[0][5 * NX + fx0])
-            - c.dtd[1] * (c.Fc[1][4 * NY + fy1] - c.Fc[1][4 * NY + fy0]);
+    T nbz = u[5] - c.dtd[0] * (ld(c.Fc[0] + 5 * NX, fx1) - ld(c.Fc[0] + 5 * NX, fx0))
+            - c.dtd[1] * (ld(c.Fc[1] + 4 * NY, fy1) - ld(c.Fc[1] + 4 * NY, fy0));
 #pragma unroll
     for (int v = 0; v < 7; ++v) {
-      u[v] = u[v] - c.dtd[0] * (c.Fc[0][v * NX + fx1] - c.Fc[0][v * NX + fx0]);
-      u[v] = u[v] - c.dtd[1] * (c.Fc[1][v * NY + fy1] - c.Fc[1][v * NY + fy0]);
+      u[v] = u[v] - c.dtd[0] * (ld(c.Fc[0] + v * NX, fx1) - ld(c.Fc[0] + v * NX, fx0));
+      u[v] = u[v] - c.dtd[1] * (ld(c.Fc[1] + v * NY, fy1) - ld(c.Fc[1] + v * NY, fy0));
     }
-    bx = (T(c.fb2[0][fx0]) + T(c.fb2[0][fx1])) / T(2);
-    T by = (T(c.fb2[1][fy0]) + T(c.fb2[1][fy1])) / T(2);
+    bx = (T(ld(c.fb2[0], fx0)) + T(ld(c.fb2[0], fx1))) / T(2);
+    T by = (T(ld(c.fb2[1], fy0)) + T(ld(c.fb2[1], fy1))) / T(2);
     u[5] = nbz;
     u[4] = by;
     cons_to_prim(u, bx, c.gamma, 0, w);
@@ -950,33 +986,33 @@ PMHD_FI void k_update(const Ctx<T>& c, I3 p) {
     int64_t fz0 = c.fid(2, p.j, p.i, p.k), fz1 = c.fid(2, p.j, p.i, p.k + 1);
 #pragma unroll
     for (int v = 0; v < 7; ++v) {
-      u[v] = u[v] - c.dtd[0] * (c.Fc[0][v * NX + fx1] - c.Fc[0][v * NX + fx0]);
-      u[v] = u[v] - c.dtd[1] * (c.Fc[1][v * NY + fy1] - c.Fc[1][v * NY + fy0]);
-      u[v] = u[v] - c.dtd[2] * (c.Fc[2][v * NZ + fz1] - c.Fc[2][v * NZ + fz0]);
+      u[v] = u[v] - c.dtd[0] * (ld(c.Fc[0] + v * NX, fx1) - ld(c.Fc[0] + v * NX, fx0));
+      u[v] = u[v] - c.dtd[1] * (ld(c.Fc[1] + v * NY, fy1) - ld(c.Fc[1] + v * NY, fy0));
+      u[v] = u[v] - c.dtd[2] * (ld(c.Fc[2] + v * NZ, fz1) - ld(c.Fc[2] + v * NZ, fz0));
     }
-    bx = T(0.5) * (T(c.fb2[0][fx0]) + T(c.fb2[0][fx1]));
-    T by = T(0.5) * (T(c.fb2[1][fy0]) + T(c.fb2[1][fy1]));
-    T bz = T(0.5) * (T(c.fb2[2][fz0]) + T(c.fb2[2][fz1]));
+    bx = T(0.5) * (T(ld(c.fb2[0], fx0)) + T(ld(c.fb2[0], fx1)));
+    T by = T(0.5) * (T(ld(c.fb2[1], fy0)) + T(ld(c.fb2[1], fy1)));
+    T bz = T(0.5) * (T(ld(c.fb2[2], fz0)) + T(ld(c.fb2[2], fz1)));
     u[4] = by;
     u[5] = bz;
     cons_to_prim(u, bx, c.gamma, 0, w);
   }
-  c.U2[idx] = u[0];
-  c.U2[N + idx] = u[1];
-  c.U2[2 * N + idx] = u[2];
-  c.U2[3 * N + idx] = u[3];
-  c.U2[4 * N + idx] = bx;
-  c.U2[5 * N + idx] = u[4];
-  c.U2[6 * N + idx] = u[5];
-  c.U2[7 * N + idx] = u[6];
-  c.W2[idx] = w[0];
-  c.W2[N + idx] = w[1];
-  c.W2[2 * N + idx] = w[2];
-  c.W2[3 * N + idx] = w[3];
-  c.W2[4 * N + idx] = bx;
-  c.W2[5 * N + idx] = w[4];
-  c.W2[6 * N + idx] = w[5];
-  c.W2[7 * N + idx] = w[6];
+  st(c.U2, idx, u[0]);
+  st(c.U2 + N, idx, u[1]);
+  st(c.U2 + 2 * N, idx, u[2]);
+  st(c.U2 + 3 * N, idx, u[3]);
+  st(c.U2 + 4 * N, idx, bx);
+  st(c.U2 + 5 * N, idx, u[4]);
+  st(c.U2 + 6 * N, idx, u[5]);
+  st(c.U2 + 7 * N, idx, u[6]);
+  st(c.W2, idx, w[0]);
+  st(c.W2 + N, idx, w[1]);
+  st(c.W2 + 2 * N, idx, w[2]);
+  st(c.W2 + 3 * N, idx, w[3]);
+  st(c.W2 + 4 * N, idx, bx);
+  st(c.W2 + 5 * N, idx, w[4]);
+  st(c.W2 + 6 * N, idx, w[5]);
+  st(c.W2 + 7 * N, idx, w[6]);
 }
 
 // ===========================================================================

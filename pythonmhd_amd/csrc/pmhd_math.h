@@ -189,7 +189,13 @@ PMHD_FI void eigen_prim(const T w[7], T bn, T gamma, PrimEig<T>& E) {
   N n_cs2, n_ca2, n_cf2, n_csl2, af, as;
   {
     // wave speeds and normalisation in N (double) precision
+#ifdef PMHD_FAST_ORDER
+    // a relative error of 1/rho scales cs2, ca2 and bt2/rho alike, so every
+    // difference below keeps its relative accuracy with the float reciprocal
+    N n_inv = N(inv);
+#else
     N n_inv = N(1) / N(rho);
+#endif
     N n_rbtsq = N(btsq) * n_inv;
     n_cs2 = N(gp) * n_inv;
     n_ca2 = N(bn) * N(bn) * n_inv;
@@ -205,8 +211,21 @@ PMHD_FI void eigen_prim(const T w[7], T bn, T gamma, PrimEig<T>& E) {
     } else if ((n_cf2 - n_cs2) <= N(0)) {
       af = N(1); as = N(0);
     } else {
+#ifdef PMHD_FAST_ORDER
+      if constexpr (sizeof(T) == 4) {
+        // the differences carry the cancellation near the degeneracies and stay
+        // double; quotient and root of these double-accurate values round in float
+        const T iss = T(1) / T(ssqrt);
+        af = N(sqrt_(T(n_cs2 - n_csl2) * iss));
+        as = N(sqrt_(T(n_cf2 - n_cs2) * iss));
+      } else {
+        af = sqrt_((n_cs2 - n_csl2) / ssqrt);
+        as = sqrt_((n_cf2 - n_cs2) / ssqrt);
+      }
+#else
       af = sqrt_((n_cs2 - n_csl2) / ssqrt);
       as = sqrt_((n_cf2 - n_cs2) / ssqrt);
+#endif
     }
   }
   const T cs2 = T(n_cs2);
@@ -541,8 +560,20 @@ PMHD_HD inline void roe_flux(const T wl[7], const T wr[7], const T ul[7], const 
     } else if ((n_cf2 - n_cs2) <= N(0)) {
       af = N(1); as = N(0);
     } else {
+#ifdef PMHD_FAST_ORDER
+      if constexpr (sizeof(T) == 4) {
+        // as in eigen_prim: double differences, float quotient and root
+        const T iq = T(1) / T(n_cf2 - n_csl2);
+        af = N(sqrt_(T(n_cs2 - n_csl2) * iq));
+        as = N(sqrt_(T(n_cf2 - n_cs2) * iq));
+      } else {
+        af = sqrt_((n_cs2 - n_csl2) / (n_cf2 - n_csl2));
+        as = sqrt_((n_cf2 - n_cs2) / (n_cf2 - n_csl2));
+      }
+#else
       af = sqrt_((n_cs2 - n_csl2) / (n_cf2 - n_csl2));
       as = sqrt_((n_cf2 - n_cs2) / (n_cf2 - n_csl2));
+#endif
     }
     hmb = T(n_hmb);
     cs2 = T(n_cs2);
