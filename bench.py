@@ -21,6 +21,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+from pythonmhd_amd.problems import adjoint_state, kh_boxes, turbulence  # noqa: E402
+
 METRIC = "cell-updates/s (FP32 vs FP64 GPU) at 3D 256^3 MHD; L1 err vs FP64; adjoint/fwd time"
 GAMMA = 5.0 / 3.0
 HBM_PEAK = 8.0e12
@@ -45,43 +47,6 @@ ALG_WORDS = {
     'update': 31 + 16,      # U, fluxes, new faces in; cons + prim out (per cell)
     'cfl': 11,
 }
-
-
-def turbulence_ic(n, seed=2024, kmax=8, mach=0.5):
-    """(prim, cons, faces) of the turbulence box in the reference layout (y, x, z)."""
-    rng = np.random.default_rng(seed)
-    k1 = np.fft.fftfreq(n, 1.0 / n)
-    ky, kx, kz = np.meshgrid(k1, k1, k1, indexing='ij')
-    kk = np.sqrt(kx * kx + ky * ky + kz * kz)
-    sel = (kk >= 1.0) & (kk <= kmax)
-    amp = np.zeros_like(kk)
-    amp[sel] = kk[sel] ** -1.0          # per-mode power |k|^-2
-    vh = []
-    for _ in range(3):
-        ph = rng.uniform(0, 2 * np.pi, kk.shape)
-        g = rng.standard_normal(kk.shape)
-        vh.append(amp * g * np.exp(1j * ph))
-    kd = np.where(kk > 0, kk * kk, 1.0)
-    kdotv = (kx * vh[0] + ky * vh[1] + kz * vh[2]) / kd
-    vx = np.real(np.fft.ifftn(vh[0] - kx * kdotv))
-    vy = np.real(np.fft.ifftn(vh[1] - ky * kdotv))
-    vz = np.real(np.fft.ifftn(vh[2] - kz * kdotv))
-    del vh, kdotv, kx, ky, kz, kk, amp
-    rms = np.sqrt(np.mean(vx * vx + vy * vy + vz * vz))
-    s = mach * np.sqrt(GAMMA) / rms
-    prim = np.zeros((8, n, n, n))
-    prim[0] = 1.0
-    prim[1], prim[2], prim[3] = vx * s, vy * s, vz * s
-    prim[4] = 0.5
-    prim[7] = 1.0
-    cons = np.empty_like(prim)
-    cons[0] = prim[0]
-    cons[1:4] = prim[1:4] * prim[0]
-    cons[4:7] = prim[4:7]
-    cons[7] = prim[7] / (GAMMA - 1.0) + 0.5 * prim[0] * (prim[1] ** 2 + prim[2] ** 2 + prim[3] ** 2) \
-        + 0.5 * (prim[4] ** 2 + prim[5] ** 2 + prim[6] ** 2)
-    faces = (np.full((n, n + 1, n), 0.5), np.zeros((n + 1, n, n)), np.zeros((n, n, n + 1)))
-    return prim, cons, faces
 
 
 def div_b(faces, h):
@@ -113,32 +78,6 @@ def run_gpu(prim, cons, faces, n, precision, steps, warmup, profile_steps=3, swe
         prof = s.profile()
         s.set_profiling(False)
     return s, el, prof, cyc + profile_steps
-
-
-def kh_boxes(nbox=256, seed=7):
-    """Sides n ~ U{64..256} and (prim, cons, faces) of each Kelvin-Helmholtz box:
-    rho = 1 and v_x = +0.5 for |y - 0.5| < 0.25, rho = 2 and v_x = -0.5 elsewhere,
-    p = 2.5, B = (0.1, 0, 0), v_y = 0.01 * uniform white noise in [-1, 1]."""
-    rng = np.random.default_rng(seed)
-    sides = rng.integers(64, 257, nbox)
-    out = []
-    for n in sides:
-        n = int(n)
-        y = (np.arange(n) + 0.5) / n
-        inner = (np.abs(y - 0.5) < 0.25)[:, None] * np.ones((1, n), dtype=bool)
-        prim = np.zeros((8, n, n))
-        prim[0] = np.where(inner, 1.0, 2.0)
-        prim[1] = np.where(inner, 0.5, -0.5)
-        prim[2] = 0.01 * rng.uniform(-1.0, 1.0, (n, n))
-        prim[4] = 0.1
-        prim[7] = 2.5
-        cons = prim.copy()
-        cons[1:4] = prim[1:4] * prim[0]
-        cons[7] = prim[7] / (GAMMA - 1.0) + 0.5 * prim[0] * (prim[1] ** 2 + prim[2] ** 2 + prim[3] ** 2) \
-            + 0.5 * (prim[4] ** 2 + prim[5] ** 2 + prim[6] ** 2)
-        faces = (np.full((n, n + 1), 0.1), np.zeros((n + 1, n)))
-        out.append((n, prim, cons, faces))
-    return out
 
 
 def run_kh(cycles=200, seq_cycles=50, nbox=256):
@@ -182,36 +121,6 @@ def run_kh(cycles=200, seq_cycles=50, nbox=256):
             "sequential": {"value": vs, "cycles": seq_cycles, "seconds": el_seq,
                            "how": "one handle per box, box after box, host dt read per cycle"},
             "speedup_vs_sequential": v / vs}
-
-
-def grf(n, corr=0.1, seed=11):
-    """Zero-mean, unit-variance periodic Gaussian random field on n^2 cells with a
-    Gaussian correlation of length corr (spectral filter exp(-k^2 corr^2 / 2))."""
-    rng = np.random.default_rng(seed)
-    k1 = 2 * np.pi * np.fft.fftfreq(n, 1.0 / n)
-    ky, kx = np.meshgrid(k1, k1, indexing='ij')
-    f = np.real(np.fft.ifft2(np.fft.fft2(rng.standard_normal((n, n))) * np.exp(-0.5 * (kx * kx + ky * ky) * corr ** 2)))
-    f = f - f.mean()
-    return f / f.std()
-
-
-def adjoint_state(n, velocity=0.0, seed=11):
-    """rho = 1 + 0.05 * GRF, p = 1, B = (0.3, 0, 0); v = 0 (the configured entropy
-    mode) or, for the gradient check, a smooth velocity field of amplitude velocity."""
-    rho = 1.0 + 0.05 * grf(n, seed=seed)
-    prim = np.zeros((8, n, n))
-    prim[0] = rho
-    if velocity:
-        prim[1] = velocity * grf(n, corr=0.15, seed=seed + 1)
-        prim[2] = velocity * grf(n, corr=0.15, seed=seed + 2)
-    prim[4] = 0.3
-    prim[7] = 1.0
-    cons = prim.copy()
-    cons[1:4] = prim[1:4] * prim[0]
-    cons[7] = prim[7] / (GAMMA - 1.0) + 0.5 * prim[0] * (prim[1] ** 2 + prim[2] ** 2 + prim[3] ** 2) \
-        + 0.5 * (prim[4] ** 2 + prim[5] ** 2 + prim[6] ** 2)
-    faces = (np.full((n, n + 1), 0.3), np.zeros((n + 1, n)))
-    return prim, cons, faces
 
 
 def _ke_cotangent(c, area):
@@ -308,7 +217,7 @@ def run_adjoint(n=256, steps=200, every=20, ncheck=32, check_steps=20):
 def cpu_baseline(n=64, cycles=2):
     """The FP64 NumPy oracle (single thread) on a bounded sample of the workload."""
     from oracle import mhd as M
-    prim, cons, faces = turbulence_ic(n)
+    prim, cons, faces = turbulence(n)
     P = M.Params(GAMMA, 0.4, order=1, entropy_fix=False)
     h = 1.0 / n
     dt = M.calc_dt(cons, faces, GAMMA, 0.4, (h, h, h))
@@ -400,7 +309,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     n, K, W = a.n, a.steps, a.warmup
-    prim, cons, faces = turbulence_ic(n)
+    prim, cons, faces = turbulence(n)
     print('bench: %d^3 turbulence IC ready' % n, file=sys.stderr, flush=True)
     sweep = [int(x) for x in a.sweep.split(',')] if a.sweep else None
     s32, el32, prof32, cyc32 = run_gpu(prim, cons, faces, n, 32, K, W, sweep=sweep)

@@ -186,17 +186,19 @@ void init_ctx(Ctx<T>& c, const pmhd_grid& g, const pmhd_params& p) {
   }
 }
 
-// Faces per thread of the fused predictor sweep along direction d (Ctx::seg): the
-// largest power of two that still leaves about 2048 threads per compute unit for
-// nfaces faces, at least 2 (a run of one face reconstructs two cells) and at most
-// 16 along x and y, 2 along z.  Long runs amortise the cell reconstructed before
-// each run; along z, the contiguous axis, neighbouring threads own neighbouring
-// runs, so short runs keep their loads close together.
-inline int sweep_segment(int d, int64_t nfaces, int cus) {
+// Faces per thread of the fused predictor sweep along direction d (Ctx::seg).  In
+// FP32 along x and y: the largest power of two, at most 16, that still leaves about
+// 2048 threads per compute unit for nfaces faces, and at least 2 (a run of one face
+// reconstructs two cells); long runs amortise the cell reconstructed before each
+// run.  Along z, the contiguous axis: one workgroup per line (-1) for lines of at
+// least 64 cells, else runs of 2.  In FP64 the fused kernels need more registers
+// than they save memory traffic, so the separate launches are used (0).
+inline int sweep_segment(int d, int64_t nfaces, int64_t n_along, int cus, bool fp64) {
+  if (fp64) return 0;
+  if (d == 2) return n_along >= 64 ? -1 : 2;
   const int64_t target = int64_t(cus > 0 ? cus : 256) * 2048;
-  const int cap = d == 2 ? 2 : 16;
   int s = 2;
-  while (s < cap && nfaces / (2 * s) >= target) s *= 2;
+  while (s < 16 && nfaces / (2 * s) >= target) s *= 2;
   return s;
 }
 
@@ -358,7 +360,8 @@ struct Handle : public pmhd_handle {
       return fail(PMHD_ENOMEM, "device allocation failed");
     }
     bind_bufs(c, sb, cur);
-    for (int d = 0; d < g.ndim; ++d) c.seg[d] = sweep_segment(d, c.nface(d), be.compute_units());
+    for (int d = 0; d < g.ndim; ++d)
+      c.seg[d] = sweep_segment(d, c.nface(d), c.n_along(d), be.compute_units(), sizeof(T) == sizeof(double));
     const char* e = be.error();
     if (e) return fail(PMHD_EHIP, e);
     return PMHD_OK;
@@ -366,7 +369,8 @@ struct Handle : public pmhd_handle {
 
   int set_sweep(const int32_t seg[3]) override {
     for (int d = 0; d < 3; ++d)
-      if (seg[d] < 0 || seg[d] > 4096) return fail(PMHD_EINVAL, "sweep run length must be in [0, 4096]");
+      if (seg[d] < (d == 2 ? -1 : 0) || seg[d] > 4096)
+        return fail(PMHD_EINVAL, "sweep run length must be in [0, 4096] (or -1 along z)");
     for (int d = 0; d < 3; ++d) c.seg[d] = d < g.ndim ? seg[d] : 0;
     return PMHD_OK;
   }

@@ -39,6 +39,69 @@ __global__ void __launch_bounds__(256) k_phase_flat(const Ctx<T> c, unsigned n1,
   Ph()(c, I3{int64_t(r / n1), int64_t(r % n1), int64_t(k)});
 }
 
+// Fused predictor sweep along z, one workgroup per line (j, i) = (blockIdx.y,
+// blockIdx.x), in chunks of blockDim.x cells: each thread reconstructs one cell
+// and keeps its left edge state; the right edge states go through LDS to the
+// thread of the next face.  Faces 0 and n, which need the edge states of both
+// ends of the line, are formed after the last chunk by threads 0 and 1, so no
+// thread reconstructs a cell twice and no wave waits on an extra reconstruction.
+// The face states are those of k_sweep_run, so the results are identical.
+template <class T>
+__global__ void __launch_bounds__(256) k_sweep_line_z(const Ctx<T> c) {
+  __shared__ T sra[7][256];
+  __shared__ T prev[7];   // right edge state of the last cell of the previous chunk
+  __shared__ T first[7];  // left edge state of cell 0
+  const int t = int(threadIdx.x), B = int(blockDim.x);
+  const int64_t n = c.nz;
+  const int bc = c.bc[2];
+  const bool from_s = c.order == 2;
+  const I3 q{int64_t(blockIdx.y), int64_t(blockIdx.x), 0};
+  for (int64_t k0 = 0; k0 < n; k0 += B) {
+    const int64_t k = k0 + t;
+    T la[7], ra[7];
+    if (k < n) {
+      cell_edge_states(c, 2, bc, with(q, 2, k), from_s, la, ra);
+#pragma unroll
+      for (int v = 0; v < 7; ++v) sra[v][t] = ra[v];
+      if (k == 0) {
+#pragma unroll
+        for (int v = 0; v < 7; ++v) first[v] = la[v];
+      }
+    }
+    __syncthreads();
+    if (k > 0 && k < n) {
+      T wl[7];
+#pragma unroll
+      for (int v = 0; v < 7; ++v) wl[v] = t == 0 ? prev[v] : sra[v][t - 1];
+      predict_face(c, 2, with(q, 2, k), wl, la);
+    }
+    __syncthreads();
+    if (t == 0) {
+      const int last = int(n - k0 < B ? n - k0 : B) - 1;
+#pragma unroll
+      for (int v = 0; v < 7; ++v) prev[v] = sra[v][last];
+    }
+    __syncthreads();
+  }
+  if (t < 2) {
+    // face 0: left edge of cell 0 on both sides (outflow) or right edge of cell n-1
+    // and left edge of cell 0 (periodic); face n: right edge of cell n-1 on both
+    // sides (outflow) or right edge of cell n-1 and left edge of cell 0 (periodic)
+    T wl[7], wr[7];
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      if (t == 0) {
+        wl[v] = bc == OUTFLOW ? first[v] : prev[v];
+        wr[v] = first[v];
+      } else {
+        wl[v] = prev[v];
+        wr[v] = bc == OUTFLOW ? prev[v] : first[v];
+      }
+    }
+    predict_face(c, 2, with(q, 2, t == 0 ? 0 : n), wl, wr);
+  }
+}
+
 __device__ inline unsigned long long umax64(unsigned long long a, unsigned long long b) {
   return a > b ? a : b;
 }
@@ -359,6 +422,18 @@ struct HipExec {
     c.extent(Ph::kind, Ph::dir, n);
     if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return;
     const int64_t total = n[0] * n[1] * n[2];
+    if constexpr (Ph::kind == EXT_LINE) {
+      // one workgroup per line along z: the smallest of 64, 128, 256 threads that
+      // covers the line, else 256 in chunks
+      static_assert(Ph::dir == 2, "line launches run along z");
+      unsigned b = 64;
+      while (b < 256 && int64_t(b) < c.nz) b <<= 1;
+      timed(Ph::id, total, [&] {
+        hipLaunchKernelGGL((k_sweep_line_z<T>), dim3(unsigned(n[1]), unsigned(n[0])), dim3(b), 0, s, c);
+      });
+      note(hipGetLastError());
+      return;
+    }
     if (Ph::kind == EXT_SEG && total < (int64_t(1) << 31)) {
       const unsigned nb = unsigned((total + flat_block - 1) / flat_block);
       timed(Ph::id, total, [&] {

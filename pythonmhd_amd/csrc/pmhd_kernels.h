@@ -16,14 +16,15 @@ PMHD_NS_BEGIN
 
 enum { OUTFLOW = 0, PERIODIC = 1 };
 // element kinds a launch runs over
-// EXT_SEG: runs of consecutive faces along the launch direction (one run per thread)
-enum ExtKind { EXT_CELL = 0, EXT_FACE = 1, EXT_SLOPE = 2, EXT_EDGE = 3, EXT_SEG = 4 };
+// EXT_SEG: runs of consecutive faces along the launch direction (one run per thread);
+// EXT_LINE: whole lines of faces along the launch direction (one per element)
+enum ExtKind { EXT_CELL = 0, EXT_FACE = 1, EXT_SLOPE = 2, EXT_EDGE = 3, EXT_SEG = 4, EXT_LINE = 5 };
 
 // launch identifiers (for per-kernel timing); "+ d" for the per-direction ones
 enum KernelId {
   K_SLOPES = 0, K_EDGES = 3, K_COPY = 6, K_PREDICT = 9, K_CELL_E = 12, K_EMF = 13, K_EMF2 = 14,
   K_CT_HALF = 17, K_HALF_E = 20, K_CORRECT = 21, K_CT_FULL = 24, K_UPDATE = 27, K_CFL = 28,
-  K_DIAG = 29, K_SWEEP = 30, K_ADJOINT = 33, K_COUNT = 34
+  K_DIAG = 29, K_SWEEP = 30, K_SWEEP_LINE_Z = 33, K_ADJOINT = 34, K_COUNT = 35
 };
 inline const char* kernel_name(int id) {
   static const char* names[K_COUNT] = {
@@ -31,7 +32,7 @@ inline const char* kernel_name(int id) {
       "predict_x", "predict_y", "predict_z", "cell_e", "emf", "emf_corr", "unused_15", "unused_16",
       "ct_half_x", "ct_half_y", "ct_half_z", "half_e", "ctu_correct_x", "ctu_correct_y", "ctu_correct_z",
       "ct_full_x", "ct_full_y", "ct_full_z", "update", "cfl", "diag", "sweep_x", "sweep_y", "sweep_z",
-      "adjoint"};
+      "sweep_z", "adjoint"};
   return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 // element kind (ExtKind) and direction of each launch id
@@ -49,6 +50,7 @@ inline void phase_shape(int id, int& kind, int& dir) {
     dir = id < K_CT_FULL ? id - K_CORRECT : id - K_CT_FULL;
   }
   else if (id >= K_SWEEP && id < K_SWEEP + 3) { kind = EXT_SEG; dir = id - K_SWEEP; }
+  else if (id == K_SWEEP_LINE_Z) { kind = EXT_LINE; dir = 2; }
 }
 
 // pointer types of a context's arrays (the adjoint's dual contexts use accessors)
@@ -89,7 +91,7 @@ struct Ctx {
   int order;         // 0 copy, 1 PLM, 2 PPM
   int efix;
   int seg[3];        // faces per thread of the fused predictor sweep along x, y, z (0: separate
-                     // slope / edge / predictor launches)
+                     // slope / edge / predictor launches; -1 along z: one workgroup per line)
   T gamma, eps, min_dens, min_pres, min_energy;
   // per-cycle scalars (all formed on the host in double)
   T dt, hdt;
@@ -148,6 +150,9 @@ struct Ctx {
       n[0] = sny(d); n[1] = snx(d); n[2] = snz(d);
     } else if (kind == EXT_EDGE) {
       n[0] = eny(); n[1] = enx(); n[2] = enz();
+    } else if (kind == EXT_LINE) {
+      n[0] = fny(d); n[1] = fnx(d); n[2] = fnz(d);
+      n[d == 0 ? 1 : (d == 1 ? 0 : 2)] = 1;
     } else if (kind == EXT_SEG) {
       n[0] = fny(d); n[1] = fnx(d); n[2] = fnz(d);
       const int64_t s = seg[d] > 0 ? seg[d] : 1;
@@ -536,12 +541,10 @@ PMHD_FI void k_predict(const Ctx<T>& c, int d, I3 f) {
 // outflow and the left edge of cell 0 on its right when periodic.  order 2
 // reads the slopes of the slope launch; order 1 forms them here.
 // ===========================================================================
+// the faces [f0, f1) of the line through q along d
 template <class T>
-PMHD_FI void k_sweep(const Ctx<T>& c, int d, int bc, I3 q) {
+PMHD_FI void k_sweep_run(const Ctx<T>& c, int d, int bc, I3 q, int64_t f0, int64_t f1) {
   const int64_t n = c.n_along(d);
-  const int64_t len = c.seg[d] > 0 ? c.seg[d] : 1;
-  const int64_t f0 = co(q, d) * len;
-  const int64_t f1 = f0 + len < n + 1 ? f0 + len : n + 1;
   const bool from_s = c.order == 2;
   T prev[7];
   {
@@ -574,6 +577,14 @@ PMHD_FI void k_sweep(const Ctx<T>& c, int d, int bc, I3 q) {
     }
     predict_face(c, d, with(q, d, f), wl, wr);
   }
+}
+
+template <class T>
+PMHD_FI void k_sweep(const Ctx<T>& c, int d, int bc, I3 q) {
+  const int64_t n = c.n_along(d);
+  const int64_t len = c.seg[d] > 0 ? c.seg[d] : 1;
+  const int64_t f0 = co(q, d) * len;
+  k_sweep_run(c, d, bc, q, f0, f0 + len < n + 1 ? f0 + len : n + 1);
 }
 
 // ===========================================================================

@@ -56,6 +56,14 @@ struct Sweep {  // fused slopes (PLM) + edges + predictor over runs of faces
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 q) const { k_sweep(c, D, c.bc[D], q); }
 };
+// the fused sweep along z over whole lines: the HIP executor gives each line one
+// workgroup whose threads reconstruct one cell each and exchange edge states
+// through LDS; other executors walk the line as one run of faces (same results)
+struct SweepLineZ {
+  PMHD_PHASE_TRAITS(K_SWEEP_LINE_Z, EXT_LINE, 2)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 q) const { k_sweep_run(c, 2, c.bc[2], q, 0, c.nz + 1); }
+};
 struct CellE {
   PMHD_PHASE_TRAITS(K_CELL_E, EXT_CELL, 0)
   template <class T>
@@ -134,9 +142,10 @@ void diagnostics(X& ex, const Ctx<T>& c, double sums[9], double maxes[2]) {
 // reconstruction and the predictor are one launch (PPM keeps its slope launch)
 template <int D, class T, class X>
 void predictor_sweep(X& ex, const Ctx<T>& c) {
-  if (c.order > 0 && c.seg[D] > 0) {
+  if (c.order > 0 && (c.seg[D] > 0 || (D == 2 && c.seg[D] < 0))) {
     if (c.order == 2) ex.template run<Slopes<D>>(c);
-    ex.template run<Sweep<D>>(c);
+    if (D == 2 && c.seg[D] < 0) ex.template run<SweepLineZ>(c);
+    else ex.template run<Sweep<D>>(c);
     return;
   }
   if (c.order > 0) {
