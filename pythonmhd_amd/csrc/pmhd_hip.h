@@ -42,15 +42,14 @@ __global__ void __launch_bounds__(256) k_phase_flat(const Ctx<T> c, unsigned n1,
 // Fused predictor sweep along z, one workgroup per line (j, i) = (blockIdx.y,
 // blockIdx.x), in chunks of blockDim.x cells: each thread reconstructs one cell
 // and keeps its left edge state; the right edge states go through LDS to the
-// thread of the next face.  Faces 0 and n, which need the edge states of both
-// ends of the line, are formed after the last chunk by threads 0 and 1, so no
-// thread reconstructs a cell twice and no wave waits on an extra reconstruction.
-// The face states are those of k_sweep_run, so the results are identical.
+// thread of the next face.  Faces 1..n-1 only: the end faces 0 and n need the edge
+// states of both ends of the line and are formed by the SweepEndsZ launch, so every
+// thread of every wave does the same work.  The face states are those of
+// k_sweep_run, so the results are identical.
 template <class T>
 __global__ void __launch_bounds__(256) k_sweep_line_z(const Ctx<T> c) {
   __shared__ T sra[7][256];
   __shared__ T prev[7];   // right edge state of the last cell of the previous chunk
-  __shared__ T first[7];  // left edge state of cell 0
   const int t = int(threadIdx.x), B = int(blockDim.x);
   const int64_t n = c.nz;
   const int bc = c.bc[2];
@@ -63,10 +62,6 @@ __global__ void __launch_bounds__(256) k_sweep_line_z(const Ctx<T> c) {
       cell_edge_states(c, 2, bc, with(q, 2, k), from_s, la, ra);
 #pragma unroll
       for (int v = 0; v < 7; ++v) sra[v][t] = ra[v];
-      if (k == 0) {
-#pragma unroll
-        for (int v = 0; v < 7; ++v) first[v] = la[v];
-      }
     }
     __syncthreads();
     if (k > 0 && k < n) {
@@ -76,29 +71,13 @@ __global__ void __launch_bounds__(256) k_sweep_line_z(const Ctx<T> c) {
       predict_face(c, 2, with(q, 2, k), wl, la);
     }
     __syncthreads();
-    if (t == 0) {
-      const int last = int(n - k0 < B ? n - k0 : B) - 1;
-#pragma unroll
-      for (int v = 0; v < 7; ++v) prev[v] = sra[v][last];
-    }
-    __syncthreads();
-  }
-  if (t < 2) {
-    // face 0: left edge of cell 0 on both sides (outflow) or right edge of cell n-1
-    // and left edge of cell 0 (periodic); face n: right edge of cell n-1 on both
-    // sides (outflow) or right edge of cell n-1 and left edge of cell 0 (periodic)
-    T wl[7], wr[7];
-#pragma unroll
-    for (int v = 0; v < 7; ++v) {
+    if (k0 + B < n) {
       if (t == 0) {
-        wl[v] = bc == OUTFLOW ? first[v] : prev[v];
-        wr[v] = first[v];
-      } else {
-        wl[v] = prev[v];
-        wr[v] = bc == OUTFLOW ? prev[v] : first[v];
+#pragma unroll
+        for (int v = 0; v < 7; ++v) prev[v] = sra[v][B - 1];
       }
+      __syncthreads();
     }
-    predict_face(c, 2, with(q, 2, t == 0 ? 0 : n), wl, wr);
   }
 }
 
@@ -422,29 +401,36 @@ struct HipExec {
     c.extent(Ph::kind, Ph::dir, n);
     if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return;
     const int64_t total = n[0] * n[1] * n[2];
-    if constexpr (Ph::kind == EXT_LINE) {
+    // profiled work: faces for the launches over runs or lines of faces
+    const int64_t work = Ph::kind == EXT_SEG || Ph::id == K_SWEEP_LINE_Z ? c.nface(Ph::dir)
+                         : (Ph::id == K_SWEEP_ENDS_Z ? 2 * total : total);
+    if constexpr (Ph::id == K_SWEEP_LINE_Z) {
       // one workgroup per line along z: the smallest of 64, 128, 256 threads that
       // covers the line, else 256 in chunks
       static_assert(Ph::dir == 2, "line launches run along z");
       unsigned b = 64;
       while (b < 256 && int64_t(b) < c.nz) b <<= 1;
-      timed(Ph::id, total, [&] {
+      timed(Ph::id, work, [&] {
         hipLaunchKernelGGL((k_sweep_line_z<T>), dim3(unsigned(n[1]), unsigned(n[0])), dim3(b), 0, s, c);
-      });
-      note(hipGetLastError());
-      return;
-    }
-    if (Ph::kind == EXT_SEG && total < (int64_t(1) << 31)) {
-      const unsigned nb = unsigned((total + flat_block - 1) / flat_block);
-      timed(Ph::id, total, [&] {
-        hipLaunchKernelGGL((k_phase_flat<Ph, T>), dim3(nb), dim3(unsigned(flat_block)), 0, s, c, unsigned(n[1]),
-                           unsigned(n[2]), unsigned(total));
       });
       note(hipGetLastError());
       return;
     }
     dim3 grid, block;
     grid_of(n, grid, block);
+    // flat numbering when the box launch would leave many lanes idle (e.g. the
+    // n + 1 faces or edge points along z: 257 = 4 x 64 + 1)
+    const double launched = double(grid.x) * block.x * double(grid.y) * block.y * double(grid.z);
+    const bool sparse = launched > 1.05 * double(total);
+    if ((Ph::kind == EXT_SEG || sparse) && total < (int64_t(1) << 31)) {
+      const unsigned nb = unsigned((total + flat_block - 1) / flat_block);
+      timed(Ph::id, work, [&] {
+        hipLaunchKernelGGL((k_phase_flat<Ph, T>), dim3(nb), dim3(unsigned(flat_block)), 0, s, c, unsigned(n[1]),
+                           unsigned(n[2]), unsigned(total));
+      });
+      note(hipGetLastError());
+      return;
+    }
     timed(Ph::id, total, [&] {
       hipLaunchKernelGGL((k_phase<Ph, T>), grid, block, 0, s, c, int(n[1]), int(n[2]));
     });

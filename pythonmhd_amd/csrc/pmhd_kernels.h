@@ -24,7 +24,7 @@ enum ExtKind { EXT_CELL = 0, EXT_FACE = 1, EXT_SLOPE = 2, EXT_EDGE = 3, EXT_SEG 
 enum KernelId {
   K_SLOPES = 0, K_EDGES = 3, K_COPY = 6, K_PREDICT = 9, K_CELL_E = 12, K_EMF = 13, K_EMF2 = 14,
   K_CT_HALF = 17, K_HALF_E = 20, K_CORRECT = 21, K_CT_FULL = 24, K_UPDATE = 27, K_CFL = 28,
-  K_DIAG = 29, K_SWEEP = 30, K_SWEEP_LINE_Z = 33, K_ADJOINT = 34, K_COUNT = 35
+  K_DIAG = 29, K_SWEEP = 30, K_SWEEP_LINE_Z = 33, K_SWEEP_ENDS_Z = 34, K_ADJOINT = 35, K_COUNT = 36
 };
 inline const char* kernel_name(int id) {
   static const char* names[K_COUNT] = {
@@ -32,7 +32,7 @@ inline const char* kernel_name(int id) {
       "predict_x", "predict_y", "predict_z", "cell_e", "emf", "emf_corr", "unused_15", "unused_16",
       "ct_half_x", "ct_half_y", "ct_half_z", "half_e", "ctu_correct_x", "ctu_correct_y", "ctu_correct_z",
       "ct_full_x", "ct_full_y", "ct_full_z", "update", "cfl", "diag", "sweep_x", "sweep_y", "sweep_z",
-      "sweep_z", "adjoint"};
+      "sweep_z", "sweep_ends_z", "adjoint"};
   return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 // element kind (ExtKind) and direction of each launch id
@@ -50,7 +50,7 @@ inline void phase_shape(int id, int& kind, int& dir) {
     dir = id < K_CT_FULL ? id - K_CORRECT : id - K_CT_FULL;
   }
   else if (id >= K_SWEEP && id < K_SWEEP + 3) { kind = EXT_SEG; dir = id - K_SWEEP; }
-  else if (id == K_SWEEP_LINE_Z) { kind = EXT_LINE; dir = 2; }
+  else if (id == K_SWEEP_LINE_Z || id == K_SWEEP_ENDS_Z) { kind = EXT_LINE; dir = 2; }
 }
 
 // pointer types of a context's arrays (the adjoint's dual contexts use accessors)

@@ -58,11 +58,22 @@ struct Sweep {  // fused slopes (PLM) + edges + predictor over runs of faces
 };
 // the fused sweep along z over whole lines: the HIP executor gives each line one
 // workgroup whose threads reconstruct one cell each and exchange edge states
-// through LDS; other executors walk the line as one run of faces (same results)
+// through LDS, for the faces 1..n-1; other executors walk the line as one run of
+// faces (same results)
 struct SweepLineZ {
   PMHD_PHASE_TRAITS(K_SWEEP_LINE_Z, EXT_LINE, 2)
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 q) const { k_sweep_run(c, 2, c.bc[2], q, 0, c.nz + 1); }
+};
+// the two end faces (0 and n) of every line along z, which need the edge states of
+// both ends of the line: one thread per line
+struct SweepEndsZ {
+  PMHD_PHASE_TRAITS(K_SWEEP_ENDS_Z, EXT_LINE, 2)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 q) const {
+    k_sweep_run(c, 2, c.bc[2], q, 0, 1);
+    k_sweep_run(c, 2, c.bc[2], q, c.nz, c.nz + 1);
+  }
 };
 struct CellE {
   PMHD_PHASE_TRAITS(K_CELL_E, EXT_CELL, 0)
@@ -144,8 +155,12 @@ template <int D, class T, class X>
 void predictor_sweep(X& ex, const Ctx<T>& c) {
   if (c.order > 0 && (c.seg[D] > 0 || (D == 2 && c.seg[D] < 0))) {
     if (c.order == 2) ex.template run<Slopes<D>>(c);
-    if (D == 2 && c.seg[D] < 0) ex.template run<SweepLineZ>(c);
-    else ex.template run<Sweep<D>>(c);
+    if (D == 2 && c.seg[D] < 0) {
+      ex.template run<SweepLineZ>(c);
+      ex.template run<SweepEndsZ>(c);
+    } else {
+      ex.template run<Sweep<D>>(c);
+    }
     return;
   }
   if (c.order > 0) {
