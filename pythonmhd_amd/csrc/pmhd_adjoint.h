@@ -252,8 +252,8 @@ struct TapeExec {
 
 template <class T, class B>
 struct AdjointHandle : public pmhd_adjoint {
-  // the adjoint keeps faces in the cell type (its unit does not define PMHD_FACES_F64)
-  static_assert(sizeof(face_t<T>) == sizeof(T), "adjoint contexts store faces in the cell type");
+  // the adjoint keeps whole faces in the cell type (its unit does not define PMHD_FACES_SPLIT)
+  static_assert(!split_faces<T>(), "adjoint contexts store whole faces");
   B be;
   pmhd_grid g;
   pmhd_params p;
@@ -354,7 +354,7 @@ struct AdjointHandle : public pmhd_adjoint {
     nsteps = n;
     cur = 0;
     bind_bufs(c, sb, 0);
-    int rc = load_state(be, c, g, prim, cons, fin, sb.U[0], sb.W[0], sb.FB[0], err);
+    int rc = load_state(be, c, g, prim, cons, fin, sb.U[0], sb.W[0], sb.FB[0], sb.FBLO[0], err);
     if (rc) return rc;
     set_dt_scalars();
     const int64_t nck = (n + every - 1) / every;
@@ -371,7 +371,7 @@ struct AdjointHandle : public pmhd_adjoint {
       step_forward();
     }
     double* fo[3] = {fout[0], fout[1], fout[2]};
-    store_state(be, c, g, sb.U[cur], sb.W[cur], sb.FB[cur], prim_out, cons_out, fo);
+    store_state(be, c, g, sb.U[cur], sb.W[cur], sb.FB[cur], sb.FBLO[cur], prim_out, cons_out, fo);
     be.sync();
     have_forward = true;
     const char* e = be.error();

@@ -205,18 +205,11 @@ inline int sweep_segment(int d, int64_t nfaces, int64_t n_along, int cus, bool f
 // the two state buffers of a grid (the update writes the other one)
 template <class T>
 struct StateBufs {
-  using F = face_t<T>;
   T* U[2] = {nullptr, nullptr};
   T* W[2] = {nullptr, nullptr};
-  F* FB[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  T* FB[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  T* FBLO[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};   // split faces only
 };
-
-// room for n face values of a context of T, from an allocator of T values
-template <class T, class A>
-face_t<T>* alloc_faces(A&& alloc, int64_t n) {
-  static_assert(sizeof(face_t<T>) % sizeof(T) == 0, "face type must be a multiple of the cell type");
-  return reinterpret_cast<face_t<T>*>(alloc(n * int64_t(sizeof(face_t<T>) / sizeof(T))));
-}
 
 // cell variables and interface-state variables of a physics
 inline int cell_vars(int physics) { return physics == 1 ? 5 : 8; }
@@ -237,7 +230,11 @@ void alloc_grid(Ctx<T>& c, StateBufs<T>& s, int order, A&& alloc) {
   for (int b = 0; b < 2; ++b) {
     s.U[b] = alloc(cv * N);
     s.W[b] = alloc(cv * N);
-    for (int d = 0; d < 3; ++d) s.FB[b][d] = alloc_faces<T>(alloc, mhd && d < nd && nd >= 2 ? c.nface(d) : 1);
+    for (int d = 0; d < 3; ++d) {
+      const int64_t nf = mhd && d < nd && nd >= 2 ? c.nface(d) : 1;
+      s.FB[b][d] = alloc(nf);
+      s.FBLO[b][d] = alloc(split_faces<T>() ? nf : 1);
+    }
   }
   c.WL = alloc(fv * maxf);
   c.WR = alloc(fv * maxf);
@@ -248,10 +245,12 @@ void alloc_grid(Ctx<T>& c, StateBufs<T>& s, int order, A&& alloc) {
     c.UR[d] = alloc(used ? fv * c.nface(d) : 1);
     c.F[d] = alloc(used ? fv * c.nface(d) : 1);
     c.Fc[d] = alloc(used && nd >= 2 ? fv * c.nface(d) : 1);
-    c.nfb[d] = alloc_faces<T>(alloc, mhd && used && nd >= 2 ? c.nface(d) : 1);
+    c.nfb[d] = alloc(mhd && used && nd >= 2 ? c.nface(d) : 1);
+    c.nfblo[d] = alloc(split_faces<T>() && mhd && used && nd >= 2 ? c.nface(d) : 1);
     c.E[d] = alloc(mhd && (nd == 3 || (nd == 2 && d == 2)) ? c.nedge() : 1);
   }
   c.Ec = alloc(mhd ? (nd == 3 ? 3 * N : N) : 1);
+  c.zend = alloc(mhd && nd == 3 ? 14 * c.ny * c.nx : 1);
 }
 
 // point the context at buffer set `cur` (the update writes 1 - cur)
@@ -265,6 +264,8 @@ void bind_bufs(Ctx<T>& c, const StateBufs<T>& s, int cur) {
   for (int d = 0; d < 3; ++d) {
     c.fb[d] = s.FB[cur][d];
     c.fb2[d] = s.FB[nx][d];
+    c.fblo[d] = s.FBLO[cur][d];
+    c.fb2lo[d] = s.FBLO[nx][d];
   }
 }
 
@@ -293,7 +294,7 @@ void download_vals(B& be, double* dst, const T* src, int64_t n) {
 // load / store the reference-layout state of one grid into buffer set `s`
 template <class T, class B>
 int load_state(B& be, Ctx<T>& c, const pmhd_grid& g, const double* prim, const double* cons,
-               const double* const fin[3], T* U, T* W, face_t<T>* const FB[3], std::string& err) {
+               const double* const fin[3], T* U, T* W, T* const FB[3], T* const FBLO[3], std::string& err) {
   if (!prim || !cons) {
     err = "prim and cons are required";
     return PMHD_EINVAL;
@@ -308,21 +309,38 @@ int load_state(B& be, Ctx<T>& c, const pmhd_grid& g, const double* prim, const d
   upload_vals(be, U, cons, cell_vars(c.physics) * N);
   upload_vals(be, W, prim, cell_vars(c.physics) * N);
   if (mhd && g.ndim >= 2)
-    for (int d = 0; d < g.ndim; ++d) upload_vals(be, FB[d], fin[d], c.nface(d));
+    for (int d = 0; d < g.ndim; ++d) {
+      if (split_faces<T>()) {
+        // high part: the float rounding; low part: the rounding of the remainder
+        const int64_t nf = c.nface(d);
+        std::vector<double> lo(static_cast<size_t>(nf));
+        for (int64_t i = 0; i < nf; ++i) lo[size_t(i)] = fin[d][i] - double(T(fin[d][i]));
+        upload_vals(be, FB[d], fin[d], nf);
+        upload_vals(be, FBLO[d], lo.data(), nf);
+      } else {
+        upload_vals(be, FB[d], fin[d], c.nface(d));
+      }
+    }
   if (mhd && g.ndim == 1) c.bx1d = T(prim[4 * N]);
   return PMHD_OK;
 }
 
 template <class T, class B>
-void store_state(B& be, const Ctx<T>& c, const pmhd_grid& g, const T* U, const T* W,
-                 const face_t<T>* const FB[3],
-                 double* prim, double* cons, double* const fo[3]) {
+void store_state(B& be, const Ctx<T>& c, const pmhd_grid& g, const T* U, const T* W, const T* const FB[3],
+                 const T* const FBLO[3], double* prim, double* cons, double* const fo[3]) {
   const int64_t N = c.ncell();
   if (cons) download_vals(be, cons, U, cell_vars(c.physics) * N);
   if (prim) download_vals(be, prim, W, cell_vars(c.physics) * N);
   if (c.physics == 0 && g.ndim >= 2)
     for (int d = 0; d < g.ndim; ++d)
-      if (fo[d]) download_vals(be, fo[d], FB[d], c.nface(d));
+      if (fo[d]) {
+        download_vals(be, fo[d], FB[d], c.nface(d));
+        if (split_faces<T>()) {
+          std::vector<double> lo(static_cast<size_t>(c.nface(d)));
+          download_vals(be, lo.data(), FBLO[d], c.nface(d));
+          for (int64_t i = 0; i < c.nface(d); ++i) fo[d][i] += lo[size_t(i)];
+        }
+      }
 }
 
 // ---------------------------------------------------------------------------
@@ -382,7 +400,7 @@ struct Handle : public pmhd_handle {
     cycles_since_set = 0;
     speeds_ready = false;
     bind_bufs(c, sb, cur);
-    int rc = load_state(be, c, g, prim, cons, fin, sb.U[0], sb.W[0], sb.FB[0], err);
+    int rc = load_state(be, c, g, prim, cons, fin, sb.U[0], sb.W[0], sb.FB[0], sb.FBLO[0], err);
     if (rc) return rc;
     be.sync();
     const char* e = be.error();
@@ -394,7 +412,7 @@ struct Handle : public pmhd_handle {
     if (which == 1 && cycles_since_set > 0) s = 1 - cur;
     double* fo[3] = {fbx, fby, fbz};
     be.sync();
-    store_state(be, c, g, sb.U[s], sb.W[s], sb.FB[s], prim, cons, fo);
+    store_state(be, c, g, sb.U[s], sb.W[s], sb.FB[s], sb.FBLO[s], prim, cons, fo);
     be.sync();
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
@@ -569,7 +587,7 @@ struct BatchHandle : public pmhd_batch {
     const double* fin[3] = {fbx, fby, fbz};
     bind_bufs(c, bufs[size_t(box)], 0);
     int rc = load_state(be, c, grids[size_t(box)], prim, cons, fin, bufs[size_t(box)].U[0], bufs[size_t(box)].W[0],
-                        bufs[size_t(box)].FB[0], err);
+                        bufs[size_t(box)].FB[0], bufs[size_t(box)].FBLO[0], err);
     if (rc) return rc;
     hclk[size_t(box)].done = 0;
     be.h2d(static_cast<Ctx<T>*>(lay.dctx) + box, &c, sizeof(Ctx<T>));
@@ -582,11 +600,12 @@ struct BatchHandle : public pmhd_batch {
     if (check_box(box)) return PMHD_EINVAL;
     const Ctx<T>& c = hctx[size_t(box)];
     const bool second = which == 1 && hclk[size_t(box)].done > 0;
-    const face_t<T>* fb[3] = {second ? c.fb2[0] : c.fb[0], second ? c.fb2[1] : c.fb[1],
-                              second ? c.fb2[2] : c.fb[2]};
+    const T* fb[3] = {second ? c.fb2[0] : c.fb[0], second ? c.fb2[1] : c.fb[1], second ? c.fb2[2] : c.fb[2]};
+    const T* fblo[3] = {second ? c.fb2lo[0] : c.fblo[0], second ? c.fb2lo[1] : c.fblo[1],
+                        second ? c.fb2lo[2] : c.fblo[2]};
     double* fo[3] = {fbx, fby, fbz};
     be.sync();
-    store_state(be, c, grids[size_t(box)], second ? c.U2 : c.U, second ? c.W2 : c.W, fb, prim, cons, fo);
+    store_state(be, c, grids[size_t(box)], second ? c.U2 : c.U, second ? c.W2 : c.W, fb, fblo, prim, cons, fo);
     be.sync();
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;

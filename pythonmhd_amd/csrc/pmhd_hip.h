@@ -43,9 +43,9 @@ __global__ void __launch_bounds__(256) k_phase_flat(const Ctx<T> c, unsigned n1,
 // blockIdx.x), in chunks of blockDim.x cells: each thread reconstructs one cell
 // and keeps its left edge state; the right edge states go through LDS to the
 // thread of the next face.  Faces 1..n-1 only: the end faces 0 and n need the edge
-// states of both ends of the line and are formed by the SweepEndsZ launch, so every
-// thread of every wave does the same work.  The face states are those of
-// k_sweep_run, so the results are identical.
+// states of both ends of the line, which are kept (store_line_end) for the
+// SweepEndsZ launch, so every thread of every wave does the same work.  The face
+// states are those of k_sweep_run, so the results are identical.
 template <class T>
 __global__ void __launch_bounds__(256) k_sweep_line_z(const Ctx<T> c) {
   __shared__ T sra[7][256];
@@ -62,6 +62,8 @@ __global__ void __launch_bounds__(256) k_sweep_line_z(const Ctx<T> c) {
       cell_edge_states(c, 2, bc, with(q, 2, k), from_s, la, ra);
 #pragma unroll
       for (int v = 0; v < 7; ++v) sra[v][t] = ra[v];
+      if (k == 0) store_line_end(c, q, 0, la);
+      if (k == n - 1) store_line_end(c, q, 1, ra);
     }
     __syncthreads();
     if (k > 0 && k < n) {
@@ -402,8 +404,7 @@ struct HipExec {
     if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return;
     const int64_t total = n[0] * n[1] * n[2];
     // profiled work: faces for the launches over runs or lines of faces
-    const int64_t work = Ph::kind == EXT_SEG || Ph::id == K_SWEEP_LINE_Z ? c.nface(Ph::dir)
-                         : (Ph::id == K_SWEEP_ENDS_Z ? 2 * total : total);
+    const int64_t work = Ph::kind == EXT_SEG || Ph::id == K_SWEEP_LINE_Z ? c.nface(Ph::dir) : total;
     if constexpr (Ph::id == K_SWEEP_LINE_Z) {
       // one workgroup per line along z: the smallest of 64, 128, 256 threads that
       // covers the line, else 256 in chunks

@@ -17,8 +17,9 @@ PMHD_NS_BEGIN
 enum { OUTFLOW = 0, PERIODIC = 1 };
 // element kinds a launch runs over
 // EXT_SEG: runs of consecutive faces along the launch direction (one run per thread);
-// EXT_LINE: whole lines of faces along the launch direction (one per element)
-enum ExtKind { EXT_CELL = 0, EXT_FACE = 1, EXT_SLOPE = 2, EXT_EDGE = 3, EXT_SEG = 4, EXT_LINE = 5 };
+// EXT_LINE: whole lines of faces along the launch direction (one per element);
+// EXT_ENDS: the two end faces of every line along the launch direction
+enum ExtKind { EXT_CELL = 0, EXT_FACE = 1, EXT_SLOPE = 2, EXT_EDGE = 3, EXT_SEG = 4, EXT_LINE = 5, EXT_ENDS = 6 };
 
 // launch identifiers (for per-kernel timing); "+ d" for the per-direction ones
 enum KernelId {
@@ -50,7 +51,8 @@ inline void phase_shape(int id, int& kind, int& dir) {
     dir = id < K_CT_FULL ? id - K_CORRECT : id - K_CT_FULL;
   }
   else if (id >= K_SWEEP && id < K_SWEEP + 3) { kind = EXT_SEG; dir = id - K_SWEEP; }
-  else if (id == K_SWEEP_LINE_Z || id == K_SWEEP_ENDS_Z) { kind = EXT_LINE; dir = 2; }
+  else if (id == K_SWEEP_LINE_Z) { kind = EXT_LINE; dir = 2; }
+  else if (id == K_SWEEP_ENDS_Z) { kind = EXT_ENDS; dir = 2; }
 }
 
 // pointer types of a context's arrays (the adjoint's dual contexts use accessors)
@@ -64,21 +66,21 @@ using CPtr = typename Mem<T>::cp;
 template <class T>
 using Ptr = typename Mem<T>::p;
 
-// storage type of a context's face-centred field: the FP32-bulk step keeps its
-// faces in double (its translation unit defines PMHD_FACES_F64), so that the
-// constrained-transport update preserves div B to double rounding
+// The FP32-bulk step keeps its face-centred field as two float arrays, the high
+// and low parts of a double value (its translation unit defines PMHD_FACES_SPLIT):
+// every consumer reads the high part, which is the float rounding of the double
+// value, and the constrained-transport update works on the double value, so div B
+// stays at double rounding while the consumers move float words.
 template <class T>
-struct FaceOf {
-  using type = T;
-};
-#ifdef PMHD_FACES_F64
+PMHD_FI constexpr bool split_faces() {
+  return false;
+}
+#ifdef PMHD_FACES_SPLIT
 template <>
-struct FaceOf<float> {
-  using type = double;
-};
+PMHD_FI constexpr bool split_faces<float>() {
+  return true;
+}
 #endif
-template <class T>
-using face_t = typename FaceOf<T>::type;
 
 template <class T>
 struct Ctx {
@@ -103,14 +105,18 @@ struct Ctx {
   T emfq[3];         // 0.5*(0.5*(dt/dx)), ...
   T bx1d;            // uniform B_x of the 1D step
   double ldh[2];     // 0.5*dt/dy as the reference's 80-bit value, hi + lo (FP64 only)
-  // state (faces in the face type face_t<T>)
+  // state
   CPtr<T> U;
   CPtr<T> W;
   Ptr<T> U2;
   Ptr<T> W2;
-  CPtr<face_t<T>> fb[3];
-  Ptr<face_t<T>> fb2[3];
-  Ptr<face_t<T>> nfb[3];
+  CPtr<T> fb[3];
+  Ptr<T> fb2[3];
+  Ptr<T> nfb[3];
+  // low parts of the face field when split_faces<T>() (else unused)
+  CPtr<T> fblo[3];
+  Ptr<T> fb2lo[3];
+  Ptr<T> nfblo[3];
   // work
   Ptr<T> WL;
   Ptr<T> WR;
@@ -120,6 +126,8 @@ struct Ctx {
   Ptr<T> F[3];           // predictor fluxes
   Ptr<T> Fc[3];          // corrector fluxes
   Ptr<T> Ec;
+  Ptr<T> zend;       // per line along z: left edge state of cell 0 (7) and right edge state
+                     // of cell nz-1 (7), planes [v][j * nx + i], for the end faces
   Ptr<T> E[3];
   // element counts
   PMHD_FI int64_t ncell() const { return ny * nx * nz; }
@@ -150,9 +158,9 @@ struct Ctx {
       n[0] = sny(d); n[1] = snx(d); n[2] = snz(d);
     } else if (kind == EXT_EDGE) {
       n[0] = eny(); n[1] = enx(); n[2] = enz();
-    } else if (kind == EXT_LINE) {
+    } else if (kind == EXT_LINE || kind == EXT_ENDS) {
       n[0] = fny(d); n[1] = fnx(d); n[2] = fnz(d);
-      n[d == 0 ? 1 : (d == 1 ? 0 : 2)] = 1;
+      n[d == 0 ? 1 : (d == 1 ? 0 : 2)] = kind == EXT_LINE ? 1 : 2;
     } else if (kind == EXT_SEG) {
       n[0] = fny(d); n[1] = fnx(d); n[2] = fnz(d);
       const int64_t s = seg[d] > 0 ? seg[d] : 1;
@@ -579,6 +587,48 @@ PMHD_FI void k_sweep_run(const Ctx<T>& c, int d, int bc, I3 q, int64_t f0, int64
   }
 }
 
+// the line (j, i) along z: keep the left edge state of cell 0 and the right edge
+// state of cell nz-1 for the end faces (k_sweep_end)
+template <class T>
+PMHD_FI void store_line_end(const Ctx<T>& c, I3 q, int side, const T s[7]) {
+  const int64_t L = c.ny * c.nx;
+  const int64_t l = q.j * c.nx + q.i;
+#pragma unroll
+  for (int v = 0; v < 7; ++v) st(c.zend + (7 * side + v) * L, l, s[v]);
+}
+template <class T>
+PMHD_FI void k_line_ends(const Ctx<T>& c, int bc, I3 q) {
+  const bool from_s = c.order == 2;
+  T la[7], ra[7];
+  cell_edge_states(c, 2, bc, with(q, 2, 0), from_s, la, ra);
+  store_line_end(c, q, 0, la);
+  cell_edge_states(c, 2, bc, with(q, 2, c.nz - 1), from_s, la, ra);
+  store_line_end(c, q, 1, ra);
+}
+// end face 0 (q.k == 0) or nz (q.k == 1) of the line (j, i) along z from the stored
+// edge states: face 0 takes the left edge of cell 0 on both sides (outflow) or the
+// right edge of cell nz-1 and the left edge of cell 0 (periodic); face nz takes the
+// right edge of cell nz-1 on both sides (outflow) or the right edge of cell nz-1
+// and the left edge of cell 0 (periodic)
+template <class T>
+PMHD_FI void k_sweep_end(const Ctx<T>& c, int bc, I3 q) {
+  const int64_t L = c.ny * c.nx;
+  const int64_t l = q.j * c.nx + q.i;
+  T l0[7], rn[7], wl[7], wr[7];
+#pragma unroll
+  for (int v = 0; v < 7; ++v) {
+    l0[v] = ld(c.zend + v * L, l);
+    rn[v] = ld(c.zend + (7 + v) * L, l);
+  }
+  const bool face0 = q.k == 0;
+#pragma unroll
+  for (int v = 0; v < 7; ++v) {
+    wl[v] = face0 ? (bc == OUTFLOW ? l0[v] : rn[v]) : rn[v];
+    wr[v] = face0 ? l0[v] : (bc == OUTFLOW ? rn[v] : l0[v]);
+  }
+  predict_face(c, 2, with(q, 2, face0 ? 0 : c.nz), wl, wr);
+}
+
 template <class T>
 PMHD_FI void k_sweep(const Ctx<T>& c, int d, int bc, I3 q) {
   const int64_t n = c.n_along(d);
@@ -900,36 +950,51 @@ PMHD_FI void ctu_states(const Ctx<T>& c, int d, I3 f, T ul[7], T ur[7]) {
 // constrained transport of the face field (Source/EvolveFunctions_MHD.py:1003-1028,
 // :1082-1104 for 3D; :579-586, :643-649 for 2D)
 // ===========================================================================
+// fin/fout: face arrays (high parts); lin/lout: their low parts (split faces only)
 template <class T, class PI, class PO>
-PMHD_FI void k_ct(const Ctx<T>& c, int d, const PI& fin, const PO& fout, const T h[3], I3 f) {
-  // in the face type F (double for the FP32-bulk step): the curl differences of the
-  // edge EMFs and the update then round at double precision
-  using F = face_t<T>;
+PMHD_FI void k_ct(const Ctx<T>& c, int d, const PI& fin, const PI& lin, const PO& fout, const PO& lout,
+                  const T h[3], I3 f) {
+  // F: double for split faces (the curl differences of the edge EMFs and the
+  // update then round at double precision), else the cell type
+  using F = typename std::conditional<split_faces<T>(), double, T>::type;
   const int64_t idx = c.fid(d, f.j, f.i, f.k);
+  F old;
+  if constexpr (split_faces<T>()) old = F(ld(fin, idx)) + F(ld(lin, idx));
+  else old = ld(fin, idx);
+  F res;
   if (c.ndim == 2) {
     const auto E = c.E[2];
     if (d == 0) {
-      st(fout, idx, ld(fin, idx) - F(h[1]) * (F(ld(E, c.eid(f.j + 1, f.i, 0))) - F(ld(E, c.eid(f.j, f.i, 0)))));
+      res = old - F(h[1]) * (F(ld(E, c.eid(f.j + 1, f.i, 0))) - F(ld(E, c.eid(f.j, f.i, 0))));
     } else {
-      st(fout, idx, ld(fin, idx) + F(h[0]) * (F(ld(E, c.eid(f.j, f.i + 1, 0))) - F(ld(E, c.eid(f.j, f.i, 0)))));
+      res = old + F(h[0]) * (F(ld(E, c.eid(f.j, f.i + 1, 0))) - F(ld(E, c.eid(f.j, f.i, 0))));
     }
-    return;
-  }
-  const auto Ex = c.E[0];
-  const auto Ey = c.E[1];
-  const auto Ez = c.E[2];
-  F a, b;
-  if (d == 0) {
-    a = F(h[2]) * (F(ld(Ey, c.eid(f.j, f.i, f.k + 1))) - F(ld(Ey, c.eid(f.j, f.i, f.k))));
-    b = F(h[1]) * (F(ld(Ez, c.eid(f.j + 1, f.i, f.k))) - F(ld(Ez, c.eid(f.j, f.i, f.k))));
-  } else if (d == 1) {
-    a = F(h[0]) * (F(ld(Ez, c.eid(f.j, f.i + 1, f.k))) - F(ld(Ez, c.eid(f.j, f.i, f.k))));
-    b = F(h[2]) * (F(ld(Ex, c.eid(f.j, f.i, f.k + 1))) - F(ld(Ex, c.eid(f.j, f.i, f.k))));
   } else {
-    a = F(h[1]) * (F(ld(Ex, c.eid(f.j + 1, f.i, f.k))) - F(ld(Ex, c.eid(f.j, f.i, f.k))));
-    b = F(h[0]) * (F(ld(Ey, c.eid(f.j, f.i + 1, f.k))) - F(ld(Ey, c.eid(f.j, f.i, f.k))));
+    const auto Ex = c.E[0];
+    const auto Ey = c.E[1];
+    const auto Ez = c.E[2];
+    F a, b;
+    if (d == 0) {
+      a = F(h[2]) * (F(ld(Ey, c.eid(f.j, f.i, f.k + 1))) - F(ld(Ey, c.eid(f.j, f.i, f.k))));
+      b = F(h[1]) * (F(ld(Ez, c.eid(f.j + 1, f.i, f.k))) - F(ld(Ez, c.eid(f.j, f.i, f.k))));
+    } else if (d == 1) {
+      a = F(h[0]) * (F(ld(Ez, c.eid(f.j, f.i + 1, f.k))) - F(ld(Ez, c.eid(f.j, f.i, f.k))));
+      b = F(h[2]) * (F(ld(Ex, c.eid(f.j, f.i, f.k + 1))) - F(ld(Ex, c.eid(f.j, f.i, f.k))));
+    } else {
+      a = F(h[1]) * (F(ld(Ex, c.eid(f.j + 1, f.i, f.k))) - F(ld(Ex, c.eid(f.j, f.i, f.k))));
+      b = F(h[0]) * (F(ld(Ey, c.eid(f.j, f.i + 1, f.k))) - F(ld(Ey, c.eid(f.j, f.i, f.k))));
+    }
+    res = old + (a - b);
   }
-  st(fout, idx, ld(fin, idx) + (a - b));
+  if constexpr (split_faces<T>()) {
+    const T hi = T(res);
+    st(fout, idx, hi);
+    st(lout, idx, T(res - F(hi)));
+  } else {
+    (void)lin;
+    (void)lout;
+    st(fout, idx, res);
+  }
 }
 
 // ===========================================================================
@@ -1045,13 +1110,18 @@ PMHD_FI void k_diag(const Ctx<T>& c, int64_t idx, double sums[9], double maxes[2
   sums[8] = bad;
   double div = 0.0, bmax = 0.0, wmin = c.wd[0];
   if (c.ndim >= 2 && c.physics == 0) {
-    double fx0 = c.fb[0][c.fid(0, p.j, p.i, p.k)], fx1 = c.fb[0][c.fid(0, p.j, p.i + 1, p.k)];
-    double fy0 = c.fb[1][c.fid(1, p.j, p.i, p.k)], fy1 = c.fb[1][c.fid(1, p.j + 1, p.i, p.k)];
+    auto face = [&](int a, int64_t i) {
+      double v = double(ld(c.fb[a], i));
+      if constexpr (split_faces<T>()) v += double(ld(c.fblo[a], i));
+      return v;
+    };
+    double fx0 = face(0, c.fid(0, p.j, p.i, p.k)), fx1 = face(0, c.fid(0, p.j, p.i + 1, p.k));
+    double fy0 = face(1, c.fid(1, p.j, p.i, p.k)), fy1 = face(1, c.fid(1, p.j + 1, p.i, p.k));
     div = (fx1 - fx0) / c.wd[0] + (fy1 - fy0) / c.wd[1];
     bmax = npmax(npmax(fabs(fx0), fabs(fx1)), npmax(fabs(fy0), fabs(fy1)));
     wmin = npmin(wmin, c.wd[1]);
     if (c.ndim == 3) {
-      double fz0 = c.fb[2][c.fid(2, p.j, p.i, p.k)], fz1 = c.fb[2][c.fid(2, p.j, p.i, p.k + 1)];
+      double fz0 = face(2, c.fid(2, p.j, p.i, p.k)), fz1 = face(2, c.fid(2, p.j, p.i, p.k + 1));
       div += (fz1 - fz0) / c.wd[2];
       bmax = npmax(bmax, npmax(fabs(fz0), fabs(fz1)));
       wmin = npmin(wmin, c.wd[2]);

@@ -56,24 +56,23 @@ struct Sweep {  // fused slopes (PLM) + edges + predictor over runs of faces
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 q) const { k_sweep(c, D, c.bc[D], q); }
 };
-// the fused sweep along z over whole lines: the HIP executor gives each line one
+// the fused sweep along z over whole lines, faces 1..n-1, keeping the edge states
+// of the two end cells of each line: the HIP executor gives each line one
 // workgroup whose threads reconstruct one cell each and exchange edge states
-// through LDS, for the faces 1..n-1; other executors walk the line as one run of
-// faces (same results)
+// through LDS; other executors walk the line as one run of faces (same results)
 struct SweepLineZ {
   PMHD_PHASE_TRAITS(K_SWEEP_LINE_Z, EXT_LINE, 2)
   template <class T>
-  PMHD_FI void operator()(const Ctx<T>& c, I3 q) const { k_sweep_run(c, 2, c.bc[2], q, 0, c.nz + 1); }
-};
-// the two end faces (0 and n) of every line along z, which need the edge states of
-// both ends of the line: one thread per line
-struct SweepEndsZ {
-  PMHD_PHASE_TRAITS(K_SWEEP_ENDS_Z, EXT_LINE, 2)
-  template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 q) const {
-    k_sweep_run(c, 2, c.bc[2], q, 0, 1);
-    k_sweep_run(c, 2, c.bc[2], q, c.nz, c.nz + 1);
+    k_sweep_run(c, 2, c.bc[2], q, 1, c.nz);
+    k_line_ends(c, c.bc[2], q);
   }
+};
+// the two end faces (0 and n) of every line along z from the kept end states
+struct SweepEndsZ {
+  PMHD_PHASE_TRAITS(K_SWEEP_ENDS_Z, EXT_ENDS, 2)
+  template <class T>
+  PMHD_FI void operator()(const Ctx<T>& c, I3 q) const { k_sweep_end(c, c.bc[2], q); }
 };
 struct CellE {
   PMHD_PHASE_TRAITS(K_CELL_E, EXT_CELL, 0)
@@ -94,7 +93,9 @@ template <int D>
 struct CtHalf {  // half-step face field
   PMHD_PHASE_TRAITS(K_CT_HALF + D, EXT_FACE, D)
   template <class T>
-  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_ct(c, D, c.fb[D], c.nfb[D], c.hdtd, p); }
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const {
+    k_ct(c, D, c.fb[D], c.fblo[D], c.nfb[D], c.nfblo[D], c.hdtd, p);
+  }
 };
 struct HalfE {
   PMHD_PHASE_TRAITS(K_HALF_E, EXT_CELL, 0)
@@ -111,7 +112,9 @@ template <int D>
 struct CtFull {  // full-step face field
   PMHD_PHASE_TRAITS(K_CT_FULL + D, EXT_FACE, D)
   template <class T>
-  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_ct(c, D, c.fb[D], c.fb2[D], c.dtd, p); }
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const {
+    k_ct(c, D, c.fb[D], c.fblo[D], c.fb2[D], c.fb2lo[D], c.dtd, p);
+  }
 };
 template <class T>
 PMHD_FI void speeds_out(const T t[3], double o[3]) {
@@ -276,11 +279,13 @@ PMHD_FI void swap_buffers(Ctx<T>& c) {
   c.W2 = const_cast<T*>(c.W);
   c.U = u;
   c.W = w;
-  using F = face_t<T>;
   for (int d = 0; d < 3; ++d) {
-    F* f = c.fb2[d];
-    c.fb2[d] = const_cast<F*>(c.fb[d]);
+    T* f = c.fb2[d];
+    c.fb2[d] = const_cast<T*>(c.fb[d]);
     c.fb[d] = f;
+    T* l = c.fb2lo[d];
+    c.fb2lo[d] = const_cast<T*>(c.fblo[d]);
+    c.fblo[d] = l;
   }
 }
 

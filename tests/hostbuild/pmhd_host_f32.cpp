@@ -1,6 +1,6 @@
-// Host test build: the FP32-bulk step (faces in double, as in the HIP unit).
+// Host test build: the FP32-bulk step (split faces, as in the HIP unit).
 #define PMHD_TU_NS f32
-#define PMHD_FACES_F64
+#define PMHD_FACES_SPLIT
 #define PMHD_FAST_ORDER
 #include "pmhd_host.h"
 
