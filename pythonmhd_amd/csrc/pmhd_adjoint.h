@@ -70,6 +70,8 @@ PMHD_FI void make_dual_ctx(const Ctx<V>& c, const Ctx<double>& a, AdState* st, C
   d.bc_zrec = c.bc_zrec;
   d.order = c.order;
   d.efix = c.efix;
+  d.ec_next = c.ec_next;
+  d.ec_valid = c.ec_valid;
   d.gamma = D(double(c.gamma));
   d.eps = D(double(c.eps));
   d.min_dens = D(double(c.min_dens));

@@ -355,6 +355,7 @@ struct Handle : public pmhd_handle {
   int cur = 0;
   int64_t cycles_since_set = 0;
   bool speeds_ready = false;   // the last update left the new state's CFL maxima
+  bool ec_ready = false;       // the last update left the new state's cell E
 
   T* dalloc(int64_t n) {
     if (n <= 0) n = 1;
@@ -380,6 +381,7 @@ struct Handle : public pmhd_handle {
     bind_bufs(c, sb, cur);
     for (int d = 0; d < g.ndim; ++d)
       c.seg[d] = sweep_segment(d, c.nface(d), c.n_along(d), be.compute_units(), sizeof(T) == sizeof(double));
+    c.ec_next = p.physics == 0 && g.ndim >= 2;
     const char* e = be.error();
     if (e) return fail(PMHD_EHIP, e);
     return PMHD_OK;
@@ -399,6 +401,7 @@ struct Handle : public pmhd_handle {
     cur = 0;
     cycles_since_set = 0;
     speeds_ready = false;
+    ec_ready = false;
     bind_bufs(c, sb, cur);
     int rc = load_state(be, c, g, prim, cons, fin, sb.U[0], sb.W[0], sb.FB[0], sb.FBLO[0], err);
     if (rc) return rc;
@@ -436,11 +439,13 @@ struct Handle : public pmhd_handle {
       c.ldh[0] = static_cast<double>(hl);
       c.ldh[1] = static_cast<double>(hl - static_cast<long double>(c.ldh[0]));
     }
+    c.ec_valid = ec_ready;
     run_cycle(be.exec(), c);
     cur = 1 - cur;
     bind_bufs(c, sb, cur);
     ++cycles_since_set;
     speeds_ready = true;
+    ec_ready = c.ec_next != 0;
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
   }
@@ -541,7 +546,10 @@ struct BatchHandle : public pmhd_batch {
     } catch (const std::bad_alloc&) {
       return fail(PMHD_ENOMEM, "device allocation failed");
     }
-    for (int b = 0; b < n; ++b) bind_bufs(hctx[size_t(b)], bufs[size_t(b)], 0);
+    for (int b = 0; b < n; ++b) {
+      bind_bufs(hctx[size_t(b)], bufs[size_t(b)], 0);
+      hctx[size_t(b)].ec_next = p.physics == 0 && grids[0].ndim >= 2;
+    }
     be.zero(lay.dred, 3 * sizeof(unsigned long long) * size_t(n));
     // tile tables
     lay.ndim = grids[0].ndim;
@@ -628,6 +636,9 @@ struct BatchHandle : public pmhd_batch {
       // later cycles take their speeds from the previous update's epilogue
       if (it == 0) bx.cfl(hctx[0]);
       bx.template begin<T>(p.cfl, t_end, max_cycles, 1);
+      // the first cycle forms the cell E of the set state; later ones take the one
+      // each box's update stored
+      hctx[0].ec_valid = it > 0;
       run_cycle(bx, hctx[0]);
       bx.template end<T>();
       if ((it + 1) % 16 == 0 && it + 1 < iters) {

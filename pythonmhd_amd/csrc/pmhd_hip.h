@@ -30,9 +30,19 @@ __global__ void __launch_bounds__(256) k_phase(const Ctx<T> c, int n1, int n2) {
 
 // one thread per element of an (n0, n1, n2) box numbered flat with n2 fastest:
 // launches over runs of faces, whose fastest axis may be short (the runs along z)
+// Workgroups are handed to the 8 XCDs in turn, and each XCD has its own L2: number
+// them so that each XCD walks a contiguous eighth of the index box, whose
+// neighbouring elements (the stencils' shared reads) then meet in one L2.
+__device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned per = nb / 8;
+  return (per >= 8 && b < per * 8) ? (b % 8) * per + b / 8 : b;
+}
+
 template <class Ph, class T>
 __global__ void __launch_bounds__(256) k_phase_flat(const Ctx<T> c, unsigned n1, unsigned n2, unsigned total) {
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  // (the sweeps' runs of faces share little between neighbouring threads)
+  const unsigned b = Ph::kind == EXT_SEG ? unsigned(blockIdx.x) : xcd_block(blockIdx.x, gridDim.x);
+  const unsigned t = b * blockDim.x + threadIdx.x;
   if (t >= total) return;
   const unsigned k = t % n2;
   const unsigned r = t / n2;
@@ -388,6 +398,7 @@ struct HipExec {
   }
 
   int flat_block = 256;   // threads per block of the flat launches (64..256)
+  bool flat_all = true;   // every phase as a flat, XCD-numbered launch
 
   static void grid_of(const int64_t n[3], dim3& grid, dim3& block) {
     unsigned bx = 1;
@@ -423,7 +434,7 @@ struct HipExec {
     // n + 1 faces or edge points along z: 257 = 4 x 64 + 1)
     const double launched = double(grid.x) * block.x * double(grid.y) * block.y * double(grid.z);
     const bool sparse = launched > 1.05 * double(total);
-    if ((Ph::kind == EXT_SEG || sparse) && total < (int64_t(1) << 31)) {
+    if ((Ph::kind == EXT_SEG || sparse || flat_all) && total < (int64_t(1) << 31)) {
       const unsigned nb = unsigned((total + flat_block - 1) / flat_block);
       timed(Ph::id, work, [&] {
         hipLaunchKernelGGL((k_phase_flat<Ph, T>), dim3(nb), dim3(unsigned(flat_block)), 0, s, c, unsigned(n[1]),

@@ -92,6 +92,8 @@ struct Ctx {
   int bc_zrec;       // boundary condition the z reconstruction/corrections use
   int order;         // 0 copy, 1 PLM, 2 PPM
   int efix;
+  int ec_next;       // the update also stores the new state's cell E (the next cycle's CellE)
+  int ec_valid;      // Ec already holds the current state's cell E (the cycle skips CellE)
   int seg[3];        // faces per thread of the fused predictor sweep along x, y, z (0: separate
                      // slope / edge / predictor launches; -1 along z: one workgroup per line)
   T gamma, eps, min_dens, min_pres, min_energy;
@@ -640,6 +642,20 @@ PMHD_FI void k_sweep(const Ctx<T>& c, int d, int bc, I3 q) {
 // ===========================================================================
 // cell-centred electric field
 // ===========================================================================
+// cell E of a cell from its density, momenta, velocities and cell B (3D: (B x M)/rho;
+// 2D: v_y B_x - v_x B_y)
+template <class T>
+PMHD_FI void store_cell_e(const Ctx<T>& c, int64_t idx, T rho, T mx, T my, T mz, T vx, T vy, T bx, T by, T bz) {
+  const int64_t N = c.ncell();
+  if (c.ndim == 3) {
+    st(c.Ec, idx, (by * mz - bz * my) / rho);
+    st(c.Ec + N, idx, (bz * mx - bx * mz) / rho);
+    st(c.Ec + 2 * N, idx, (bx * my - by * mx) / rho);
+  } else {
+    st(c.Ec, idx, vy * bx - vx * by);
+  }
+}
+
 template <class T>
 PMHD_FI void k_cell_e(const Ctx<T>& c, I3 p) {
   const int64_t N = c.ncell();
@@ -647,11 +663,10 @@ PMHD_FI void k_cell_e(const Ctx<T>& c, I3 p) {
   if (c.ndim == 3) {
     T rho = ld(c.U, idx), mx = ld(c.U + N, idx), my = ld(c.U + 2 * N, idx), mz = ld(c.U + 3 * N, idx);
     T bx = ld(c.W + 4 * N, idx), by = ld(c.W + 5 * N, idx), bz = ld(c.W + 6 * N, idx);
-    st(c.Ec, idx, (by * mz - bz * my) / rho);
-    st(c.Ec + N, idx, (bz * mx - bx * mz) / rho);
-    st(c.Ec + 2 * N, idx, (bx * my - by * mx) / rho);
+    store_cell_e<T>(c, idx, rho, mx, my, mz, T(0), T(0), bx, by, bz);
   } else {
-    st(c.Ec, idx, ld(c.W + 2 * N, idx) * ld(c.W + 4 * N, idx) - ld(c.W + N, idx) * ld(c.W + 5 * N, idx));
+    store_cell_e<T>(c, idx, T(0), T(0), T(0), T(0), ld(c.W + N, idx), ld(c.W + 2 * N, idx), ld(c.W + 4 * N, idx),
+                 ld(c.W + 5 * N, idx), T(0));
   }
 }
 
@@ -1089,6 +1104,8 @@ PMHD_FI void k_update(const Ctx<T>& c, I3 p) {
   st(c.W2 + 5 * N, idx, w[4]);
   st(c.W2 + 6 * N, idx, w[5]);
   st(c.W2 + 7 * N, idx, w[6]);
+  // the next cycle's cell E from the stored values (what CellE would read back)
+  if (c.ec_next && c.ndim >= 2) store_cell_e<T>(c, idx, u[0], u[1], u[2], u[3], w[1], w[2], bx, w[4], w[5]);
 }
 
 // ===========================================================================

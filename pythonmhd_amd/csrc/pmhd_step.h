@@ -193,7 +193,7 @@ void cycle(X& ex, const Ctx<T>& c) {
     ex.template run_max<Update>(c);
     return;
   }
-  ex.template run<CellE>(c);
+  if (!c.ec_valid) ex.template run<CellE>(c);   // else the last update stored it
   ex.template run<EmfPredictor>(c);   // corner EMFs of the predictor fluxes
   each_dir<CtHalf>(ex, c);            // half-step face field
   ex.template run<HalfE>(c);
