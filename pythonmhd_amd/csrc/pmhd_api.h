@@ -546,9 +546,23 @@ struct BatchHandle : public pmhd_batch {
     } catch (const std::bad_alloc&) {
       return fail(PMHD_ENOMEM, "device allocation failed");
     }
+    // fused predictor sweeps with one run length per direction for all boxes (a
+    // batch tile lies in one y row, so along x in 2D each thread takes one face)
+    int seg[3] = {0, 0, 0};
+    {
+      int64_t nf[3] = {0, 0, 0};
+      for (int b = 0; b < n; ++b)
+        for (int d = 0; d < grids[0].ndim; ++d) nf[d] += hctx[size_t(b)].nface(d);
+      for (int d = 0; d < grids[0].ndim; ++d) {
+        seg[d] = sweep_segment(d, nf[d], int64_t(0), be.compute_units(), sizeof(T) == sizeof(double));
+        if (seg[d] < 0) seg[d] = 2;   // no line launches in batches
+        if (seg[d] > 0 && grids[0].ndim == 2 && d == 0) seg[d] = 1;
+      }
+    }
     for (int b = 0; b < n; ++b) {
       bind_bufs(hctx[size_t(b)], bufs[size_t(b)], 0);
       hctx[size_t(b)].ec_next = p.physics == 0 && grids[0].ndim >= 2;
+      for (int d = 0; d < 3; ++d) hctx[size_t(b)].seg[d] = seg[d];
     }
     be.zero(lay.dred, 3 * sizeof(unsigned long long) * size_t(n));
     // tile tables
