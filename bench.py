@@ -259,6 +259,27 @@ def measured_traffic(kernel, n, word_bytes):
     return None if row is None else row['fetch_bytes'] + row['write_bytes']
 
 
+VALU = os.path.join(ROOT, 'profiles', 'valu_fp32_turbulence256.json')
+SIMDS, CLOCK_HZ, CYCLES_PER_VALU = 1024, 2.4e9, 4     # MI355X: 256 CUs x 4 SIMDs; wave64 on a SIMD16
+
+
+def valu_roof(ms_per_step, n=256, sweep=None):
+    """The step against the VALU issue rate: the counted VALU instructions of one
+    cycle (rocprofv3 SQ_INSTS_VALU per launch, committed for the FP32 256^3 default
+    launch plan) at one wave64 instruction per SIMD per 4 cycles, over the measured
+    cycle time.  None for other workloads."""
+    if n != 256 or sweep is not None or not os.path.exists(VALU):
+        return None
+    with open(VALU) as f:
+        v = json.load(f)
+    inst = sum(v['per_cycle'].values())
+    issue_ms = inst * CYCLES_PER_VALU / SIMDS / CLOCK_HZ * 1e3
+    cells = n ** 3
+    return {"valu_instructions_per_cell_update": round(inst * 64 / cells, 1), "issue_ms_per_step": round(issue_ms, 4),
+            "frac": round(issue_ms / ms_per_step, 4), "ceiling_cell_updates_per_s": cells / issue_ms * 1e3,
+            "source": os.path.relpath(VALU, ROOT)}
+
+
 def roofline(prof, word_bytes, n=256):
     if not prof:
         return None
@@ -324,6 +345,7 @@ def main():
                       "cells": cells, "reconstruct_order": 1, "parallelism": "single GPU"},
            "roofline": roofline(prof32, 4, n),
            "hbm_roof_fraction_step": v32 * 64 / HBM_PEAK,
+           "valu_roof_step": valu_roof(el32 / K * 1e3, n, sweep),
            "kernel_ms_per_step": {k: round(v[0] / 3, 4) for k, v in sorted(prof32.items())}}
     if sweep is not None:
         out["config"]["sweep_run_lengths"] = sweep

@@ -422,36 +422,55 @@ PMHD_FI void trace_cell(const PrimEig<T>& E, const T wl[7], const T wc[7], const
   }
   T c3 = T(0.5) * npmax(E.lam[6], T(0)) * dtdx;
   T c4 = T(-0.5) * npmin(E.lam[0], T(0)) * dtdx;
+  const T hdt = T(0.5) * dtdx;
+#ifdef PMHD_FAST_ORDER
+  // L_i . (v3 (c1 + c2) + v4 c2) = v3 (L_i.c1 + L_i.c2) + v4 L_i.c2: project c1 and c2
+  // once and weight the projections per wave, without branches (rounds differently
+  // from the per-wave projections below).  PLM has c2 = 0: its terms are left out
+  // (a product with zero is not folded without finite-math assumptions).
+  T p1[7];
+  const T vl = hdt * E.lam[0], vr = hdt * E.lam[6];
+  if (order == 2) {
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      la[v] = la[v] + c4 * (c1[v] + (T(1) - T(kFourThirds) * c4) * c2[v]);
+      ra[v] = ra[v] - c3 * (c1[v] - (T(1) - T(kFourThirds) * c3) * c2[v]);
+    }
+    T p2[7];
+    eig_project(E, c1, p1);
+    eig_project(E, c2, p2);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      const T v2 = hdt * E.lam[i];
+      const T v3l = vl - v2, v4l = T(kFourThirds) * (vl * vl - v2 * v2);
+      const T v3r = vr - v2, v4r = T(kFourThirds) * (vr * vr - v2 * v2);
+      const T fl = E.lam[i] <= T(0) ? v3l * (p1[i] + p2[i]) + v4l * p2[i] : T(0);
+      const T fr = E.lam[i] >= T(0) ? v3r * (p1[i] - p2[i]) + v4r * p2[i] : T(0);
+      eig_add_col(E, i, fl, la);
+      eig_add_col(E, i, fr, ra);
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+      la[v] = la[v] + c4 * c1[v];
+      ra[v] = ra[v] - c3 * c1[v];
+    }
+    eig_project(E, c1, p1);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      const T v2 = hdt * E.lam[i];
+      const T fl = E.lam[i] <= T(0) ? (vl - v2) * p1[i] : T(0);
+      const T fr = E.lam[i] >= T(0) ? (vr - v2) * p1[i] : T(0);
+      eig_add_col(E, i, fl, la);
+      eig_add_col(E, i, fr, ra);
+    }
+  }
+#else
 #pragma unroll
   for (int v = 0; v < 7; ++v) {
     la[v] = la[v] + c4 * (c1[v] + (T(1) - T(kFourThirds) * c4) * c2[v]);
     ra[v] = ra[v] - c3 * (c1[v] - (T(1) - T(kFourThirds) * c3) * c2[v]);
   }
-  const T hdt = T(0.5) * dtdx;
-#ifdef PMHD_FAST_ORDER
-  // L_i . (v3 (c1 + c2) + v4 c2) = v3 (L_i.c1 + L_i.c2) + v4 L_i.c2: project c1 and c2
-  // once and weight the projections per wave, without branches (rounds differently
-  // from the per-wave projections below)
-  T p1[7], p2[7];
-  eig_project(E, c1, p1);
-  if (order == 2) {
-    eig_project(E, c2, p2);
-  } else {
-#pragma unroll
-    for (int i = 0; i < 7; ++i) p2[i] = T(0);
-  }
-  const T vl = hdt * E.lam[0], vr = hdt * E.lam[6];
-#pragma unroll
-  for (int i = 0; i < 7; ++i) {
-    const T v2 = hdt * E.lam[i];
-    const T v3l = vl - v2, v4l = T(kFourThirds) * (vl * vl - v2 * v2);
-    const T v3r = vr - v2, v4r = T(kFourThirds) * (vr * vr - v2 * v2);
-    const T fl = E.lam[i] <= T(0) ? v3l * (p1[i] + p2[i]) + v4l * p2[i] : T(0);
-    const T fr = E.lam[i] >= T(0) ? v3r * (p1[i] - p2[i]) + v4r * p2[i] : T(0);
-    eig_add_col(E, i, fl, la);
-    eig_add_col(E, i, fr, ra);
-  }
-#else
 #pragma unroll
   for (int i = 0; i < 7; ++i) {
     if (E.lam[i] <= T(0)) {
