@@ -68,6 +68,24 @@ PMHD_FI float npmin(float a, float b) { return fminf(a, b); }
 template <class T> PMHD_FI T sq(T x) { return x * x; }
 PMHD_FI double sqrt_(double x) { return sqrt(x); }
 PMHD_FI float sqrt_(float x) { return sqrtf(x); }
+// reciprocal square root: the FP32 fast unit uses the hardware estimate where one
+// root and one quotient of the same value are both needed
+template <class T> PMHD_FI T rsqrt_(T x) { return T(1) / sqrt_(x); }
+PMHD_FI float rsqrt_(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / std::sqrt(x);
+#endif
+}
+// true in the FP32 fast unit for float arithmetic (reassociated forms allowed)
+template <class T> PMHD_FI constexpr bool fast_f32() {
+#ifdef PMHD_FAST_ORDER
+  return sizeof(T) == 4;
+#else
+  return false;
+#endif
+}
 PMHD_FI double fabs_(double x) { return fabs(x); }
 PMHD_FI float fabs_(float x) { return fabsf(x); }
 template <class T> PMHD_FI T npabs(T x) { return fabs_(x); }
@@ -182,10 +200,19 @@ PMHD_FI void eigen_prim(const T w[7], T bn, T gamma, PrimEig<T>& E) {
   using N = island_t<T>;
   const T rho = w[0], vn = w[1], bt1 = w[4], bt2 = w[5], p = w[6];
   T btsq = bt1 * bt1 + bt2 * bt2;
-  T bt = sqrt_(btsq);
   T gp = gamma * p;
-  T inv = T(1) / rho;
-  T sqd = sqrt_(rho);
+  // FP32 fast unit: one reciprocal root each gives sqrt(rho), 1/rho, 1/sqrt(rho)
+  // and 1/|B_t|
+  T inv, sqd, isqd = T(0), ibt = T(0);
+  if constexpr (fast_f32<T>()) {
+    isqd = rsqrt_(rho);
+    sqd = rho * isqd;
+    inv = isqd * isqd;
+    ibt = rsqrt_(btsq);
+  } else {
+    inv = T(1) / rho;
+    sqd = sqrt_(rho);
+  }
   N n_cs2, n_ca2, n_cf2, n_csl2, af, as;
   {
     // wave speeds and normalisation in N (double) precision
@@ -234,8 +261,15 @@ PMHD_FI void eigen_prim(const T w[7], T bn, T gamma, PrimEig<T>& E) {
   const T cf = sqrt_(T(n_cf2));
   const T csl = sqrt_(T(n_csl2));
   const T Af_ = T(af), As_ = T(as);
-  T b1 = bt == T(0) ? T(1) : (bt > T(0) ? bt1 / bt : T(0));
-  T b2 = bt > T(0) ? bt2 / bt : T(0);
+  T b1, b2;
+  if constexpr (fast_f32<T>()) {
+    b1 = btsq == T(0) ? T(1) : (btsq > T(0) ? bt1 * ibt : T(0));
+    b2 = btsq > T(0) ? bt2 * ibt : T(0);
+  } else {
+    T bt = sqrt_(btsq);
+    b1 = bt == T(0) ? T(1) : (bt > T(0) ? bt1 / bt : T(0));
+    b2 = bt > T(0) ? bt2 / bt : T(0);
+  }
   T sg = npsign(bn);
   sg = sg == T(0) ? T(1) : sg;
   T qf = cf * Af_ * sg;
@@ -262,7 +296,14 @@ PMHD_FI void eigen_prim(const T w[7], T bn, T gamma, PrimEig<T>& E) {
   E.r50 = a_s * b2;
   E.r51 = b1 * sg * sqd;
   E.r52 = -a_f * b2;
-  T nf = T(0.5) / cs2;
+  T ics2 = T(0);
+  T nf;
+  if constexpr (fast_f32<T>()) {
+    ics2 = T(1) / cs2;
+    nf = T(0.5) * ics2;
+  } else {
+    nf = T(0.5) / cs2;
+  }
   T qf2 = nf * qf;
   T qs2 = nf * qs;
   T afp = nf * a_f * inv;
@@ -275,15 +316,21 @@ PMHD_FI void eigen_prim(const T w[7], T bn, T gamma, PrimEig<T>& E) {
   E.l05 = asp * b2;
   E.l12 = T(-0.5) * b2;
   E.l13 = T(0.5) * b1;
-  E.l14 = T(-0.5) * b2 * sg / sqd;
-  E.l15 = T(0.5) * b1 * sg / sqd;
+  if constexpr (fast_f32<T>()) {
+    E.l14 = T(-0.5) * b2 * sg * isqd;
+    E.l15 = T(0.5) * b1 * sg * isqd;
+  } else {
+    E.l14 = T(-0.5) * b2 * sg / sqd;
+    E.l15 = T(0.5) * b1 * sg / sqd;
+  }
   E.l21 = T(-1) * nf * csl * As_;
   E.l22 = T(-1) * qf2 * b1;
   E.l23 = T(-1) * qf2 * b2;
   E.l26 = nf * As_ * inv;
   E.l24 = T(-1) * afp * b1;
   E.l25 = T(-1) * afp * b2;
-  E.l36 = T(-1) / cs2;
+  if constexpr (fast_f32<T>()) E.l36 = -ics2;
+  else E.l36 = T(-1) / cs2;
 }
 
 // L . d with the reference's summation order (rho, v_n, v_t1, v_t2, p, B_t1, B_t2)
@@ -536,7 +583,18 @@ PMHD_HD inline void roe_flux(const T wl[7], const T wr[7], const T ul[7], const 
                              T g, bool efix, T eps, T F[7]) {
   using N = island_t<T>;
   const T rl = wl[0], rr = wr[0];
-  T sl = sqrt_(rl), sr = sqrt_(rr);
+  // FP32 fast unit: reciprocal roots of the two densities, of the Cargo-Gallice
+  // factor and of B_t^2 replace the roots and the quotients by them
+  T sl, sr, isl = T(0), isr = T(0);
+  if constexpr (fast_f32<T>()) {
+    isl = rsqrt_(rl);
+    isr = rsqrt_(rr);
+    sl = rl * isl;
+    sr = rr * isr;
+  } else {
+    sl = sqrt_(rl);
+    sr = sqrt_(rr);
+  }
   T isum = T(1) / (sl + sr);
   T rd = sl * sr;
   T vn = (sl * wl[1] + sr * wr[1]) * isum;
@@ -548,16 +606,31 @@ PMHD_HD inline void roe_flux(const T wl[7], const T wr[7], const T ul[7], const 
   T hbl = T(0.5) * (bn * bn + wl[4] * wl[4] + wl[5] * wl[5]);
   T hbr = T(0.5) * (bn * bn + wr[4] * wr[4] + wr[5] * wr[5]);
   T pl = wl[6], pr = wr[6];
-  T H = ((ul[6] + pl + hbl) / sl + (ur[6] + pr + hbr) / sr) * isum;
-  T ird = T(1) / rd;
+  T H, ird, btc, rhoc, bt, bts, ibt = T(0), ifac = T(0);
   T d4 = wl[4] - wr[4], d5 = wl[5] - wr[5];
-  T ssd = sl + sr;
-  T btc = T(0.5) * (d4 * d4 + d5 * d5) / (ssd * ssd);
-  T rhoc = T(0.5) * (rl + rr) / rd;
   T btsq = bt1 * bt1 + bt2 * bt2;
-  T bt = sqrt_(btsq);
-  T btsqs = ((g - T(1)) - (g - T(2)) * rhoc) * btsq;
-  T bts = sqrt_(btsqs);
+  T btsqs;
+  if constexpr (fast_f32<T>()) {
+    H = ((ul[6] + pl + hbl) * isl + (ur[6] + pr + hbr) * isr) * isum;
+    ird = isl * isr;
+    btc = T(0.5) * (d4 * d4 + d5 * d5) * (isum * isum);
+    rhoc = T(0.5) * (rl + rr) * ird;
+    const T fac = (g - T(1)) - (g - T(2)) * rhoc;
+    ifac = rsqrt_(fac);
+    ibt = rsqrt_(btsq);
+    bt = btsq > T(0) ? btsq * ibt : T(0);
+    btsqs = fac * btsq;
+    bts = (fac * ifac) * bt;
+  } else {
+    H = ((ul[6] + pl + hbl) / sl + (ur[6] + pr + hbr) / sr) * isum;
+    ird = T(1) / rd;
+    T ssd = sl + sr;
+    btc = T(0.5) * (d4 * d4 + d5 * d5) / (ssd * ssd);
+    rhoc = T(0.5) * (rl + rr) / rd;
+    bt = sqrt_(btsq);
+    btsqs = ((g - T(1)) - (g - T(2)) * rhoc) * btsq;
+    bts = sqrt_(btsqs);
+  }
   T hmb, cs2, cf, ca, csl, cs;
   N af, as;
   {
@@ -629,16 +702,30 @@ PMHD_HD inline void roe_flux(const T wl[7], const T wr[7], const T ul[7], const 
   T Af_ = T(af), As_ = T(as);
   T sg = npsign(bn);
   sg = sg == T(0) ? T(1) : sg;
-  T srd = sqrt_(rd);
-  T isrd = T(1) / srd;
+  T srd, isrd;
+  if constexpr (fast_f32<T>()) {
+    isrd = rsqrt_(rd);
+    srd = rd * isrd;
+  } else {
+    srd = sqrt_(rd);
+    isrd = T(1) / srd;
+  }
   T qf = cf * Af_ * sg;
   T qs = csl * As_ * sg;
   T afp = cs * Af_ * isrd;
   T asp = cs * As_ * isrd;
-  T b1 = bt == T(0) ? T(1) : (bt > T(0) ? bt1 / bt : T(0));
-  T b2 = bt > T(0) ? bt2 / bt : T(0);
-  T b1s = b1 / sqrt_((g - T(1)) - (g - T(2)) * rhoc);
-  T b2s = b2 / sqrt_((g - T(1)) - (g - T(2)) * rhoc);
+  T b1, b2, b1s, b2s;
+  if constexpr (fast_f32<T>()) {
+    b1 = btsq == T(0) ? T(1) : (btsq > T(0) ? bt1 * ibt : T(0));
+    b2 = btsq > T(0) ? bt2 * ibt : T(0);
+    b1s = b1 * ifac;
+    b2s = b2 * ifac;
+  } else {
+    b1 = bt == T(0) ? T(1) : (bt > T(0) ? bt1 / bt : T(0));
+    b2 = bt > T(0) ? bt2 / bt : T(0);
+    b1s = b1 / sqrt_((g - T(1)) - (g - T(2)) * rhoc);
+    b2s = b2 / sqrt_((g - T(1)) - (g - T(2)) * rhoc);
+  }
   T sb = vt1 * b1s + vt2 * b2s;
   T bss = b1s * b1s + b2s * b2s;
   T afb = afp * bts * bss;
@@ -671,8 +758,15 @@ PMHD_HD inline void roe_flux(const T wl[7], const T wr[7], const T ul[7], const 
   nf = nf * (g - T(1));
   T afn = Af_ * nf;
   T asn = As_ * nf;
-  T b1c = b1s / bss;
-  T b2c = b2s / bss;
+  T b1c, b2c;
+  if constexpr (fast_f32<T>()) {
+    const T ibss = T(1) / bss;
+    b1c = b1s * ibss;
+    b2c = b2s * ibss;
+  } else {
+    b1c = b1s / bss;
+    b2c = b2s / bss;
+  }
   T sbs = vt1 * b1c + vt2 * b2c;
   nf = nf * T(2);
   T L00 = afn * (vsq - hmb) + cff * (cf + vn) - qs * sbs - asb;
