@@ -85,10 +85,11 @@ int pmhd_advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int64_
  * direction reconstructs the cells (PLM slopes, or the PPM slopes of a separate
  * launch), traces the edge states and forms the predictor's interface states and
  * Roe fluxes for s consecutive faces per thread; 0 selects separate slope, edge
- * and predictor launches; seg_z = -1 gives each line along z one workgroup whose
- * threads reconstruct one cell each and pass edge states through LDS.  Results
- * are identical for every choice (bit for bit in FP64).  The default is picked
- * from the grid size and the device.  Lengths must lie in [0, 4096]. */
+ * and predictor launches; -1 along the contiguous axis (z in 3D, x in 2D) gives
+ * each line one workgroup whose threads reconstruct one cell each and pass edge
+ * states through LDS.  Results are identical for every choice (bit for bit in
+ * FP64).  The default is picked from the grid size and the device.  Lengths must
+ * lie in [0, 4096], or be -1 along the contiguous axis (PMHD_EINVAL otherwise). */
 int pmhd_set_sweep(pmhd_handle* h, int32_t seg_x, int32_t seg_y, int32_t seg_z);
 
 /* Wait for all queued work of the handle. */

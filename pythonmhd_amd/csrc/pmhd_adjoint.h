@@ -227,7 +227,7 @@ struct TapeExec {
   bool fused = false;   // a fused launch was met (it has no per-element adjoint)
   template <class Ph>
   void run(const Ctx<T>& c) {
-    if constexpr (Ph::kind == EXT_SEG || Ph::kind == EXT_LINE) {
+    if constexpr (Ph::kind == EXT_SEG || Ph::kind == EXT_LINE || Ph::kind == EXT_ENDS) {
       // the adjoint's contexts keep seg = 0: separate slope, edge and predictor
       // launches, whose elements are small enough to differentiate one by one
       (void)c;

@@ -187,15 +187,17 @@ void init_ctx(Ctx<T>& c, const pmhd_grid& g, const pmhd_params& p) {
 }
 
 // Faces per thread of the fused predictor sweep along direction d (Ctx::seg).  In
-// FP32 along x and y: the largest power of two, at most 16, that still leaves about
-// 2048 threads per compute unit for nfaces faces, and at least 2 (a run of one face
-// reconstructs two cells); long runs amortise the cell reconstructed before each
-// run.  Along z, the contiguous axis: one workgroup per line (-1) for lines of at
-// least 64 cells, else runs of 2.  In FP64 the fused kernels need more registers
-// than they save memory traffic, so the separate launches are used (0).
-inline int sweep_segment(int d, int64_t nfaces, int64_t n_along, int cus, bool fp64) {
+// FP32 across the contiguous axis: the largest power of two, at most 16, that still
+// leaves about 2048 threads per compute unit for nfaces faces, and at least 2 (a run
+// of one face reconstructs two cells); long runs amortise the cell reconstructed
+// before each run.  Along the contiguous axis (z in 3D, x in 2D), where the threads
+// of a run launch would read addresses a run apart: one workgroup per line (-1) for
+// lines of at least 64 cells, else runs of 2.  In FP64 the fused kernels need more
+// registers than they save memory traffic, so the separate launches are used (0).
+inline bool line_axis(int ndim, int d) { return (ndim == 3 && d == 2) || (ndim == 2 && d == 0); }
+inline int sweep_segment(int ndim, int d, int64_t nfaces, int64_t n_along, int cus, bool fp64) {
   if (fp64) return 0;
-  if (d == 2) return n_along >= 64 ? -1 : 2;
+  if (line_axis(ndim, d)) return n_along >= 64 ? -1 : 2;
   const int64_t target = int64_t(cus > 0 ? cus : 256) * 2048;
   int s = 2;
   while (s < 16 && nfaces / (2 * s) >= target) s *= 2;
@@ -250,7 +252,7 @@ void alloc_grid(Ctx<T>& c, StateBufs<T>& s, int order, A&& alloc) {
     c.E[d] = alloc(mhd && (nd == 3 || (nd == 2 && d == 2)) ? c.nedge() : 1);
   }
   c.Ec = alloc(mhd ? (nd == 3 ? 3 * N : N) : 1);
-  c.zend = alloc(mhd && nd == 3 ? 14 * c.ny * c.nx : 1);
+  c.zend = alloc(mhd && nd >= 2 ? 14 * (c.ncell() / c.n_along(nd == 3 ? 2 : 0)) : 1);
 }
 
 // point the context at buffer set `cur` (the update writes 1 - cur)
@@ -380,7 +382,7 @@ struct Handle : public pmhd_handle {
     }
     bind_bufs(c, sb, cur);
     for (int d = 0; d < g.ndim; ++d)
-      c.seg[d] = sweep_segment(d, c.nface(d), c.n_along(d), be.compute_units(), sizeof(T) == sizeof(double));
+      c.seg[d] = sweep_segment(g.ndim, d, c.nface(d), c.n_along(d), be.compute_units(), sizeof(T) == sizeof(double));
     c.ec_next = p.physics == 0 && g.ndim >= 2;
     const char* e = be.error();
     if (e) return fail(PMHD_EHIP, e);
@@ -388,9 +390,12 @@ struct Handle : public pmhd_handle {
   }
 
   int set_sweep(const int32_t seg[3]) override {
-    for (int d = 0; d < 3; ++d)
-      if (seg[d] < (d == 2 ? -1 : 0) || seg[d] > 4096)
-        return fail(PMHD_EINVAL, "sweep run length must be in [0, 4096] (or -1 along z)");
+    for (int d = 0; d < 3; ++d) {
+      const int lo = d >= g.ndim || line_axis(g.ndim, d) ? -1 : 0;
+      if (seg[d] < lo || seg[d] > 4096)
+        return fail(PMHD_EINVAL, "sweep run length must be in [0, 4096], or -1 (one workgroup per line) along "
+                                 "the contiguous axis (z in 3D, x in 2D)");
+    }
     for (int d = 0; d < 3; ++d) c.seg[d] = d < g.ndim ? seg[d] : 0;
     return PMHD_OK;
   }
@@ -546,17 +551,17 @@ struct BatchHandle : public pmhd_batch {
     } catch (const std::bad_alloc&) {
       return fail(PMHD_ENOMEM, "device allocation failed");
     }
-    // fused predictor sweeps with one run length per direction for all boxes (a
-    // batch tile lies in one y row, so along x in 2D each thread takes one face)
+    // fused predictor sweeps with one plan per direction for all boxes: line
+    // launches along the contiguous axis (one workgroup per line of a box), runs
+    // across it
     int seg[3] = {0, 0, 0};
     {
       int64_t nf[3] = {0, 0, 0};
       for (int b = 0; b < n; ++b)
         for (int d = 0; d < grids[0].ndim; ++d) nf[d] += hctx[size_t(b)].nface(d);
       for (int d = 0; d < grids[0].ndim; ++d) {
-        seg[d] = sweep_segment(d, nf[d], int64_t(0), be.compute_units(), sizeof(T) == sizeof(double));
-        if (seg[d] < 0) seg[d] = 2;   // no line launches in batches
-        if (seg[d] > 0 && grids[0].ndim == 2 && d == 0) seg[d] = 1;
+        seg[d] = sweep_segment(grids[0].ndim, d, nf[d], int64_t(0), be.compute_units(), sizeof(T) == sizeof(double));
+        if (seg[d] > 0 && line_axis(grids[0].ndim, d)) seg[d] = -1;
       }
     }
     for (int b = 0; b < n; ++b) {
@@ -586,8 +591,13 @@ struct BatchHandle : public pmhd_batch {
       be.h2d(lay.dprefix + size_t(id) * size_t(nbox + 1), lay.hprefix.data() + size_t(id) * size_t(nbox + 1),
              sizeof(int) * size_t(nbox + 1));
     };
-    for (int id = 0; id < K_COUNT; ++id)
-      if (!lay.apply_tile(id, default_tile(lay))) return fail(PMHD_EINVAL, "batch too large: more than 2^31 tiles");
+    for (int id = 0; id < K_COUNT; ++id) {
+      int kind, dir;
+      phase_shape(id, kind, dir);
+      // line launches: one tile per line
+      const TileCfg t = kind == EXT_LINE ? TileCfg{1, 1, 0} : default_tile(lay);
+      if (!lay.apply_tile(id, t)) return fail(PMHD_EINVAL, "batch too large: more than 2^31 tiles");
+    }
     push_ctx();
     const char* e = be.error();
     if (e) return fail(PMHD_EHIP, e);

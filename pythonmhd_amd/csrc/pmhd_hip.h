@@ -49,38 +49,36 @@ __global__ void __launch_bounds__(256) k_phase_flat(const Ctx<T> c, unsigned n1,
   Ph()(c, I3{int64_t(r / n1), int64_t(r % n1), int64_t(k)});
 }
 
-// Fused predictor sweep along z, one workgroup per line (j, i) = (blockIdx.y,
-// blockIdx.x), in chunks of blockDim.x cells: each thread reconstructs one cell
-// and keeps its left edge state; the right edge states go through LDS to the
+// Fused predictor sweep over one line along the contiguous axis D (z in 3D, x in
+// 2D) by one workgroup, in chunks of blockDim.x cells: each thread reconstructs one
+// cell and keeps its left edge state; the right edge states go through LDS to the
 // thread of the next face.  Faces 1..n-1 only: the end faces 0 and n need the edge
 // states of both ends of the line, which are kept (store_line_end) for the
-// SweepEndsZ launch, so every thread of every wave does the same work.  The face
-// states are those of k_sweep_run, so the results are identical.
-template <class T>
-__global__ void __launch_bounds__(256) k_sweep_line_z(const Ctx<T> c) {
-  __shared__ T sra[7][256];
-  __shared__ T prev[7];   // right edge state of the last cell of the previous chunk
+// SweepEnds launch, so every thread of every wave does the same work.  The face
+// states are those of k_sweep_run, so the results are identical.  q: the line's
+// first cell.  Every thread of the workgroup calls it (it synchronises).
+template <int D, class T>
+__device__ inline void sweep_line(const Ctx<T>& c, const I3 q, T (&sra)[7][256], T (&prev)[7]) {
   const int t = int(threadIdx.x), B = int(blockDim.x);
-  const int64_t n = c.nz;
-  const int bc = c.bc[2];
+  const int64_t n = c.n_along(D);
+  const int bc = c.bc[D];
   const bool from_s = c.order == 2;
-  const I3 q{int64_t(blockIdx.y), int64_t(blockIdx.x), 0};
   for (int64_t k0 = 0; k0 < n; k0 += B) {
     const int64_t k = k0 + t;
     T la[7], ra[7];
     if (k < n) {
-      cell_edge_states(c, 2, bc, with(q, 2, k), from_s, la, ra);
+      cell_edge_states(c, D, bc, with(q, D, k), from_s, la, ra);
 #pragma unroll
       for (int v = 0; v < 7; ++v) sra[v][t] = ra[v];
-      if (k == 0) store_line_end(c, q, 0, la);
-      if (k == n - 1) store_line_end(c, q, 1, ra);
+      if (k == 0) store_line_end(c, D, q, 0, la);
+      if (k == n - 1) store_line_end(c, D, q, 1, ra);
     }
     __syncthreads();
     if (k > 0 && k < n) {
       T wl[7];
 #pragma unroll
       for (int v = 0; v < 7; ++v) wl[v] = t == 0 ? prev[v] : sra[v][t - 1];
-      predict_face(c, 2, with(q, 2, k), wl, la);
+      predict_face(c, D, with(q, D, k), wl, la);
     }
     __syncthreads();
     if (k0 + B < n) {
@@ -91,6 +89,14 @@ __global__ void __launch_bounds__(256) k_sweep_line_z(const Ctx<T> c) {
       __syncthreads();
     }
   }
+}
+
+// one workgroup per line; line blockIdx.x
+template <int D, class T>
+__global__ void __launch_bounds__(256) k_sweep_line(const Ctx<T> c) {
+  __shared__ T sra[7][256];
+  __shared__ T prev[7];   // right edge state of the last cell of the previous chunk
+  sweep_line<D>(c, line_start(c, D, int64_t(blockIdx.x)), sra, prev);
 }
 
 __device__ inline unsigned long long umax64(unsigned long long a, unsigned long long b) {
@@ -228,6 +234,21 @@ __global__ void __launch_bounds__(256) k_batch(const Ctx<T>* __restrict__ ctxs, 
     const Ctx<T>& c = ctxs[r.box];
     if (!c.active) continue;
     if (r.in) Ph()(c, I3{r.j, r.i, r.k});
+  }
+}
+
+// line sweeps of a batch: one workgroup per line of one box (prefix: lines per box)
+template <int D, class T>
+__global__ void __launch_bounds__(256) k_batch_line(const Ctx<T>* __restrict__ ctxs, const int* __restrict__ prefix,
+                                                    int nbox) {
+  __shared__ T sra[7][256];
+  __shared__ T prev[7];
+  const int total = prefix[nbox];
+  for (int vb = int(blockIdx.x); vb < total; vb += int(gridDim.x)) {
+    const TileRef r = locate_tile(ctxs, prefix, nbox, vb, EXT_LINE, D, 1, 1, 0);
+    const Ctx<T>& c = ctxs[r.box];
+    if (!c.active) continue;
+    sweep_line<D>(c, I3{r.j, r.i, r.k}, sra, prev);
   }
 }
 
@@ -415,15 +436,16 @@ struct HipExec {
     if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return;
     const int64_t total = n[0] * n[1] * n[2];
     // profiled work: faces for the launches over runs or lines of faces
-    const int64_t work = Ph::kind == EXT_SEG || Ph::id == K_SWEEP_LINE_Z ? c.nface(Ph::dir) : total;
-    if constexpr (Ph::id == K_SWEEP_LINE_Z) {
-      // one workgroup per line along z: the smallest of 64, 128, 256 threads that
-      // covers the line, else 256 in chunks
-      static_assert(Ph::dir == 2, "line launches run along z");
+    const int64_t work = Ph::kind == EXT_SEG || Ph::kind == EXT_LINE ? c.nface(Ph::dir) : total;
+    if constexpr (Ph::kind == EXT_LINE) {
+      // one workgroup per line: the smallest of 64, 128, 256 threads that covers
+      // the line, else 256 in chunks
       unsigned b = 64;
-      while (b < 256 && int64_t(b) < c.nz) b <<= 1;
+      while (b < 256 && int64_t(b) < c.n_along(Ph::dir)) b <<= 1;
+      // (a grid has below 2^29 cells, so the line count fits the grid's x extent)
+      const int64_t lines = line_count(c, Ph::dir);
       timed(Ph::id, work, [&] {
-        hipLaunchKernelGGL((k_sweep_line_z<T>), dim3(unsigned(n[1]), unsigned(n[0])), dim3(b), 0, s, c);
+        hipLaunchKernelGGL((k_sweep_line<Ph::dir, T>), dim3(unsigned(lines)), dim3(b), 0, s, c);
       });
       note(hipGetLastError());
       return;
@@ -544,6 +566,20 @@ struct HipBatchExec {
 
   template <class Ph, class T>
   void run(const Ctx<T>&) {
+    if constexpr (Ph::kind == EXT_LINE) {
+      // one workgroup of 64 threads per line (lines of uneven boxes waste at most
+      // the tail of their last chunk); the layout's tile is one line
+      const Ctx<T>* dctx = static_cast<const Ctx<T>*>(lay->dctx);
+      const int* prefix = lay->dprefix + Ph::id * (lay->nbox + 1);
+      const int total = lay->total[Ph::id];
+      if (total <= 0) return;
+      ex->timed(Ph::id, lay->elems[Ph::id], [&] {
+        hipLaunchKernelGGL((k_batch_line<Ph::dir, T>), dim3(unsigned(total)), dim3(64u), 0, ex->s, dctx, prefix,
+                           lay->nbox);
+      });
+      ex->note(hipGetLastError());
+      return;
+    }
     if (tuning) tune<Ph, T>();
     int bk, bi, grid;
     launch_cfg(Ph::id, bk, bi, grid);

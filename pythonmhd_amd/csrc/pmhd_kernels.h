@@ -25,7 +25,8 @@ enum ExtKind { EXT_CELL = 0, EXT_FACE = 1, EXT_SLOPE = 2, EXT_EDGE = 3, EXT_SEG 
 enum KernelId {
   K_SLOPES = 0, K_EDGES = 3, K_COPY = 6, K_PREDICT = 9, K_CELL_E = 12, K_EMF = 13, K_EMF2 = 14,
   K_CT_HALF = 17, K_HALF_E = 20, K_CORRECT = 21, K_CT_FULL = 24, K_UPDATE = 27, K_CFL = 28,
-  K_DIAG = 29, K_SWEEP = 30, K_SWEEP_LINE_Z = 33, K_SWEEP_ENDS_Z = 34, K_ADJOINT = 35, K_COUNT = 36
+  K_DIAG = 29, K_SWEEP = 30, K_SWEEP_LINE_Z = 33, K_SWEEP_ENDS_Z = 34, K_ADJOINT = 35, K_SWEEP_LINE_X = 36,
+  K_SWEEP_ENDS_X = 37, K_COUNT = 38
 };
 inline const char* kernel_name(int id) {
   static const char* names[K_COUNT] = {
@@ -33,7 +34,7 @@ inline const char* kernel_name(int id) {
       "predict_x", "predict_y", "predict_z", "cell_e", "emf", "emf_corr", "unused_15", "unused_16",
       "ct_half_x", "ct_half_y", "ct_half_z", "half_e", "ctu_correct_x", "ctu_correct_y", "ctu_correct_z",
       "ct_full_x", "ct_full_y", "ct_full_z", "update", "cfl", "diag", "sweep_x", "sweep_y", "sweep_z",
-      "sweep_z", "sweep_ends_z", "adjoint"};
+      "sweep_z", "sweep_ends_z", "adjoint", "sweep_x", "sweep_ends_x"};
   return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
 // element kind (ExtKind) and direction of each launch id
@@ -53,6 +54,8 @@ inline void phase_shape(int id, int& kind, int& dir) {
   else if (id >= K_SWEEP && id < K_SWEEP + 3) { kind = EXT_SEG; dir = id - K_SWEEP; }
   else if (id == K_SWEEP_LINE_Z) { kind = EXT_LINE; dir = 2; }
   else if (id == K_SWEEP_ENDS_Z) { kind = EXT_ENDS; dir = 2; }
+  else if (id == K_SWEEP_LINE_X) { kind = EXT_LINE; dir = 0; }
+  else if (id == K_SWEEP_ENDS_X) { kind = EXT_ENDS; dir = 0; }
 }
 
 // pointer types of a context's arrays (the adjoint's dual contexts use accessors)
@@ -128,8 +131,9 @@ struct Ctx {
   Ptr<T> F[3];           // predictor fluxes
   Ptr<T> Fc[3];          // corrector fluxes
   Ptr<T> Ec;
-  Ptr<T> zend;       // per line along z: left edge state of cell 0 (7) and right edge state
-                     // of cell nz-1 (7), planes [v][j * nx + i], for the end faces
+  Ptr<T> zend;       // per line along the contiguous axis (z in 3D, x in 2D): left edge state
+                     // of the first cell (7) and right edge state of the last cell (7),
+                     // planes [v][line], for the end faces
   Ptr<T> E[3];
   // element counts
   PMHD_FI int64_t ncell() const { return ny * nx * nz; }
@@ -589,46 +593,62 @@ PMHD_FI void k_sweep_run(const Ctx<T>& c, int d, int bc, I3 q, int64_t f0, int64
   }
 }
 
-// the line (j, i) along z: keep the left edge state of cell 0 and the right edge
-// state of cell nz-1 for the end faces (k_sweep_end)
+// Lines along the contiguous axis d (z in 3D, x in 2D): a line is numbered by its
+// other two coordinates, slowest first.
 template <class T>
-PMHD_FI void store_line_end(const Ctx<T>& c, I3 q, int side, const T s[7]) {
-  const int64_t L = c.ny * c.nx;
-  const int64_t l = q.j * c.nx + q.i;
+PMHD_FI int64_t line_count(const Ctx<T>& c, int d) {
+  return c.ncell() / c.n_along(d);
+}
+template <class T>
+PMHD_FI int64_t line_of(const Ctx<T>& c, int d, I3 q) {
+  return d == 2 ? q.j * c.nx + q.i : (d == 0 ? q.j * c.nz + q.k : q.i * c.nz + q.k);
+}
+template <class T>
+PMHD_FI I3 line_start(const Ctx<T>& c, int d, int64_t l) {
+  if (d == 2) return I3{l / c.nx, l % c.nx, 0};
+  if (d == 0) return I3{l / c.nz, 0, l % c.nz};
+  return I3{0, l / c.nz, l % c.nz};
+}
+// keep the left edge state of the first cell and the right edge state of the last
+// cell of the line through q along d for the end faces (k_sweep_end)
+template <class T>
+PMHD_FI void store_line_end(const Ctx<T>& c, int d, I3 q, int side, const T s[7]) {
+  const int64_t L = line_count(c, d);
+  const int64_t l = line_of(c, d, q);
 #pragma unroll
   for (int v = 0; v < 7; ++v) st(c.zend + (7 * side + v) * L, l, s[v]);
 }
 template <class T>
-PMHD_FI void k_line_ends(const Ctx<T>& c, int bc, I3 q) {
+PMHD_FI void k_line_ends(const Ctx<T>& c, int d, int bc, I3 q) {
   const bool from_s = c.order == 2;
   T la[7], ra[7];
-  cell_edge_states(c, 2, bc, with(q, 2, 0), from_s, la, ra);
-  store_line_end(c, q, 0, la);
-  cell_edge_states(c, 2, bc, with(q, 2, c.nz - 1), from_s, la, ra);
-  store_line_end(c, q, 1, ra);
+  cell_edge_states(c, d, bc, with(q, d, 0), from_s, la, ra);
+  store_line_end(c, d, q, 0, la);
+  cell_edge_states(c, d, bc, with(q, d, c.n_along(d) - 1), from_s, la, ra);
+  store_line_end(c, d, q, 1, ra);
 }
-// end face 0 (q.k == 0) or nz (q.k == 1) of the line (j, i) along z from the stored
-// edge states: face 0 takes the left edge of cell 0 on both sides (outflow) or the
-// right edge of cell nz-1 and the left edge of cell 0 (periodic); face nz takes the
-// right edge of cell nz-1 on both sides (outflow) or the right edge of cell nz-1
-// and the left edge of cell 0 (periodic)
+// end face 0 (co(q, d) == 0) or n (co(q, d) == 1) of the line through q along d
+// from the stored edge states: face 0 takes the left edge of cell 0 on both sides
+// (outflow) or the right edge of cell n-1 and the left edge of cell 0 (periodic);
+// face n takes the right edge of cell n-1 on both sides (outflow) or the right edge
+// of cell n-1 and the left edge of cell 0 (periodic)
 template <class T>
-PMHD_FI void k_sweep_end(const Ctx<T>& c, int bc, I3 q) {
-  const int64_t L = c.ny * c.nx;
-  const int64_t l = q.j * c.nx + q.i;
+PMHD_FI void k_sweep_end(const Ctx<T>& c, int d, int bc, I3 q) {
+  const int64_t L = line_count(c, d);
+  const int64_t l = line_of(c, d, q);
   T l0[7], rn[7], wl[7], wr[7];
 #pragma unroll
   for (int v = 0; v < 7; ++v) {
     l0[v] = ld(c.zend + v * L, l);
     rn[v] = ld(c.zend + (7 + v) * L, l);
   }
-  const bool face0 = q.k == 0;
+  const bool face0 = co(q, d) == 0;
 #pragma unroll
   for (int v = 0; v < 7; ++v) {
     wl[v] = face0 ? (bc == OUTFLOW ? l0[v] : rn[v]) : rn[v];
     wr[v] = face0 ? l0[v] : (bc == OUTFLOW ? rn[v] : l0[v]);
   }
-  predict_face(c, 2, with(q, 2, face0 ? 0 : c.nz), wl, wr);
+  predict_face(c, d, with(q, d, face0 ? 0 : c.n_along(d)), wl, wr);
 }
 
 template <class T>

@@ -56,23 +56,25 @@ struct Sweep {  // fused slopes (PLM) + edges + predictor over runs of faces
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 q) const { k_sweep(c, D, c.bc[D], q); }
 };
-// the fused sweep along z over whole lines, faces 1..n-1, keeping the edge states
-// of the two end cells of each line: the HIP executor gives each line one
-// workgroup whose threads reconstruct one cell each and exchange edge states
-// through LDS; other executors walk the line as one run of faces (same results)
-struct SweepLineZ {
-  PMHD_PHASE_TRAITS(K_SWEEP_LINE_Z, EXT_LINE, 2)
+// the fused sweep over whole lines along the contiguous axis D (z in 3D, x in 2D),
+// faces 1..n-1, keeping the edge states of both end cells (one workgroup per line
+// on the GPU; here the reference order of one thread per line)
+template <int D>
+struct SweepLine {
+  static_assert(D == 0 || D == 2, "line sweeps run along the contiguous axis");
+  PMHD_PHASE_TRAITS(D == 2 ? K_SWEEP_LINE_Z : K_SWEEP_LINE_X, EXT_LINE, D)
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 q) const {
-    k_sweep_run(c, 2, c.bc[2], q, 1, c.nz);
-    k_line_ends(c, c.bc[2], q);
+    k_sweep_run(c, D, c.bc[D], q, 1, c.n_along(D));
+    k_line_ends(c, D, c.bc[D], q);
   }
 };
-// the two end faces (0 and n) of every line along z from the kept end states
-struct SweepEndsZ {
-  PMHD_PHASE_TRAITS(K_SWEEP_ENDS_Z, EXT_ENDS, 2)
+// the two end faces (0 and n) of every line along D from the kept end states
+template <int D>
+struct SweepEnds {
+  PMHD_PHASE_TRAITS(D == 2 ? K_SWEEP_ENDS_Z : K_SWEEP_ENDS_X, EXT_ENDS, D)
   template <class T>
-  PMHD_FI void operator()(const Ctx<T>& c, I3 q) const { k_sweep_end(c, c.bc[2], q); }
+  PMHD_FI void operator()(const Ctx<T>& c, I3 q) const { k_sweep_end(c, D, c.bc[D], q); }
 };
 struct CellE {
   PMHD_PHASE_TRAITS(K_CELL_E, EXT_CELL, 0)
@@ -156,14 +158,19 @@ void diagnostics(X& ex, const Ctx<T>& c, double sums[9], double maxes[2]) {
 // reconstruction and the predictor are one launch (PPM keeps its slope launch)
 template <int D, class T, class X>
 void predictor_sweep(X& ex, const Ctx<T>& c) {
-  if (c.order > 0 && (c.seg[D] > 0 || (D == 2 && c.seg[D] < 0))) {
+  // line launches exist along the contiguous axis only (z in 3D, x in 2D)
+  constexpr bool can_line = D == 2 || D == 0;
+  const bool line = can_line && c.seg[D] < 0 && (D == 2 ? c.ndim == 3 : c.ndim == 2);
+  if (c.order > 0 && (c.seg[D] > 0 || line)) {
     if (c.order == 2) ex.template run<Slopes<D>>(c);
-    if (D == 2 && c.seg[D] < 0) {
-      ex.template run<SweepLineZ>(c);
-      ex.template run<SweepEndsZ>(c);
-    } else {
-      ex.template run<Sweep<D>>(c);
+    if constexpr (can_line) {
+      if (line) {
+        ex.template run<SweepLine<D>>(c);
+        ex.template run<SweepEnds<D>>(c);
+        return;
+      }
     }
+    ex.template run<Sweep<D>>(c);
     return;
   }
   if (c.order > 0) {

@@ -108,6 +108,8 @@ def test_sweep_run_lengths_identical(lib, nd, order, bcs, precision):
     segs = [(k,) * nd for k in (0, 1, 3, 7, 64)]
     if nd == 3:
         segs.append((3, 5, -1))   # one workgroup per line along z
+    if nd == 2:
+        segs.append((-1, 3))      # one workgroup per line along x
     for seg in segs:
         print('case: sweep run lengths %s nd=%d order=%d bcs=%s fp%d' % (seg, nd, order, bcs, precision),
               flush=True)
@@ -134,8 +136,9 @@ def test_sweep_run_lengths_identical(lib, nd, order, bcs, precision):
 def test_sweep_rejects_bad_run_length(lib):
     from pythonmhd_amd._lib import PmhdError
     s = Solver()((8, 8), (0.125, 0.125), (1, 1), GAMMA, 0.8, order=1, lib=lib)
+    s.set_sweep((-1, 2))           # line launches along x, the contiguous axis in 2D
     with pytest.raises(PmhdError):
-        s.set_sweep((-1, 2))
+        s.set_sweep((2, -1))
     with pytest.raises(PmhdError):
         s.set_sweep((2, 5000))
     s.close()
@@ -143,6 +146,8 @@ def test_sweep_rejects_bad_run_length(lib):
     s.set_sweep((4, 4, -1))
     with pytest.raises(PmhdError):
         s.set_sweep((4, 4, -2))
+    with pytest.raises(PmhdError):
+        s.set_sweep((-1, 4, 4))
     s.close()
 
 
