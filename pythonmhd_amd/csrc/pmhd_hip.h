@@ -138,23 +138,24 @@ __device__ inline unsigned long long speed_bits(double x) { return (unsigned lon
 
 // a phase whose elements also yield three speeds: each block writes its maxima
 // to part[3 * block] (no atomics; reduced by k_max_parts)
+// (flat numbering with n2 fastest, XCD-contiguous like k_phase_flat)
 template <class Ph, class T>
-__global__ void __launch_bounds__(256) k_phase_max(const Ctx<T> c, int n1, int n2, unsigned long long* part) {
-  const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
-  const int i = int(blockIdx.y * blockDim.y + threadIdx.y);
-  const int j = int(blockIdx.z);
+__global__ void __launch_bounds__(256) k_phase_max(const Ctx<T> c, unsigned n1, unsigned n2, unsigned total,
+                                                   unsigned long long* part) {
+  const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   unsigned long long m[3] = {0ull, 0ull, 0ull};
-  if (k < n2 && i < n1) {
+  if (t < total) {
+    const unsigned k = t % n2;
+    const unsigned r = t / n2;
     double o[3];
-    Ph()(c, I3{j, i, k}, o);
+    Ph()(c, I3{int64_t(r / n1), int64_t(r % n1), int64_t(k)}, o);
 #pragma unroll
     for (int d = 0; d < 3; ++d) m[d] = speed_bits(o[d]);
   }
   block_max3(m);
-  if (threadIdx.x == 0 && threadIdx.y == 0) {
-    const size_t b = size_t(blockIdx.x) + size_t(gridDim.x) * (size_t(blockIdx.y) + size_t(gridDim.y) * blockIdx.z);
+  if (threadIdx.x == 0) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) part[3 * b + d] = m[d];
+    for (int d = 0; d < 3; ++d) part[3 * size_t(blockIdx.x) + d] = m[d];
   }
 }
 
@@ -480,9 +481,8 @@ struct HipExec {
   void run_max(const Ctx<T>& c) {
     int64_t n[3];
     c.extent(Ph::kind, Ph::dir, n);
-    dim3 grid, block;
-    grid_of(n, grid, block);
-    const int64_t nb = int64_t(grid.x) * grid.y * grid.z;
+    const int64_t total = n[0] * n[1] * n[2];   // cells: below 2^29
+    const int64_t nb = (total + flat_block - 1) / flat_block;
     if (nb > parts_cap) {
       if (parts) note(hipFree(parts));
       parts = nullptr;
@@ -490,8 +490,9 @@ struct HipExec {
       parts_cap = nb;
     }
     parts_n = nb;
-    timed(Ph::id, n[0] * n[1] * n[2], [&] {
-      hipLaunchKernelGGL((k_phase_max<Ph, T>), grid, block, 0, s, c, int(n[1]), int(n[2]), parts);
+    timed(Ph::id, total, [&] {
+      hipLaunchKernelGGL((k_phase_max<Ph, T>), dim3(unsigned(nb)), dim3(unsigned(flat_block)), 0, s, c,
+                         unsigned(n[1]), unsigned(n[2]), unsigned(total), parts);
     });
     note(hipGetLastError());
   }
