@@ -40,8 +40,8 @@ ALG_WORDS = {
     'emf': 12 + 3,          # three face-flux components per face array + cell E in (per edge point)
     'ctu_correct': 34 + 1 + 7,  # states, transverse fluxes, EMF, faces, cell B/v, half-step B in; flux out
     'emf_corr': 12 + 3,
-    'ct_half': 3 + 1,
-    'ct_full': 3 + 1,
+    'ct_half': 6 + 3,       # faces of the three directions and three edge EMFs in, faces out (per edge point)
+    'ct_full': 6 + 3,
     'half_e': 19 + 3,
     'correct': 15 + 7,
     'update': 31 + 16,      # U, fluxes, new faces in; cons + prim out (per cell)

@@ -58,7 +58,9 @@ __global__ void __launch_bounds__(256) k_phase_flat(const Ctx<T> c, unsigned n1,
 // states are those of k_sweep_run, so the results are identical.  q: the line's
 // first cell.  Every thread of the workgroup calls it (it synchronises).
 template <int D, class T>
-__device__ inline void sweep_line(const Ctx<T>& c, const I3 q, T (&sra)[7][256], T (&prev)[7]) {
+__device__ inline void sweep_line(const Ctx<T>& c, const I3 q) {
+  __shared__ T sra[7][256];
+  __shared__ T prev[7];   // right edge state of the last cell of the previous chunk
   const int t = int(threadIdx.x), B = int(blockDim.x);
   const int64_t n = c.n_along(D);
   const int bc = c.bc[D];
@@ -91,12 +93,13 @@ __device__ inline void sweep_line(const Ctx<T>& c, const I3 q, T (&sra)[7][256],
   }
 }
 
-// one workgroup per line; line blockIdx.x
+// one workgroup per line: (j, i) = (blockIdx.y, blockIdx.x) along z in 3D, j =
+// blockIdx.x along x in 2D
 template <int D, class T>
 __global__ void __launch_bounds__(256) k_sweep_line(const Ctx<T> c) {
-  __shared__ T sra[7][256];
-  __shared__ T prev[7];   // right edge state of the last cell of the previous chunk
-  sweep_line<D>(c, line_start(c, D, int64_t(blockIdx.x)), sra, prev);
+  static_assert(D == 0 || D == 2, "line sweeps run along the contiguous axis");
+  const I3 q = D == 2 ? I3{int64_t(blockIdx.y), int64_t(blockIdx.x), 0} : I3{int64_t(blockIdx.x), 0, 0};
+  sweep_line<D>(c, q);
 }
 
 __device__ inline unsigned long long umax64(unsigned long long a, unsigned long long b) {
@@ -242,14 +245,12 @@ __global__ void __launch_bounds__(256) k_batch(const Ctx<T>* __restrict__ ctxs, 
 template <int D, class T>
 __global__ void __launch_bounds__(256) k_batch_line(const Ctx<T>* __restrict__ ctxs, const int* __restrict__ prefix,
                                                     int nbox) {
-  __shared__ T sra[7][256];
-  __shared__ T prev[7];
   const int total = prefix[nbox];
   for (int vb = int(blockIdx.x); vb < total; vb += int(gridDim.x)) {
     const TileRef r = locate_tile(ctxs, prefix, nbox, vb, EXT_LINE, D, 1, 1, 0);
     const Ctx<T>& c = ctxs[r.box];
     if (!c.active) continue;
-    sweep_line<D>(c, I3{r.j, r.i, r.k}, sra, prev);
+    sweep_line<D>(c, I3{r.j, r.i, r.k});
   }
 }
 
@@ -443,10 +444,10 @@ struct HipExec {
       // the line, else 256 in chunks
       unsigned b = 64;
       while (b < 256 && int64_t(b) < c.n_along(Ph::dir)) b <<= 1;
-      // (a grid has below 2^29 cells, so the line count fits the grid's x extent)
-      const int64_t lines = line_count(c, Ph::dir);
+      // lines (j, i) along z in 3D, j along x in 2D (nz = 1)
+      const dim3 grid = Ph::dir == 2 ? dim3(unsigned(c.nx), unsigned(c.ny)) : dim3(unsigned(c.ny));
       timed(Ph::id, work, [&] {
-        hipLaunchKernelGGL((k_sweep_line<Ph::dir, T>), dim3(unsigned(lines)), dim3(b), 0, s, c);
+        hipLaunchKernelGGL((k_sweep_line<Ph::dir, T>), grid, dim3(b), 0, s, c);
       });
       note(hipGetLastError());
       return;

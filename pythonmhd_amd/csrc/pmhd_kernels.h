@@ -32,8 +32,8 @@ inline const char* kernel_name(int id) {
   static const char* names[K_COUNT] = {
       "slopes_x", "slopes_y", "slopes_z", "edges_x", "edges_y", "edges_z", "copy_x", "copy_y", "copy_z",
       "predict_x", "predict_y", "predict_z", "cell_e", "emf", "emf_corr", "unused_15", "unused_16",
-      "ct_half_x", "ct_half_y", "ct_half_z", "half_e", "ctu_correct_x", "ctu_correct_y", "ctu_correct_z",
-      "ct_full_x", "ct_full_y", "ct_full_z", "update", "cfl", "diag", "sweep_x", "sweep_y", "sweep_z",
+      "ct_half", "unused_18", "unused_19", "half_e", "ctu_correct_x", "ctu_correct_y", "ctu_correct_z",
+      "ct_full", "unused_25", "unused_26", "update", "cfl", "diag", "sweep_x", "sweep_y", "sweep_z",
       "sweep_z", "sweep_ends_z", "adjoint", "sweep_x", "sweep_ends_x"};
   return (id >= 0 && id < K_COUNT) ? names[id] : "?";
 }
@@ -46,11 +46,8 @@ inline void phase_shape(int id, int& kind, int& dir) {
   else if (id < K_PREDICT) { kind = EXT_FACE; dir = id - K_COPY; }
   else if (id < K_CELL_E) { kind = EXT_FACE; dir = id - K_PREDICT; }
   else if (id == K_EMF || id == K_EMF2) { kind = EXT_EDGE; }
-  else if (id >= K_CT_HALF && id < K_HALF_E) { kind = EXT_FACE; dir = id - K_CT_HALF; }
-  else if (id >= K_CORRECT && id < K_UPDATE) {
-    kind = EXT_FACE;
-    dir = id < K_CT_FULL ? id - K_CORRECT : id - K_CT_FULL;
-  }
+  else if (id == K_CT_HALF || id == K_CT_FULL) { kind = EXT_EDGE; }
+  else if (id >= K_CORRECT && id < K_CT_FULL) { kind = EXT_FACE; dir = id - K_CORRECT; }
   else if (id >= K_SWEEP && id < K_SWEEP + 3) { kind = EXT_SEG; dir = id - K_SWEEP; }
   else if (id == K_SWEEP_LINE_Z) { kind = EXT_LINE; dir = 2; }
   else if (id == K_SWEEP_ENDS_Z) { kind = EXT_ENDS; dir = 2; }
@@ -597,17 +594,11 @@ PMHD_FI void k_sweep_run(const Ctx<T>& c, int d, int bc, I3 q, int64_t f0, int64
 // other two coordinates, slowest first.
 template <class T>
 PMHD_FI int64_t line_count(const Ctx<T>& c, int d) {
-  return c.ncell() / c.n_along(d);
+  return d == 2 ? c.ny * c.nx : (d == 0 ? c.ny * c.nz : c.nx * c.nz);
 }
 template <class T>
 PMHD_FI int64_t line_of(const Ctx<T>& c, int d, I3 q) {
   return d == 2 ? q.j * c.nx + q.i : (d == 0 ? q.j * c.nz + q.k : q.i * c.nz + q.k);
-}
-template <class T>
-PMHD_FI I3 line_start(const Ctx<T>& c, int d, int64_t l) {
-  if (d == 2) return I3{l / c.nx, l % c.nx, 0};
-  if (d == 0) return I3{l / c.nz, 0, l % c.nz};
-  return I3{0, l / c.nz, l % c.nz};
 }
 // keep the left edge state of the first cell and the right edge state of the last
 // cell of the line through q along d for the end faces (k_sweep_end)

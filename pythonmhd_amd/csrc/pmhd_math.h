@@ -95,26 +95,29 @@ template <class T> PMHD_FI T npabs(T x) { return fabs_(x); }
 // (Source/PrimCons_HelperModule.py:208-452).  ke selects the order of the
 // kinetic-energy sum: 0 -> (1,2,3), 1 -> (2,3,1), 2 -> (3,1,2).
 // ---------------------------------------------------------------------------
+// |v|^2 of slots 1..3 summed from slot a = 1 + ke cyclically (the order is uniform
+// over a launch: written as branches, not as a run-time index into the registers)
+template <class T>
+PMHD_FI T ke_sum(const T v[7], int ke) {
+  if (ke == 0) return v[1] * v[1] + v[2] * v[2] + v[3] * v[3];
+  if (ke == 1) return v[2] * v[2] + v[3] * v[3] + v[1] * v[1];
+  return v[3] * v[3] + v[1] * v[1] + v[2] * v[2];
+}
+
 template <class T>
 PMHD_FI void prim_to_cons(const T w[7], T bn, T gamma, int ke, T u[7]) {
-  const int a = ke == 0 ? 1 : (ke == 1 ? 2 : 3);
-  const int b = ke == 0 ? 2 : (ke == 1 ? 3 : 1);
-  const int c = ke == 0 ? 3 : (ke == 1 ? 1 : 2);
   u[0] = w[0];
   u[1] = w[0] * w[1];
   u[2] = w[0] * w[2];
   u[3] = w[0] * w[3];
   u[4] = w[4];
   u[5] = w[5];
-  T e = w[6] / (gamma - T(1)) + T(0.5) * w[0] * (w[a] * w[a] + w[b] * w[b] + w[c] * w[c]);
+  T e = w[6] / (gamma - T(1)) + T(0.5) * w[0] * ke_sum(w, ke);
   u[6] = e + T(0.5) * (bn * bn + w[4] * w[4] + w[5] * w[5]);
 }
 
 template <class T>
 PMHD_FI void cons_to_prim(const T u[7], T bn, T gamma, int ke, T w[7]) {
-  const int a = ke == 0 ? 1 : (ke == 1 ? 2 : 3);
-  const int b = ke == 0 ? 2 : (ke == 1 ? 3 : 1);
-  const int c = ke == 0 ? 3 : (ke == 1 ? 1 : 2);
   T inv = T(1) / u[0];
   w[0] = u[0];
   w[1] = u[1] * inv;
@@ -122,7 +125,7 @@ PMHD_FI void cons_to_prim(const T u[7], T bn, T gamma, int ke, T w[7]) {
   w[3] = u[3] * inv;
   w[4] = u[4];
   w[5] = u[5];
-  T p = u[6] - T(0.5) * (u[a] * u[a] + u[b] * u[b] + u[c] * u[c]) * inv;
+  T p = u[6] - T(0.5) * ke_sum(u, ke) * inv;
   p = p - T(0.5) * (bn * bn + w[4] * w[4] + w[5] * w[5]);
   w[6] = p * (gamma - T(1));
 }

@@ -91,12 +91,21 @@ struct EmfCorrector {
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_emf(c, c.Fc, p); }
 };
-template <int D>
 struct CtHalf {  // half-step face field
-  PMHD_PHASE_TRAITS(K_CT_HALF + D, EXT_FACE, D)
+  PMHD_PHASE_TRAITS(K_CT_HALF, EXT_EDGE, 0)
+  // one launch over the edge points for all directions: element (j, i, k) updates
+  // the faces (j, i, k) of every direction that has one there
+  template <int D, class T>
+  PMHD_FI static void face(const Ctx<T>& c, I3 p) {
+    if (D < c.ndim && p.j < c.fny(D) && p.i < c.fnx(D) && p.k < c.fnz(D))
+      k_ct(c, D, c.fb[D], c.fblo[D], c.nfb[D], c.nfblo[D], c.hdtd, p);
+  }
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 p) const {
-    k_ct(c, D, c.fb[D], c.fblo[D], c.nfb[D], c.nfblo[D], c.hdtd, p);
+    // (compile-time directions: the context's arrays are never indexed at run time)
+    face<0>(c, p);
+    face<1>(c, p);
+    face<2>(c, p);
   }
 };
 struct HalfE {
@@ -110,12 +119,18 @@ struct Correct {  // CTU transverse corrections + corrector flux
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_correct(c, D, p); }
 };
-template <int D>
-struct CtFull {  // full-step face field
-  PMHD_PHASE_TRAITS(K_CT_FULL + D, EXT_FACE, D)
+struct CtFull {  // full-step face field (one launch for all directions, as CtHalf)
+  PMHD_PHASE_TRAITS(K_CT_FULL, EXT_EDGE, 0)
+  template <int D, class T>
+  PMHD_FI static void face(const Ctx<T>& c, I3 p) {
+    if (D < c.ndim && p.j < c.fny(D) && p.i < c.fnx(D) && p.k < c.fnz(D))
+      k_ct(c, D, c.fb[D], c.fblo[D], c.fb2[D], c.fb2lo[D], c.dtd, p);
+  }
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 p) const {
-    k_ct(c, D, c.fb[D], c.fblo[D], c.fb2[D], c.fb2lo[D], c.dtd, p);
+    face<0>(c, p);
+    face<1>(c, p);
+    face<2>(c, p);
   }
 };
 template <class T>
@@ -202,11 +217,11 @@ void cycle(X& ex, const Ctx<T>& c) {
   }
   if (!c.ec_valid) ex.template run<CellE>(c);   // else the last update stored it
   ex.template run<EmfPredictor>(c);   // corner EMFs of the predictor fluxes
-  each_dir<CtHalf>(ex, c);            // half-step face field
+  ex.template run<CtHalf>(c);         // half-step face field
   ex.template run<HalfE>(c);
   each_dir<Correct>(ex, c);           // CTU transverse corrections + corrector fluxes
   ex.template run<EmfCorrector>(c);   // corner EMFs of the corrector fluxes
-  each_dir<CtFull>(ex, c);            // full-step face field
+  ex.template run<CtFull>(c);         // full-step face field
   ex.template run_max<Update>(c);
 }
 
