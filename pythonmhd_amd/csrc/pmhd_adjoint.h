@@ -242,6 +242,11 @@ struct TapeExec {
     }
   }
   template <class Ph>
+  void run_side(const Ctx<T>& c) {
+    run<Ph>(c);
+  }
+  void join_side() {}
+  template <class Ph>
   void run_max(const Ctx<T>& c) {
     Ctx<T> cv;
     Ctx<double> av;

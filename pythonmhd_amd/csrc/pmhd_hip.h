@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "pmhd_adjoint.h"
@@ -423,6 +424,57 @@ struct HipExec {
   int flat_block = 256;   // threads per block of the flat launches (64..256)
   bool flat_all = true;   // every phase as a flat, XCD-numbered launch
 
+  // A second stream (default priority) for launches that overlap the following
+  // ones: run_side forks after the work queued so far, join_side makes the main
+  // stream wait for it.  With per-launch timing on, side launches run in order.
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  bool side_pending = false;
+  template <class L>
+  void on_side(L launch) {
+    if (timing || !side_ready()) {
+      launch();
+      return;
+    }
+    note(hipEventRecord(fork_ev, s));
+    note(hipStreamWaitEvent(side, fork_ev, 0));
+    std::swap(s, side);
+    launch();
+    std::swap(s, side);
+    note(hipEventRecord(join_ev, side));
+    side_pending = true;
+  }
+  bool side_ready() {
+    if (side) return true;
+    if (hipStreamCreate(&side) != hipSuccess) {
+      side = nullptr;
+      (void)hipGetLastError();
+      return false;
+    }
+    note(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+    note(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+    return true;
+  }
+  void join_side() {
+    if (!side_pending) return;
+    note(hipStreamWaitEvent(s, join_ev, 0));
+    side_pending = false;
+  }
+  void destroy_side() {
+    if (side) {
+      (void)hipStreamSynchronize(side);
+      (void)hipStreamDestroy(side);
+    }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
+    side = nullptr;
+    fork_ev = join_ev = nullptr;
+  }
+  template <class Ph, class T>
+  void run_side(const Ctx<T>& c) {
+    on_side([&] { run<Ph>(c); });
+  }
+
   static void grid_of(const int64_t n[3], dim3& grid, dim3& block) {
     unsigned bx = 1;
     while (bx < 64 && int64_t(bx) < n[2]) bx <<= 1;
@@ -567,6 +619,15 @@ struct HipBatchExec {
   }
 
   template <class Ph, class T>
+  void run_side(const Ctx<T>& c) {
+    if (tuning) {
+      run<Ph>(c);
+      return;
+    }
+    ex->on_side([&] { run<Ph>(c); });
+  }
+  void join_side() { ex->join_side(); }
+  template <class Ph, class T>
   void run(const Ctx<T>&) {
     if constexpr (Ph::kind == EXT_LINE) {
       // one workgroup of 64 threads per line (lines of uneven boxes waste at most
@@ -704,6 +765,7 @@ struct HipBackend {
   }
   ~HipBackend() {
     ex.destroy_events();
+    ex.destroy_side();
     if (ex.s) hipStreamSynchronize(ex.s);
     if (ex.red) hipFree(ex.red);
     if (ex.part) hipFree(ex.part);

@@ -181,7 +181,8 @@ void predictor_sweep(X& ex, const Ctx<T>& c) {
     if constexpr (can_line) {
       if (line) {
         ex.template run<SweepLine<D>>(c);
-        ex.template run<SweepEnds<D>>(c);
+        // the end faces overlap the other directions' sweeps (joined before the EMFs)
+        ex.template run_side<SweepEnds<D>>(c);
         return;
       }
     }
@@ -208,9 +209,12 @@ void each_dir(X& ex, const Ctx<T>& c) {
 template <class T, class X>
 void cycle(X& ex, const Ctx<T>& c) {
   const int nd = c.ndim;
+  // the contiguous axis first (z in 3D): its line sweep's end-face launch then
+  // overlaps the other sweeps
+  if (nd == 3) predictor_sweep<2>(ex, c);
   predictor_sweep<0>(ex, c);
   if (nd >= 2) predictor_sweep<1>(ex, c);
-  if (nd == 3) predictor_sweep<2>(ex, c);
+  ex.join_side();
   if (nd == 1) {
     ex.template run_max<Update>(c);
     return;

@@ -60,6 +60,12 @@ struct HostExec {
   void run(const Ctx<T>& c) {
     host_each<Ph>(c);
   }
+  // (a GPU executor overlaps these launches with the following ones)
+  template <class Ph, class T>
+  void run_side(const Ctx<T>& c) {
+    run<Ph>(c);
+  }
+  void join_side() {}
   template <class Ph, class T>
   void run_max(const Ctx<T>& c) {
     for (int d = 0; d < 3; ++d) last[d] = 0;
@@ -102,6 +108,11 @@ struct HostBatchExec {
   // tuning on the host walks the phase with every candidate tile (checking each
   // walk) and keeps a candidate that varies with the launch id, so later launches
   // exercise many shapes
+  template <class Ph, class T>
+  void run_side(const Ctx<T>& c0) {
+    run<Ph>(c0);
+  }
+  void join_side() {}
   template <class Ph, class T>
   void run(const Ctx<T>& c0) {
     if (tuning) {
