@@ -372,9 +372,16 @@ PMHD_FI void cell_edge_states(const Ctx<T>& c, int d, int bc, I3 p, bool from_s,
       return (s.j * c.snx(d) + s.i) * c.snz(d) + s.k;
     };
     T sl[7], sc[7], sr[7];
-    load7(c.S, NS, sidx(e - 1), sl);
     load7(c.S, NS, sidx(e), sc);
-    load7(c.S, NS, sidx(e + 1), sr);
+    if (c.order == 2) {
+      load7(c.S, NS, sidx(e - 1), sl);
+      load7(c.S, NS, sidx(e + 1), sr);
+    } else {
+      // PLM traces with the cell's own slope only (its neighbours' slopes are not
+      // read, so the adjoint does not differentiate with respect to them)
+#pragma unroll
+      for (int v = 0; v < 7; ++v) sl[v] = sr[v] = sc[v];
+    }
     trace_cell(E, wl, wc, wr, sl, sc, sr, c.dtd[d], c.order, la, ra);
   } else {
     T sc[7];
