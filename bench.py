@@ -21,7 +21,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-from pythonmhd_amd.problems import adjoint_state, kh_boxes, turbulence  # noqa: E402
+from pythonmhd_amd.problems import adjoint_state, brio_wu, kh_boxes, orszag_tang, turbulence  # noqa: E402
 
 METRIC = "cell-updates/s (FP32 vs FP64 GPU) at 3D 256^3 MHD; L1 err vs FP64; adjoint/fwd time"
 GAMMA = 5.0 / 3.0
@@ -78,6 +78,46 @@ def run_gpu(prim, cons, faces, n, precision, steps, warmup, profile_steps=3, swe
         prof = s.profile()
         s.set_profiling(False)
     return s, el, prof, cyc + profile_steps
+
+
+def run_to_time(shape, widths, bcs, gamma, cfl, order, precision, prim, cons, faces, t_end):
+    """One handle run from t = 0 to t_end (state uploaded before the clock starts;
+    the host reads dt each cycle, as the reference's loop does)."""
+    from pythonmhd_amd._lib import Solver
+    s = Solver(shape, widths, bcs, gamma, cfl, order=order, entropy_fix=False, precision=precision)
+    s.set_state(prim, cons, faces)
+    s.synchronize()
+    t0 = time.perf_counter()
+    t, cyc, _ = s.advance(0.0, t_end, 0, 10 ** 9)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    state = s.get_state(0)
+    s.close()
+    return t, cyc, el, state
+
+
+def run_configs():
+    """The 1D Brio-Wu and 2D Orszag-Tang configurations, each to its end time:
+    wall time, cycles and cell-updates/s on the GPU (FP64, and FP32 for Orszag-Tang
+    with its density L1 distance from the FP64 run)."""
+    out = {}
+    n = 512
+    prim, cons, _ = brio_wu(n)
+    run_to_time((n,), (1.0 / n,), (0,), 2.0, 0.8, 1, 64, prim, cons, (), 0.01)   # warm-up
+    t, cyc, el, _ = run_to_time((n,), (1.0 / n,), (0,), 2.0, 0.8, 1, 64, prim, cons, (), 0.1)
+    out["brio_wu"] = {"grid": "512 cells, outflow, gamma 2, PLM, CFL 0.8, FP64", "t_end": t, "cycles": cyc,
+                      "seconds": el, "cell_updates_per_s": n * cyc / el}
+    prim, cons, faces = orszag_tang(n)
+    ot = {"grid": "512^2 periodic, gamma 5/3, PPM, CFL 0.8", "t_end": 0.5}
+    rho = {}
+    for prec in (32, 64):
+        run_to_time((n, n), (1.0 / n,) * 2, (1, 1), GAMMA, 0.8, 2, prec, prim, cons, faces, 0.005)  # warm-up
+        t, cyc, el, st = run_to_time((n, n), (1.0 / n,) * 2, (1, 1), GAMMA, 0.8, 2, prec, prim, cons, faces, 0.5)
+        ot["fp%d" % prec] = {"cycles": cyc, "seconds": el, "cell_updates_per_s": n * n * cyc / el}
+        rho[prec] = st[1][0]
+    ot["l1_rho_fp32_vs_fp64"] = float(np.mean(np.abs(rho[32] - rho[64])))
+    out["orszag_tang"] = ot
+    return out
 
 
 def run_kh(cycles=200, seq_cycles=50, nbox=256):
@@ -310,6 +350,7 @@ def main():
     ap.add_argument('--no-kh', action='store_true')
     ap.add_argument('--kh-cycles', type=int, default=200)
     ap.add_argument('--no-adjoint', action='store_true')
+    ap.add_argument('--no-configs', action='store_true', help='skip the Brio-Wu and Orszag-Tang runs')
     ap.add_argument('--adj-steps', type=int, default=200)
     ap.add_argument('--adj-n', type=int, default=256)
     ap.add_argument('--copy-probe', action='store_true',
@@ -378,6 +419,11 @@ def main():
         out["kh_batch"] = run_kh(cycles=a.kh_cycles)
         print('bench: KH batch %.3e cell-updates/s (sequential %.3e)' % (
             out["kh_batch"]["value"], out["kh_batch"]["sequential"]["value"]), file=sys.stderr, flush=True)
+    if not a.no_configs:
+        out.update(run_configs())
+        print('bench: Brio-Wu fp64 %.3f s (%d cycles); Orszag-Tang to t=0.5: fp32 %.3e, fp64 %.3e cell-updates/s' % (
+            out["brio_wu"]["seconds"], out["brio_wu"]["cycles"], out["orszag_tang"]["fp32"]["cell_updates_per_s"],
+            out["orszag_tang"]["fp64"]["cell_updates_per_s"]), file=sys.stderr, flush=True)
     if not a.no_adjoint:
         out["adjoint"] = run_adjoint(n=a.adj_n, steps=a.adj_steps)
     if not a.no_cpu:
