@@ -232,7 +232,8 @@ def _batch_cases(nd, rng):
     return shapes, bcs, states
 
 
-@pytest.mark.parametrize('nd,precision,order', [(2, 64, 1), (2, 32, 1), (2, 64, 2), (3, 64, 2), (3, 32, 1)])
+@pytest.mark.parametrize('nd,precision,order', [(2, 64, 1), (2, 32, 1), (2, 64, 2), (2, 32, 2), (3, 64, 2), (3, 32, 1),
+                                                (3, 32, 2)])
 def test_batch_matches_single(lib, nd, precision, order):
     """Uneven boxes with mixed boundary conditions in one batch equal one handle per
     box -- bit for bit in FP64; in FP32, whose multiply-adds the compiler may
