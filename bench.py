@@ -254,7 +254,7 @@ def run_adjoint(n=256, steps=200, every=20, ncheck=32, check_steps=20):
     return out
 
 
-def cpu_baseline(n=64, cycles=2):
+def cpu_baseline(n=64, cycles=8):
     """The FP64 NumPy oracle (single thread) on a bounded sample of the workload."""
     from oracle import mhd as M
     prim, cons, faces = turbulence(n)
