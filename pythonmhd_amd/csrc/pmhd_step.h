@@ -11,6 +11,7 @@
 // template parameter so the rotations and index arithmetic fold at compile time.
 // (Source/EvolveFunctions_MHD.py:252-317 for 1D, :464-692 for 2D, :840-1152 for 3D)
 #pragma once
+#include <type_traits>
 #include "pmhd_kernels.h"
 
 PMHD_NS_BEGIN
@@ -146,9 +147,15 @@ struct Update {
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 p, double o[3]) const {
     k_update(c, p);
-    T t[3];
-    k_cfl(c, c.U2, c.fb2, p, t);
-    speeds_out(t, o);
+    if constexpr (std::is_floating_point<T>::value) {
+      T t[3];
+      k_cfl(c, c.U2, c.fb2, p, t);
+      speeds_out(t, o);
+    } else {
+      // the adjoint's dual evaluation: the speeds only pick the next dt, which the
+      // adjoint keeps fixed
+      o[0] = o[1] = o[2] = 0.0;
+    }
   }
 };
 struct CflSpeeds {  // per-cell speeds of the current state, maximised by X::cfl
