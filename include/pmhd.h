@@ -26,6 +26,7 @@ extern "C" {
 #define PMHD_EINVAL -1   /* bad shape, boundary condition or parameter */
 #define PMHD_EHIP -2     /* a HIP runtime call failed */
 #define PMHD_ENOMEM -3   /* device allocation failed */
+#define PMHD_ENONFINITE -4 /* the state became non-finite during pmhd_advance (NaN timestep) */
 
 typedef struct pmhd_grid {
   int32_t ndim;            /* 1, 2 or 3 */
@@ -67,6 +68,25 @@ int pmhd_set_state(pmhd_handle* h, const double* prim, const double* cons,
 int pmhd_get_state(pmhd_handle* h, int32_t which, double* prim, double* cons,
                    double* fbx, double* fby, double* fbz);
 
+/* pmhd_set_state / pmhd_get_state with DEVICE float64 arrays in the same
+ * reference layout (for instance torch ROCm tensors of dtype float64): the
+ * device-resident boundary.  The arrays stay owned by the caller (the library
+ * never frees them); the conversion to the handle's storage (float, and the FP32
+ * step's face field as high and low float parts) runs on the device, on the
+ * handle's stream, and both calls synchronise that stream before returning.  No
+ * PCIe traffic except, in 1D MHD, the uniform B_x (one value).  Same arguments and
+ * errors as the host versions. */
+int pmhd_set_state_device(pmhd_handle* h, const double* prim, const double* cons,
+                          const double* fbx, const double* fby, const double* fbz);
+int pmhd_get_state_device(pmhd_handle* h, int32_t which, double* prim, double* cons,
+                          double* fbx, double* fby, double* fbz);
+
+/* Launch the handle's work on the caller's HIP stream (a hipStream_t, for
+ * instance torch.cuda.current_stream().cuda_stream), so that it is ordered with
+ * the caller's own work on that stream; NULL returns to the handle's own stream.
+ * Work queued so far is finished first.  The stream must outlive its use. */
+int pmhd_set_stream(pmhd_handle* h, void* stream);
+
 /* CFL timestep of the current state (calcDt_{1,2,3}D_mhd,
  * Source/TimestepCalculations_HelperModule.py:183-361). */
 int pmhd_compute_dt(pmhd_handle* h, double* dt);
@@ -76,7 +96,9 @@ int pmhd_step(pmhd_handle* h, double dt);
 
 /* The evolve loop: cycles until t_end or max_cycles with the reference's
  * clipping of the last dt.  Returns the final time and cycle and the time of
- * the second-last state. */
+ * the second-last state.  When the state becomes non-finite the loop does what
+ * the reference's does (the cycle with the NaN timestep runs, the clock becomes
+ * NaN and the loop ends), fills the outputs and returns PMHD_ENONFINITE. */
 int pmhd_advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int64_t max_cycles,
                  double* t_out, int64_t* cycle_out, double* t_second_last);
 
@@ -103,6 +125,32 @@ int pmhd_synchronize(pmhd_handle* h);
  * no counterpart (its divergence check is a TODO, Source/SimulationGrid_HelperModule.py:406). */
 int pmhd_diagnostics(pmhd_handle* h, double* totals, double* max_divb, double* max_b,
                      int64_t* nonfinite);
+
+/* Floor and health statistics of a grid.  The floors are the reference's
+ * applyFloors (Source/PrimCons_HelperModule.py:499-554), which floors silently;
+ * these counters record what it did.  Call sites, in the reference's call order:
+ *   0, 1  x predictor left / right    (Source/EvolveFunctions_MHD.py:278-280, 513-515, 911-914)
+ *   2, 3  y predictor top / bottom    (:517-519, 915-918)
+ *   4, 5  z predictor back / forward  (:919-922)
+ *   6, 7  2D post-CTU x left / right  (:562-565)
+ *   8, 9  2D post-CTU y top / bottom  (:573-576)
+ * (hydro: sites 0-5 at Source/EvolveFunctions_Hydro.py:286-288, 437-443, 675-685).
+ * [site][0] counts densities raised to min_dens, [site][1] pressures raised to
+ * min_pres (predictor) or energies raised to min_energy (post-CTU).  The reference's
+ * 3D applyFloors raises IndexError when one call floors three or more values of one
+ * variable (its message loop reuses its index, :539-543): floored_max >= 3 at a 3D
+ * site marks a run the reference would not have completed. */
+typedef struct pmhd_stats {
+  int64_t cycles;              /* cycles since the state was set */
+  int64_t floored[10][2];      /* values floored since the state was set */
+  int64_t floored_max[10][2];  /* the most values one call site floored in one cycle */
+  int64_t floored_total;       /* sum of floored[][] */
+  int64_t nonfinite;           /* non-finite conserved values of the current state */
+} pmhd_stats;
+
+/* Statistics of the current state (synchronises the handle's stream).  No
+ * counterpart in the reference. */
+int pmhd_get_stats(pmhd_handle* h, pmhd_stats* out);
 
 /* Per-launch timing with HIP events on the handle's stream (measurement only).
  * enable = 1 clears the totals and starts recording; 0 stops. */
@@ -161,6 +209,8 @@ int pmhd_batch_set_profiling(pmhd_batch* b, int32_t enable);
 int pmhd_batch_get_profile(pmhd_batch* b, int32_t max_entries, char* names, double* total_ms,
                            int64_t* launches, int64_t* elements, int32_t* n_entries);
 const char* pmhd_batch_last_error(const pmhd_batch* b);
+/* pmhd_get_stats of one box. */
+int pmhd_batch_get_stats(pmhd_batch* b, int32_t box, pmhd_stats* out);
 
 /* ------------------------------------------------------------------------
  * Adjoint of a run of fixed-dt steps, with checkpoint-and-recompute.  No
@@ -174,6 +224,8 @@ const char* pmhd_batch_last_error(const pmhd_batch* b);
  * ------------------------------------------------------------------------ */
 typedef struct pmhd_adjoint pmhd_adjoint;
 
+/* 2D and 3D MHD and 1D-3D hydrodynamics (PMHD_EINVAL for 1D MHD, whose uniform
+ * B_x is a scalar of the step that the adjoint does not differentiate). */
 int pmhd_adjoint_create(const pmhd_grid* grid, const pmhd_params* params, int32_t checkpoint_every,
                         pmhd_adjoint** out);
 int pmhd_adjoint_destroy(pmhd_adjoint* a);
@@ -185,6 +237,18 @@ int pmhd_adjoint_backward(pmhd_adjoint* a, const double* gprim, const double* gc
                           const double* gfbx, const double* gfby, const double* gfbz,
                           double* gprim0, double* gcons0, double* gfbx0, double* gfby0,
                           double* gfbz0);
+/* forward / backward with DEVICE float64 arrays of the same layouts (see
+ * pmhd_set_state_device): the torch.autograd wrapper passes its ROCm tensors. */
+int pmhd_adjoint_forward_device(pmhd_adjoint* a, const double* prim, const double* cons,
+                                const double* fbx, const double* fby, const double* fbz, double dt,
+                                int64_t nsteps, double* prim_out, double* cons_out, double* fbx_out,
+                                double* fby_out, double* fbz_out);
+int pmhd_adjoint_backward_device(pmhd_adjoint* a, const double* gprim, const double* gcons,
+                                 const double* gfbx, const double* gfby, const double* gfbz,
+                                 double* gprim0, double* gcons0, double* gfbx0, double* gfby0,
+                                 double* gfbz0);
+/* pmhd_set_stream for the adjoint's launches. */
+int pmhd_adjoint_set_stream(pmhd_adjoint* a, void* stream);
 int pmhd_adjoint_set_profiling(pmhd_adjoint* a, int32_t enable);
 int pmhd_adjoint_get_profile(pmhd_adjoint* a, int32_t max_entries, char* names, double* total_ms,
                              int64_t* launches, int64_t* elements, int32_t* n_entries);

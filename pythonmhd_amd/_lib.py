@@ -12,14 +12,16 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_LIB = os.path.join(_HERE, 'libpmhd.so')
 
-EXPORTS = ('pmhd_create', 'pmhd_destroy', 'pmhd_set_state', 'pmhd_get_state', 'pmhd_compute_dt',
+EXPORTS = ('pmhd_create', 'pmhd_destroy', 'pmhd_set_state', 'pmhd_get_state', 'pmhd_set_state_device',
+           'pmhd_get_state_device', 'pmhd_set_stream', 'pmhd_compute_dt',
            'pmhd_step', 'pmhd_set_sweep', 'pmhd_advance', 'pmhd_synchronize', 'pmhd_diagnostics',
            'pmhd_set_profiling',
-           'pmhd_get_profile', 'pmhd_last_error',
+           'pmhd_get_profile', 'pmhd_get_stats', 'pmhd_last_error',
            'pmhd_batch_create', 'pmhd_batch_destroy', 'pmhd_batch_set_state', 'pmhd_batch_get_state',
            'pmhd_batch_advance', 'pmhd_batch_autotune', 'pmhd_batch_get_tiles', 'pmhd_batch_synchronize',
-           'pmhd_batch_set_profiling', 'pmhd_batch_get_profile', 'pmhd_batch_last_error',
+           'pmhd_batch_set_profiling', 'pmhd_batch_get_profile', 'pmhd_batch_get_stats', 'pmhd_batch_last_error',
            'pmhd_adjoint_create', 'pmhd_adjoint_destroy', 'pmhd_adjoint_forward', 'pmhd_adjoint_backward',
+           'pmhd_adjoint_forward_device', 'pmhd_adjoint_backward_device', 'pmhd_adjoint_set_stream',
            'pmhd_adjoint_set_profiling', 'pmhd_adjoint_get_profile', 'pmhd_adjoint_last_error')
 
 
@@ -39,8 +41,32 @@ class PmhdParams(ctypes.Structure):
                 ('precision', ctypes.c_int32), ('physics', ctypes.c_int32)]
 
 
+class PmhdStats(ctypes.Structure):
+    _fields_ = [('cycles', ctypes.c_int64), ('floored', (ctypes.c_int64 * 2) * 10),
+                ('floored_max', (ctypes.c_int64 * 2) * 10), ('floored_total', ctypes.c_int64),
+                ('nonfinite', ctypes.c_int64)]
+
+
+# return codes of include/pmhd.h
+OK, EINVAL, EHIP, ENOMEM, ENONFINITE = 0, -1, -2, -3, -4
+
+# floor call sites (pmhd_stats rows): predictor x/y/z left/right, 2D post-CTU x/y
+FLOOR_SITES = ('x left', 'x right', 'y top', 'y bottom', 'z back', 'z forward',
+               'x left after CTU', 'x right after CTU', 'y top after CTU', 'y bottom after CTU')
+
+
 class PmhdError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+def _stats_dict(st):
+    return {'cycles': st.cycles,
+            'floored': np.array([[st.floored[s][v] for v in range(2)] for s in range(10)], dtype=np.int64),
+            'floored_max': np.array([[st.floored_max[s][v] for v in range(2)] for s in range(10)],
+                                    dtype=np.int64),
+            'floored_total': st.floored_total, 'nonfinite': st.nonfinite}
 
 
 _cache = {}
@@ -53,6 +79,9 @@ def _declare(lib):
     lib.pmhd_destroy.argtypes = [H]
     lib.pmhd_set_state.argtypes = [H, P, P, P, P, P]
     lib.pmhd_get_state.argtypes = [H, ctypes.c_int32, P, P, P, P, P]
+    lib.pmhd_set_state_device.argtypes = [H, P, P, P, P, P]
+    lib.pmhd_get_state_device.argtypes = [H, ctypes.c_int32, P, P, P, P, P]
+    lib.pmhd_set_stream.argtypes = [H, P]
     lib.pmhd_compute_dt.argtypes = [H, ctypes.POINTER(ctypes.c_double)]
     lib.pmhd_step.argtypes = [H, ctypes.c_double]
     lib.pmhd_set_sweep.argtypes = [H, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
@@ -66,6 +95,8 @@ def _declare(lib):
     lib.pmhd_get_profile.argtypes = [H, ctypes.c_int32, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int32)]
+    lib.pmhd_get_stats.argtypes = [H, ctypes.POINTER(PmhdStats)]
+    lib.pmhd_batch_get_stats.argtypes = [H, ctypes.c_int32, ctypes.POINTER(PmhdStats)]
     lib.pmhd_last_error.argtypes = [H]
     lib.pmhd_last_error.restype = ctypes.c_char_p
     I32, I64, D = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
@@ -85,6 +116,9 @@ def _declare(lib):
     lib.pmhd_adjoint_destroy.argtypes = [H]
     lib.pmhd_adjoint_forward.argtypes = [H, P, P, P, P, P, D, I64, P, P, P, P, P]
     lib.pmhd_adjoint_backward.argtypes = [H] + [P] * 10
+    lib.pmhd_adjoint_forward_device.argtypes = [H, P, P, P, P, P, D, I64, P, P, P, P, P]
+    lib.pmhd_adjoint_backward_device.argtypes = [H] + [P] * 10
+    lib.pmhd_adjoint_set_stream.argtypes = [H, P]
     lib.pmhd_adjoint_set_profiling.argtypes = [H, I32]
     lib.pmhd_adjoint_get_profile.argtypes = lib.pmhd_get_profile.argtypes
     lib.pmhd_adjoint_last_error.argtypes = [H]
@@ -97,19 +131,58 @@ def _declare(lib):
 
 def load(path=None):
     """Open the step library (default: the in-tree HIP build).  Raises OSError /
-    PmhdError when it is absent -- the product has no CPU fallback."""
+    PmhdError when it is absent -- the product has no CPU fallback.
+
+    torch (when installed) is imported first: it brings its own HIP runtime, whose
+    soname the library's dependency then resolves to, so that torch's device
+    pointers and streams and the library's share one runtime."""
     path = os.path.abspath(path or DEFAULT_LIB)
     if path in _cache:
         return _cache[path]
     if not os.path.exists(path):
         raise PmhdError('libpmhd not built: %s is missing (run __graft_entry__.build())' % path)
+    if path == DEFAULT_LIB:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
     lib = _declare(ctypes.CDLL(path))
+    lib._pmhd_gpu = path == DEFAULT_LIB or not os.path.basename(path).startswith('libpmhd_host')
     _cache[path] = lib
     return lib
 
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def on_gpu(lib):
+    """True for the HIP library (device arrays are GPU memory); False for the host
+    test build of the same kernel bodies, whose "device" arrays are host memory."""
+    return getattr(lib, '_pmhd_gpu', True)
+
+
+def _dptr(t, shape=None, what='array', gpu=True):
+    """Device address of a contiguous float64 torch tensor (None for None).  The
+    tensor must live where the library's device arrays do: GPU memory for the HIP
+    library (a host address there would fault the GPU), host memory for the host
+    test build."""
+    if t is None:
+        return None
+    import torch
+    if t.dtype != torch.float64 or not t.is_contiguous():
+        raise ValueError('%s must be a contiguous float64 tensor' % what)
+    if bool(t.is_cuda) != bool(gpu):
+        raise ValueError('%s must be a %s tensor for this library' % (what, 'GPU' if gpu else 'CPU'))
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError('%s has shape %s, expected %s' % (what, tuple(t.shape), tuple(shape)))
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def torch_stream(device=None):
+    """The current torch stream's hipStream_t (as an integer), for set_stream."""
+    import torch
+    return torch.cuda.current_stream(device).cuda_stream
 
 
 def make_grid(shape, widths, bcs):
@@ -199,7 +272,7 @@ class Solver:
 
     def _check(self, rc, what):
         if rc != 0:
-            raise PmhdError('%s failed (%d): %s' % (what, rc, self.lib.pmhd_last_error(self.h).decode()))
+            raise PmhdError('%s failed (%d): %s' % (what, rc, self.lib.pmhd_last_error(self.h).decode()), rc)
 
     def face_shapes(self):
         return face_shapes(self.shape, self.physics)
@@ -217,6 +290,44 @@ class Solver:
         self._check(self.lib.pmhd_get_state(self.h, which, _ptr(prim), _ptr(cons), *fp), 'pmhd_get_state')
         return prim, cons, fs
 
+    def set_state_device(self, prim, cons, faces=()):
+        """set_state from device tensors (contiguous float64, the reference layout),
+        converted on the device; no host copy."""
+        nv = nvars(self.physics)
+        want = face_shapes(self.shape, self.physics)
+        faces = list(faces)
+        if len(faces) != len(want):
+            raise ValueError('expected %d face arrays' % len(want))
+        g = on_gpu(self.lib)
+        args = [_dptr(prim, (nv,) + self.shape, 'prim', g), _dptr(cons, (nv,) + self.shape, 'cons', g)]
+        args += [_dptr(f, s, 'face array', g) for f, s in zip(faces, want)] + [None] * (3 - len(want))
+        self._check(self.lib.pmhd_set_state_device(self.h, *args), 'pmhd_set_state_device')
+
+    def get_state_device(self, which=0, out=None):
+        """The current (0) or second-last (1) state as new device tensors (or into
+        ``out`` = (prim, cons, faces)) of float64 in the reference layout."""
+        import torch
+        nv = nvars(self.physics)
+        if out is None:
+            dev = torch.device('cuda', torch.cuda.current_device()) if on_gpu(self.lib) else torch.device('cpu')
+            prim = torch.empty((nv,) + self.shape, dtype=torch.float64, device=dev)
+            cons = torch.empty_like(prim)
+            faces = [torch.empty(s, dtype=torch.float64, device=dev) for s in self.face_shapes()]
+        else:
+            prim, cons, faces = out[0], out[1], list(out[2])
+        want = self.face_shapes()
+        g = on_gpu(self.lib)
+        args = [_dptr(prim, (nv,) + self.shape, 'prim', g), _dptr(cons, (nv,) + self.shape, 'cons', g)]
+        args += [_dptr(f, s, 'face array', g) for f, s in zip(faces, want)] + [None] * (3 - len(want))
+        self._check(self.lib.pmhd_get_state_device(self.h, int(which), *args), 'pmhd_get_state_device')
+        return prim, cons, faces
+
+    def set_stream(self, stream):
+        """Launch on the given hipStream_t (an integer, e.g. torch_stream()); None:
+        the handle's own stream."""
+        self._check(self.lib.pmhd_set_stream(self.h, None if stream is None else ctypes.c_void_p(int(stream))),
+                    'pmhd_set_stream')
+
     def compute_dt(self):
         dt = ctypes.c_double()
         self._check(self.lib.pmhd_compute_dt(self.h, ctypes.byref(dt)), 'pmhd_compute_dt')
@@ -231,14 +342,26 @@ class Solver:
         seg = list(seg) + [0] * (3 - len(seg))
         self._check(self.lib.pmhd_set_sweep(self.h, int(seg[0]), int(seg[1]), int(seg[2])), 'pmhd_set_sweep')
 
-    def advance(self, t0, t_end, cycle0, max_cycles):
+    def advance(self, t0, t_end, cycle0, max_cycles, allow_nonfinite=False):
+        """The evolve loop; returns (t, cycle, t of the second-last state).  A state
+        that becomes non-finite raises PmhdError (code ENONFINITE) unless
+        allow_nonfinite, which returns the reference loop's NaN clock instead."""
         t = ctypes.c_double()
         cyc = ctypes.c_int64()
         t2 = ctypes.c_double()
-        self._check(self.lib.pmhd_advance(self.h, float(t0), float(t_end), int(cycle0), int(max_cycles),
-                                          ctypes.byref(t), ctypes.byref(cyc), ctypes.byref(t2)),
-                    'pmhd_advance')
+        rc = self.lib.pmhd_advance(self.h, float(t0), float(t_end), int(cycle0), int(max_cycles),
+                                   ctypes.byref(t), ctypes.byref(cyc), ctypes.byref(t2))
+        if not (rc == ENONFINITE and allow_nonfinite):
+            self._check(rc, 'pmhd_advance')
         return t.value, cyc.value, t2.value
+
+    def stats(self):
+        """Floor counters per call site (FLOOR_SITES rows; columns density and
+        pressure/energy), their per-cycle maxima, cycles since set_state and the
+        non-finite count of the current state."""
+        st = PmhdStats()
+        self._check(self.lib.pmhd_get_stats(self.h, ctypes.byref(st)), 'pmhd_get_stats')
+        return _stats_dict(st)
 
     def diagnostics(self):
         """FP64 conserved totals (8), dimensionless max |div B|, max |B_face|, non-finite count."""
@@ -342,6 +465,13 @@ class Batch:
         self._check(self.lib.pmhd_batch_advance(self.h, float(t0), float(t_end), int(cycle0), int(max_cycles),
                                                 _ptr(t), _ptr(cyc), _ptr(t2)), 'pmhd_batch_advance')
         return t, cyc, t2
+
+    def stats(self, box):
+        """Solver.stats of one box."""
+        self._shape(box)
+        st = PmhdStats()
+        self._check(self.lib.pmhd_batch_get_stats(self.h, int(box), ctypes.byref(st)), 'pmhd_batch_get_stats')
+        return _stats_dict(st)
 
     def autotune(self, reps=3):
         self._check(self.lib.pmhd_batch_autotune(self.h, int(reps)), 'pmhd_batch_autotune')

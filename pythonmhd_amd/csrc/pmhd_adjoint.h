@@ -23,10 +23,12 @@ struct pmhd_adjoint {
   virtual ~pmhd_adjoint() {}
   std::string err;
   virtual int init(const pmhd_grid& g, const pmhd_params& p, int checkpoint_every) = 0;
+  // dev: the arrays are device float64 arrays (else host float64 arrays)
   virtual int forward(const double* prim, const double* cons, const double* const fin[3], double dt,
-                      int64_t nsteps, double* prim_out, double* cons_out, double* const fout[3]) = 0;
+                      int64_t nsteps, double* prim_out, double* cons_out, double* const fout[3], bool dev) = 0;
   virtual int backward(const double* gprim, const double* gcons, const double* const gf[3], double* gprim0,
-                       double* gcons0, double* const gf0[3]) = 0;
+                       double* gcons0, double* const gf0[3], bool dev) = 0;
+  virtual int set_stream(void* stream) = 0;
   virtual int set_profiling(int on) = 0;
   virtual int get_profile(int maxn, char* names, double* ms, int64_t* launches, int64_t* elems, int32_t* n) = 0;
   int fail(int code, const std::string& m) {
@@ -108,6 +110,7 @@ PMHD_FI void make_dual_ctx(const Ctx<V>& c, const Ctx<double>& a, AdState* st, C
   d.WR = ad_ptr<V>(c.WR, a.WR, st);
   d.S = ad_ptr<V>(c.S, a.S, st);
   d.Ec = ad_ptr<V>(c.Ec, a.Ec, st);
+  d.fcnt = nullptr;   // the dual evaluations floor without counting
 }
 
 // adjoint of one output element of phase Ph; returns false when the element has
@@ -334,6 +337,7 @@ struct AdjointHandle : public pmhd_adjoint {
       return fail(PMHD_ENOMEM, "device allocation failed");
     }
     bind_bufs(c, sb, 0);
+    be.zero(c.fcnt, FLOOR_WORDS * sizeof(unsigned long long));
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
   }
@@ -355,13 +359,14 @@ struct AdjointHandle : public pmhd_adjoint {
   }
 
   int forward(const double* prim, const double* cons, const double* const fin[3], double dt_, int64_t n,
-              double* prim_out, double* cons_out, double* const fout[3]) override {
+              double* prim_out, double* cons_out, double* const fout[3], bool dev) override {
     if (!(dt_ > 0) || n < 1) return fail(PMHD_EINVAL, "dt must be positive and nsteps at least 1");
     dt = dt_;
     nsteps = n;
     cur = 0;
     bind_bufs(c, sb, 0);
-    int rc = load_state(be, c, g, prim, cons, fin, sb.U[0], sb.W[0], sb.FB[0], sb.FBLO[0], err);
+    int rc = dev ? load_state_dev(be, c, g, prim, cons, fin, sb.U[0], sb.W[0], sb.FB[0], sb.FBLO[0], err)
+                 : load_state(be, c, g, prim, cons, fin, sb.U[0], sb.W[0], sb.FB[0], sb.FBLO[0], err);
     if (rc) return rc;
     set_dt_scalars();
     const int64_t nck = (n + every - 1) / every;
@@ -378,27 +383,35 @@ struct AdjointHandle : public pmhd_adjoint {
       step_forward();
     }
     double* fo[3] = {fout[0], fout[1], fout[2]};
-    store_state(be, c, g, sb.U[cur], sb.W[cur], sb.FB[cur], sb.FBLO[cur], prim_out, cons_out, fo);
+    if (dev) store_state_dev(be, c, g, sb.U[cur], sb.W[cur], sb.FB[cur], sb.FBLO[cur], prim_out, cons_out, fo);
+    else store_state(be, c, g, sb.U[cur], sb.W[cur], sb.FB[cur], sb.FBLO[cur], prim_out, cons_out, fo);
     be.sync();
     have_forward = true;
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
   }
 
-  void upload_adj(double* dst, const double* src, int64_t n) {
-    if (src) be.h2d(dst, src, sizeof(double) * size_t(n));
+  void upload_adj(double* dst, const double* src, int64_t n, bool dev) {
+    if (src && dev) be.d2d(dst, src, sizeof(double) * size_t(n));
+    else if (src) be.h2d(dst, src, sizeof(double) * size_t(n));
     else be.zero(dst, sizeof(double) * size_t(n));
   }
 
+  int set_stream(void* stream) override {
+    be.set_stream(stream);
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+
   int backward(const double* gprim, const double* gcons, const double* const gf[3], double* gprim0,
-               double* gcons0, double* const gf0[3]) override {
+               double* gcons0, double* const gf0[3], bool dev) override {
     if (!have_forward) return fail(PMHD_EINVAL, "backward needs a preceding forward");
     const int64_t N = c.ncell();
     int out = 0;   // adj_state[out]: cotangent of the current step's output state
-    upload_adj(adj_state[out][0], gcons, nv() * N);
-    upload_adj(adj_state[out][1], gprim, nv() * N);
+    upload_adj(adj_state[out][0], gcons, nv() * N, dev);
+    upload_adj(adj_state[out][1], gprim, nv() * N, dev);
     for (int d = 0; d < 3; ++d)
-      if (nface(d) > 0) upload_adj(adj_state[out][2 + d], gf[d], nface(d));
+      if (nface(d) > 0) upload_adj(adj_state[out][2 + d], gf[d], nface(d), dev);
     const int64_t nck = (nsteps + every - 1) / every;
     for (int64_t sgi = nck - 1; sgi >= 0; --sgi) {
       const int64_t s0 = sgi * every;
@@ -447,7 +460,8 @@ struct AdjointHandle : public pmhd_adjoint {
     // cotangent of the initial state
     std::vector<double> tmp;
     auto down = [&](double* dst, const double* src, int64_t n) {
-      if (dst) be.d2h(dst, src, sizeof(double) * size_t(n));
+      if (dst && dev) be.d2d(dst, src, sizeof(double) * size_t(n));
+      else if (dst) be.d2h(dst, src, sizeof(double) * size_t(n));
     };
     down(gcons0, adj_state[out][0], nv() * N);
     down(gprim0, adj_state[out][1], nv() * N);
@@ -479,10 +493,18 @@ inline int create_adjoint(const pmhd_grid* g, const pmhd_params* p, int every, p
   *out = nullptr;
   int rc = check_grid(g, err);
   if (rc) return rc;
-  rc = check_launch_extents(g, err);
+  // the phase adjoints run 64-thread blocks
+  rc = check_launch_extents(g, err, 64);
   if (rc) return rc;
   rc = check_params(p, err);
   if (rc) return rc;
+  if (g->ndim == 1 && p->physics == 0) {
+    // the 1D MHD step holds B_x as a scalar of the context (the reference's uniform
+    // Bx, Source/EvolveFunctions_MHD.py:209-212), which the dual evaluation cannot
+    // differentiate: its cotangent would be silently zero
+    err = "the adjoint supports 2D and 3D MHD and 1D-3D hydrodynamics, not 1D MHD";
+    return PMHD_EINVAL;
+  }
   if (every < 1) {
     err = "checkpoint_every must be at least 1";
     return PMHD_EINVAL;
@@ -529,8 +551,19 @@ PMHD_NS_END
     if (!a) return PMHD_EINVAL;                                                                     \
     const double* fin[3] = {fbx, fby, fbz};                                                         \
     double* fo[3] = {fbx_out, fby_out, fbz_out};                                                    \
-    try { return a->forward(prim, cons, fin, dt, nsteps, prim_out, cons_out, fo); }                 \
+    try { return a->forward(prim, cons, fin, dt, nsteps, prim_out, cons_out, fo, false); }          \
     catch (...) { return a->fail(PMHD_EINVAL, "exception in pmhd_adjoint_forward"); }               \
+  }                                                                                                 \
+  extern "C" int pmhd_adjoint_forward_device(pmhd_adjoint* a, const double* prim,                   \
+                                             const double* cons, const double* fbx,                 \
+                                             const double* fby, const double* fbz, double dt,       \
+                                             int64_t nsteps, double* prim_out, double* cons_out,    \
+                                             double* fbx_out, double* fby_out, double* fbz_out) {   \
+    if (!a) return PMHD_EINVAL;                                                                     \
+    const double* fin[3] = {fbx, fby, fbz};                                                         \
+    double* fo[3] = {fbx_out, fby_out, fbz_out};                                                    \
+    try { return a->forward(prim, cons, fin, dt, nsteps, prim_out, cons_out, fo, true); }           \
+    catch (...) { return a->fail(PMHD_EINVAL, "exception in pmhd_adjoint_forward_device"); }        \
   }                                                                                                 \
   extern "C" int pmhd_adjoint_backward(pmhd_adjoint* a, const double* gprim, const double* gcons,   \
                                        const double* gfbx, const double* gfby, const double* gfbz,  \
@@ -539,8 +572,23 @@ PMHD_NS_END
     if (!a) return PMHD_EINVAL;                                                                     \
     const double* gf[3] = {gfbx, gfby, gfbz};                                                       \
     double* g0[3] = {gfbx0, gfby0, gfbz0};                                                          \
-    try { return a->backward(gprim, gcons, gf, gprim0, gcons0, g0); }                               \
+    try { return a->backward(gprim, gcons, gf, gprim0, gcons0, g0, false); }                        \
     catch (...) { return a->fail(PMHD_EINVAL, "exception in pmhd_adjoint_backward"); }              \
+  }                                                                                                 \
+  extern "C" int pmhd_adjoint_backward_device(pmhd_adjoint* a, const double* gprim,                 \
+                                              const double* gcons, const double* gfbx,              \
+                                              const double* gfby, const double* gfbz,               \
+                                              double* gprim0, double* gcons0, double* gfbx0,        \
+                                              double* gfby0, double* gfbz0) {                       \
+    if (!a) return PMHD_EINVAL;                                                                     \
+    const double* gf[3] = {gfbx, gfby, gfbz};                                                       \
+    double* g0[3] = {gfbx0, gfby0, gfbz0};                                                          \
+    try { return a->backward(gprim, gcons, gf, gprim0, gcons0, g0, true); }                         \
+    catch (...) { return a->fail(PMHD_EINVAL, "exception in pmhd_adjoint_backward_device"); }       \
+  }                                                                                                 \
+  extern "C" int pmhd_adjoint_set_stream(pmhd_adjoint* a, void* stream) {                           \
+    if (!a) return PMHD_EINVAL;                                                                     \
+    return a->set_stream(stream);                                                                   \
   }                                                                                                 \
   extern "C" int pmhd_adjoint_set_profiling(pmhd_adjoint* a, int32_t enable) {                      \
     if (!a) return PMHD_EINVAL;                                                                     \

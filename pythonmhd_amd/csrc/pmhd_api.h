@@ -33,6 +33,10 @@ struct pmhd_handle {
   virtual int get_profile(int maxn, char* names, double* ms, int64_t* launches, int64_t* elems,
                           int32_t* n) = 0;
   virtual int set_sweep(const int32_t seg[3]) = 0;
+  virtual int get_stats(pmhd_stats* out) = 0;
+  virtual int set_state_device(const double*, const double*, const double*, const double*, const double*) = 0;
+  virtual int get_state_device(int, double*, double*, double*, double*, double*) = 0;
+  virtual int set_stream(void* stream) = 0;
   int fail(int code, const std::string& m) {
     err = m;
     return code;
@@ -64,6 +68,7 @@ struct pmhd_batch {
   virtual int set_profiling(int on) = 0;
   virtual int get_profile(int maxn, char* names, double* ms, int64_t* launches, int64_t* elems,
                           int32_t* n) = 0;
+  virtual int get_stats(int box, pmhd_stats* out) = 0;
   int fail(int code, const std::string& m) {
     err = m;
     return code;
@@ -71,6 +76,17 @@ struct pmhd_batch {
 };
 
 PMHD_NS_BEGIN
+
+// floor counters (device words, FLOOR_WORDS of them) -> the ABI's statistics
+inline void floor_stats(const unsigned long long* f, pmhd_stats* out) {
+  out->floored_total = 0;
+  for (int s = 0; s < FLOOR_SITES; ++s)
+    for (int v = 0; v < 2; ++v) {
+      out->floored[s][v] = int64_t(f[FLOOR_TOT + 2 * s + v]);
+      out->floored_max[s][v] = int64_t(f[FLOOR_MAX + 2 * s + v]);
+      out->floored_total += out->floored[s][v];
+    }
+}
 
 // ---------------------------------------------------------------------------
 // batch launch tables: per launch id, every box's index box and the tile shape
@@ -252,6 +268,7 @@ void alloc_grid(Ctx<T>& c, StateBufs<T>& s, int order, A&& alloc) {
     c.E[d] = alloc(mhd && (nd == 3 || (nd == 2 && d == 2)) ? c.nedge() : 1);
   }
   c.Ec = alloc(mhd ? (nd == 3 ? 3 * N : N) : 1);
+  c.fcnt = reinterpret_cast<unsigned long long*>(alloc(int64_t(FLOOR_WORDS * sizeof(unsigned long long) / sizeof(T))));
   c.zend = alloc(mhd && nd >= 2 ? 14 * (c.ncell() / c.n_along(nd == 3 ? 2 : 0)) : 1);
 }
 
@@ -327,6 +344,51 @@ int load_state(B& be, Ctx<T>& c, const pmhd_grid& g, const double* prim, const d
   return PMHD_OK;
 }
 
+// the same from / to device float64 arrays of the reference layout (conversions on
+// the device, on the backend's stream)
+template <class T, class B>
+int load_state_dev(B& be, Ctx<T>& c, const pmhd_grid& g, const double* prim, const double* cons,
+                   const double* const fin[3], T* U, T* W, T* const FB[3], T* const FBLO[3], std::string& err) {
+  if (!prim || !cons) {
+    err = "prim and cons are required";
+    return PMHD_EINVAL;
+  }
+  const bool mhd = c.physics == 0;
+  for (int d = 0; mhd && d < g.ndim && g.ndim >= 2; ++d)
+    if (!fin[d]) {
+      err = "missing face array";
+      return PMHD_EINVAL;
+    }
+  const int64_t N = c.ncell();
+  be.dev_in(U, cons, cell_vars(c.physics) * N);
+  be.dev_in(W, prim, cell_vars(c.physics) * N);
+  if (mhd && g.ndim >= 2)
+    for (int d = 0; d < g.ndim; ++d) {
+      if (split_faces<T>()) be.dev_split(FB[d], FBLO[d], fin[d], c.nface(d));
+      else be.dev_in(FB[d], fin[d], c.nface(d));
+    }
+  if (mhd && g.ndim == 1) {
+    double bx = 0.0;
+    be.d2h(&bx, prim + 4 * N, sizeof(double));
+    c.bx1d = T(bx);
+  }
+  return PMHD_OK;
+}
+
+template <class T, class B>
+void store_state_dev(B& be, const Ctx<T>& c, const pmhd_grid& g, const T* U, const T* W, const T* const FB[3],
+                     const T* const FBLO[3], double* prim, double* cons, double* const fo[3]) {
+  const int64_t N = c.ncell();
+  if (cons) be.dev_out(cons, U, cell_vars(c.physics) * N);
+  if (prim) be.dev_out(prim, W, cell_vars(c.physics) * N);
+  if (c.physics == 0 && g.ndim >= 2)
+    for (int d = 0; d < g.ndim; ++d)
+      if (fo[d]) {
+        if (split_faces<T>()) be.dev_join(fo[d], FB[d], FBLO[d], c.nface(d));
+        else be.dev_out(fo[d], FB[d], c.nface(d));
+      }
+}
+
 template <class T, class B>
 void store_state(B& be, const Ctx<T>& c, const pmhd_grid& g, const T* U, const T* W, const T* const FB[3],
                  const T* const FBLO[3], double* prim, double* cons, double* const fo[3]) {
@@ -381,6 +443,7 @@ struct Handle : public pmhd_handle {
       return fail(PMHD_ENOMEM, "device allocation failed");
     }
     bind_bufs(c, sb, cur);
+    be.zero(c.fcnt, FLOOR_WORDS * sizeof(unsigned long long));
     for (int d = 0; d < g.ndim; ++d)
       c.seg[d] = sweep_segment(g.ndim, d, c.nface(d), c.n_along(d), be.compute_units(), sizeof(T) == sizeof(double));
     c.ec_next = p.physics == 0 && g.ndim >= 2;
@@ -410,6 +473,7 @@ struct Handle : public pmhd_handle {
     bind_bufs(c, sb, cur);
     int rc = load_state(be, c, g, prim, cons, fin, sb.U[0], sb.W[0], sb.FB[0], sb.FBLO[0], err);
     if (rc) return rc;
+    be.zero(c.fcnt, FLOOR_WORDS * sizeof(unsigned long long));
     be.sync();
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
@@ -422,6 +486,38 @@ struct Handle : public pmhd_handle {
     be.sync();
     store_state(be, c, g, sb.U[s], sb.W[s], sb.FB[s], sb.FBLO[s], prim, cons, fo);
     be.sync();
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+
+  int set_state_device(const double* prim, const double* cons, const double* fbx, const double* fby,
+                       const double* fbz) override {
+    const double* fin[3] = {fbx, fby, fbz};
+    cur = 0;
+    cycles_since_set = 0;
+    speeds_ready = false;
+    ec_ready = false;
+    bind_bufs(c, sb, cur);
+    int rc = load_state_dev(be, c, g, prim, cons, fin, sb.U[0], sb.W[0], sb.FB[0], sb.FBLO[0], err);
+    if (rc) return rc;
+    be.zero(c.fcnt, FLOOR_WORDS * sizeof(unsigned long long));
+    be.sync();
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+
+  int get_state_device(int which, double* prim, double* cons, double* fbx, double* fby, double* fbz) override {
+    int s = cur;
+    if (which == 1 && cycles_since_set > 0) s = 1 - cur;
+    double* fo[3] = {fbx, fby, fbz};
+    store_state_dev(be, c, g, sb.U[s], sb.W[s], sb.FB[s], sb.FBLO[s], prim, cons, fo);
+    be.sync();
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+
+  int set_stream(void* stream) override {
+    be.set_stream(stream);
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
   }
@@ -482,6 +578,20 @@ struct Handle : public pmhd_handle {
 
   int get_profile(int maxn, char* names, double* ms, int64_t* launches, int64_t* elems, int32_t* n) override {
     *n = be.profile(maxn, names, ms, launches, elems);
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+
+  int get_stats(pmhd_stats* out) override {
+    std::memset(out, 0, sizeof(*out));
+    unsigned long long f[FLOOR_WORDS];
+    be.sync();
+    be.d2h(f, c.fcnt, sizeof(f));
+    floor_stats(f, out);
+    double sums[9], maxes[2];
+    pmhd::diagnostics(be.exec(), c, sums, maxes);
+    out->nonfinite = int64_t(sums[8]);
+    out->cycles = cycles_since_set;
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
   }
@@ -570,6 +680,7 @@ struct BatchHandle : public pmhd_batch {
       for (int d = 0; d < 3; ++d) hctx[size_t(b)].seg[d] = seg[d];
     }
     be.zero(lay.dred, 3 * sizeof(unsigned long long) * size_t(n));
+    for (int b = 0; b < n; ++b) be.zero(hctx[size_t(b)].fcnt, FLOOR_WORDS * sizeof(unsigned long long));
     // tile tables
     lay.ndim = grids[0].ndim;
     lay.cus = be.compute_units();
@@ -622,6 +733,7 @@ struct BatchHandle : public pmhd_batch {
                         bufs[size_t(box)].FB[0], bufs[size_t(box)].FBLO[0], err);
     if (rc) return rc;
     hclk[size_t(box)].done = 0;
+    be.zero(c.fcnt, FLOOR_WORDS * sizeof(unsigned long long));
     be.h2d(static_cast<Ctx<T>*>(lay.dctx) + box, &c, sizeof(Ctx<T>));
     be.sync();
     const char* e = be.error();
@@ -698,6 +810,8 @@ struct BatchHandle : public pmhd_batch {
     run_cycle(bx, hctx[0]);
     bx.tuning = false;
     be.zero(lay.dred, 3 * sizeof(unsigned long long) * size_t(nbox));
+    // the tuning launches floored and counted: start the counts afresh
+    for (int b = 0; b < nbox; ++b) be.zero(hctx[size_t(b)].fcnt, FLOOR_WORDS * sizeof(unsigned long long));
     be.sync();
     for (BoxClock& k : hclk) k.done = 0;
     pull_ctx();
@@ -736,6 +850,21 @@ struct BatchHandle : public pmhd_batch {
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
   }
+
+  int get_stats(int box, pmhd_stats* out) override {
+    if (check_box(box)) return PMHD_EINVAL;
+    std::memset(out, 0, sizeof(*out));
+    unsigned long long f[FLOOR_WORDS];
+    be.sync();
+    be.d2h(f, hctx[size_t(box)].fcnt, sizeof(f));
+    floor_stats(f, out);
+    double sums[9], maxes[2];
+    pmhd::diagnostics(be.exec(), hctx[size_t(box)], sums, maxes);
+    out->nonfinite = int64_t(sums[8]);
+    out->cycles = hclk[size_t(box)].done;
+    const char* e = be.error();
+    return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
 };
 
 inline int check_grid(const pmhd_grid* g, std::string& err) {
@@ -764,9 +893,10 @@ inline int check_grid(const pmhd_grid* g, std::string& err) {
 }
 
 // every launch of a single grid fits the HIP grid limits (blocks along y and z
-// below 65536) with the block shape the single-grid executor uses: 256 threads,
-// the smallest power of two >= n2 (at most 64) of them along the contiguous axis
-inline int check_launch_extents(const pmhd_grid* g, std::string& err) {
+// below 65536) with the block shape the single-grid executor uses: `threads`
+// threads (256; the adjoint's phase launches use 64), the smallest power of two
+// >= n2 (at most 64) of them along the contiguous axis
+inline int check_launch_extents(const pmhd_grid* g, std::string& err, int threads = 256) {
   Ctx<double> c;
   std::memset(&c, 0, sizeof(c));
   c.ndim = g->ndim;
@@ -779,7 +909,7 @@ inline int check_launch_extents(const pmhd_grid* g, std::string& err) {
       c.extent(kind, d, n);
       int64_t bx = 1;
       while (bx < 64 && bx < n[2]) bx <<= 1;
-      const int64_t by = 256 / bx;
+      const int64_t by = threads / bx > 0 ? threads / bx : 1;
       if (n[0] > 65535 || (n[1] + by - 1) / by > 65535) {
         err = "grid too large for one launch (cells along y, or along x per block row)";
         return PMHD_EINVAL;
@@ -861,11 +991,15 @@ inline int advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int6
                    double* t_out, int64_t* cycle_out, double* t2_out) {
   double t = t0, t2 = t0;
   int64_t cyc = cycle0;
+  bool finite = true;
   while (t < t_end && cyc < max_cycles) {
     t2 = t;
     double dt;
     int rc = h->compute_dt(&dt);
     if (rc) return rc;
+    // a non-finite state gives a NaN timestep; the reference still runs that cycle
+    // (its clock becomes NaN and its loop ends), and so does this loop
+    if (!(dt - dt == 0.0)) finite = false;
     dt = clip_dt(h->g.ndim, h->p.physics, t, t_end, dt);
     rc = h->step(dt);
     if (rc) return rc;
@@ -875,6 +1009,8 @@ inline int advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int6
   if (t_out) *t_out = t;
   if (cycle_out) *cycle_out = cyc;
   if (t2_out) *t2_out = t2;
+  if (!finite) return h->fail(PMHD_ENONFINITE, "the state became non-finite (NaN timestep); the results are those "
+                                               "of the reference's loop, which ends with a NaN time");
   return PMHD_OK;
 }
 
@@ -911,6 +1047,22 @@ PMHD_NS_END
     if (!h) return PMHD_EINVAL;                                                             \
     try { return h->get_state(which, prim, cons, fbx, fby, fbz); }                          \
     catch (...) { return h->fail(PMHD_EINVAL, "exception in pmhd_get_state"); }             \
+  }                                                                                         \
+  extern "C" int pmhd_set_state_device(pmhd_handle* h, const double* prim, const double* cons, \
+                                       const double* fbx, const double* fby, const double* fbz) { \
+    if (!h) return PMHD_EINVAL;                                                             \
+    try { return h->set_state_device(prim, cons, fbx, fby, fbz); }                          \
+    catch (...) { return h->fail(PMHD_EINVAL, "exception in pmhd_set_state_device"); }      \
+  }                                                                                         \
+  extern "C" int pmhd_get_state_device(pmhd_handle* h, int32_t which, double* prim,         \
+                                       double* cons, double* fbx, double* fby, double* fbz) { \
+    if (!h) return PMHD_EINVAL;                                                             \
+    try { return h->get_state_device(which, prim, cons, fbx, fby, fbz); }                   \
+    catch (...) { return h->fail(PMHD_EINVAL, "exception in pmhd_get_state_device"); }      \
+  }                                                                                         \
+  extern "C" int pmhd_set_stream(pmhd_handle* h, void* stream) {                            \
+    if (!h) return PMHD_EINVAL;                                                             \
+    return h->set_stream(stream);                                                           \
   }                                                                                         \
   extern "C" int pmhd_compute_dt(pmhd_handle* h, double* dt) {                              \
     if (!h || !dt) return PMHD_EINVAL;                                                      \
@@ -950,6 +1102,11 @@ PMHD_NS_END
                                   int32_t* n_entries) {                                     \
     if (!h || !names || !total_ms || !launches || !elements || !n_entries) return PMHD_EINVAL; \
     return h->get_profile(max_entries, names, total_ms, launches, elements, n_entries);     \
+  }                                                                                         \
+  extern "C" int pmhd_get_stats(pmhd_handle* h, pmhd_stats* out) {                          \
+    if (!h || !out) return PMHD_EINVAL;                                                     \
+    try { return h->get_stats(out); }                                                       \
+    catch (...) { return h->fail(PMHD_EINVAL, "exception in pmhd_get_stats"); }             \
   }                                                                                         \
   extern "C" const char* pmhd_last_error(const pmhd_handle* h) {                            \
     return h ? h->err.c_str() : g_pmhd_create_error.c_str();                                \
@@ -1012,6 +1169,11 @@ PMHD_NS_END
                                         int64_t* elements, int32_t* n_entries) {            \
     if (!b || !names || !total_ms || !launches || !elements || !n_entries) return PMHD_EINVAL; \
     return b->get_profile(max_entries, names, total_ms, launches, elements, n_entries);     \
+  }                                                                                         \
+  extern "C" int pmhd_batch_get_stats(pmhd_batch* b, int32_t box, pmhd_stats* out) {       \
+    if (!b || !out) return PMHD_EINVAL;                                                     \
+    try { return b->get_stats(box, out); }                                                  \
+    catch (...) { return b->fail(PMHD_EINVAL, "exception in pmhd_batch_get_stats"); }       \
   }                                                                                         \
   extern "C" const char* pmhd_batch_last_error(const pmhd_batch* b) {                       \
     return b ? b->err.c_str() : g_pmhd_create_error.c_str();                                \

@@ -147,6 +147,11 @@ template <class Ph, class T>
 __global__ void __launch_bounds__(256) k_phase_max(const Ctx<T> c, unsigned n1, unsigned n2, unsigned total,
                                                    unsigned long long* part) {
   const unsigned t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if constexpr (Ph::id == K_UPDATE) {
+    // the cycle's floor counts are complete (every flooring launch precedes the
+    // update on the stream): one thread folds them
+    if (blockIdx.x == 0 && threadIdx.x == 0 && c.fcnt) fold_floor_counts(c.fcnt);
+  }
   unsigned long long m[3] = {0ull, 0ull, 0ull};
   if (t < total) {
     const unsigned k = t % n2;
@@ -287,6 +292,10 @@ __global__ void __launch_bounds__(256) k_batch_max(const Ctx<T>* __restrict__ ct
     const TileRef r = tile_of(ctxs, prefix, nbox, vb, Ph::kind, Ph::dir, bk, bi);
     const Ctx<T>& c = ctxs[r.box];
     if (!c.active) continue;
+    if constexpr (Ph::id == K_UPDATE) {
+      // one thread of the box's first tile folds the box's floor counts
+      if (threadIdx.x == 0 && vb == prefix[r.box] && c.fcnt) fold_floor_counts(c.fcnt);
+    }
     unsigned long long m[3] = {0ull, 0ull, 0ull};
     if (r.in) {
       double o[3];
@@ -326,6 +335,35 @@ template <class T>
 __global__ void k_box_activate(Ctx<T>* ctxs, int nbox) {
   const int b = int(blockIdx.x * blockDim.x + threadIdx.x);
   if (b < nbox) ctxs[b].active = 1;
+}
+
+// ---------------------------------------------------------------------------
+// device float64 arrays of the reference layout <-> the state's storage type
+// (grid-stride loops; the FP32 face field as high and low float parts)
+// ---------------------------------------------------------------------------
+template <class D, class S>
+__global__ void __launch_bounds__(256) k_convert(D* __restrict__ dst, const S* __restrict__ src, int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) dst[i] = D(src[i]);
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_split(T* __restrict__ hi, T* __restrict__ lo, const double* __restrict__ src,
+                                               int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+    const double v = src[i];
+    const T h = T(v);
+    hi[i] = h;
+    lo[i] = T(v - double(h));
+  }
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_join(double* __restrict__ dst, const T* __restrict__ hi,
+                                              const T* __restrict__ lo, int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+    dst[i] = double(hi[i]) + double(lo[i]);
+}
+inline unsigned stride_blocks(int64_t n) {
+  const int64_t b = (n + 255) / 256;
+  return unsigned(b < 1 ? 1 : (b > 8192 ? 8192 : b));
 }
 
 // ---------------------------------------------------------------------------
@@ -767,6 +805,7 @@ struct HipBackend {
     ex.destroy_events();
     ex.destroy_side();
     if (ex.s) hipStreamSynchronize(ex.s);
+    if (own) ex.s = own;   // destroy the handle's own stream, never the caller's
     if (ex.red) hipFree(ex.red);
     if (ex.part) hipFree(ex.part);
     if (ex.parts) hipFree(ex.parts);
@@ -801,6 +840,46 @@ struct HipBackend {
   }
   void d2d(void* dst, const void* src, size_t bytes) {
     ex.note(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ex.s));
+  }
+  template <class T>
+  void dev_in(T* dst, const double* src, int64_t n) {
+    if (n <= 0) return;
+    if constexpr (sizeof(T) == sizeof(double)) {
+      d2d(dst, src, size_t(n) * sizeof(double));
+    } else {
+      hipLaunchKernelGGL((k_convert<T, double>), dim3(stride_blocks(n)), dim3(256), 0, ex.s, dst, src, n);
+      ex.note(hipGetLastError());
+    }
+  }
+  template <class T>
+  void dev_out(double* dst, const T* src, int64_t n) {
+    if (n <= 0) return;
+    if constexpr (sizeof(T) == sizeof(double)) {
+      d2d(dst, src, size_t(n) * sizeof(double));
+    } else {
+      hipLaunchKernelGGL((k_convert<double, T>), dim3(stride_blocks(n)), dim3(256), 0, ex.s, dst, src, n);
+      ex.note(hipGetLastError());
+    }
+  }
+  template <class T>
+  void dev_split(T* hi, T* lo, const double* src, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_split<T>), dim3(stride_blocks(n)), dim3(256), 0, ex.s, hi, lo, src, n);
+    ex.note(hipGetLastError());
+  }
+  template <class T>
+  void dev_join(double* dst, const T* hi, const T* lo, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_join<T>), dim3(stride_blocks(n)), dim3(256), 0, ex.s, dst, hi, lo, n);
+    ex.note(hipGetLastError());
+  }
+  // launch on the caller's stream (nullptr: the handle's own); the work queued so
+  // far is finished first
+  hipStream_t own = nullptr;
+  void set_stream(void* s) {
+    if (!own) own = ex.s;
+    ex.note(hipStreamSynchronize(ex.s));
+    ex.s = s ? static_cast<hipStream_t>(s) : own;
   }
   int* adj_flag = nullptr;
   template <class Ph, class T>

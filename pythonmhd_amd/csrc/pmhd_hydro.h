@@ -437,6 +437,19 @@ PMHD_FI void k_copy_states(const Ctx<T>& c, int d, int bc, I3 f) {
   store5(c.WR, NF, idx, wr);
 }
 
+// floors of a predictor state, counting the floored values at `site` (the call
+// sites of Source/EvolveFunctions_Hydro.py:286-288, 437-443, 675-685)
+template <class T>
+PMHD_FI void floors_counted5(const Ctx<T>& c, int site, T v[NV]) {
+  const bool lo0 = v[0] < c.min_dens;
+  const bool lo1 = v[4] < c.min_pres;
+  floors(v, c.min_dens, c.min_pres);
+  if (c.fcnt && (lo0 || lo1)) {
+    if (lo0) count_add(c.fcnt + FLOOR_CUR + 2 * site, 1ull);
+    if (lo1) count_add(c.fcnt + FLOOR_CUR + 2 * site + 1, 1ull);
+  }
+}
+
 // floors, conserved interface states (kept for the transverse update), Roe flux
 template <class T>
 PMHD_FI void k_predict(const Ctx<T>& c, int d, I3 f) {
@@ -445,8 +458,8 @@ PMHD_FI void k_predict(const Ctx<T>& c, int d, I3 f) {
   T wl[NV], wr[NV];
   load5(c.WL, NF, idx, wl);
   load5(c.WR, NF, idx, wr);
-  floors(wl, c.min_dens, c.min_pres);
-  floors(wr, c.min_dens, c.min_pres);
+  floors_counted5(c, 2 * d, wl);
+  floors_counted5(c, 2 * d + 1, wr);
   if (c.ndim >= 2) {
     T ul[NV], ur[NV];
     prim_to_cons(wl, c.gamma, d, ul);
@@ -553,9 +566,15 @@ struct Update {  // update, then the CFL speeds of the new state
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 p, double o[3]) const {
     hydro::k_update(c, p);
-    T t[3];
-    k_cfl(c, c.U2, c.fb2, p, t);
-    speeds_out(t, o);
+    if constexpr (std::is_floating_point<T>::value) {
+      T t[3];
+      k_cfl(c, c.U2, c.fb2, p, t);
+      speeds_out(t, o);
+    } else {
+      // the adjoint's dual evaluation: the speeds only pick the next dt, which the
+      // adjoint keeps fixed
+      o[0] = o[1] = o[2] = 0.0;
+    }
   }
 };
 

@@ -132,6 +132,10 @@ struct Ctx {
                      // of the first cell (7) and right edge state of the last cell (7),
                      // planes [v][line], for the end faces
   Ptr<T> E[3];
+  // floor counters (FLOOR_WORDS words; null: not counted): the values floored in
+  // the running cycle per call site and variable, their totals since the state was
+  // set, and the most one call site floored in one cycle
+  unsigned long long* fcnt;
   // element counts
   PMHD_FI int64_t ncell() const { return ny * nx * nz; }
   PMHD_FI int64_t n_along(int d) const { return d == 0 ? nx : (d == 1 ? ny : nz); }
@@ -177,6 +181,45 @@ struct Ctx {
 struct I3 {
   int64_t j, i, k;
 };
+
+// Floor call sites, in the order of the reference's applyFloors calls: the
+// predictor's left and right states along x, y, z (sites 0-5;
+// Source/EvolveFunctions_MHD.py:278-280, 513-519, 911-922), then the 2D post-CTU
+// left and right states along x and y (sites 6-9; :562-576).  Word 2*site counts
+// floored densities, 2*site+1 floored pressures (predictor) or energies (post-CTU).
+enum { FLOOR_SITES = 10, FLOOR_CUR = 0, FLOOR_TOT = 2 * FLOOR_SITES, FLOOR_MAX = 4 * FLOOR_SITES,
+       FLOOR_WORDS = 64 };
+
+PMHD_FI void count_add(unsigned long long* p, unsigned long long v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  atomicAdd(p, v);
+#else
+  __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
+#endif
+}
+
+// the end of a cycle's counting (run by one thread before the update of the cycle,
+// after every launch that floors): totals and per-cycle maxima, then a clear
+PMHD_FI void fold_floor_counts(unsigned long long* f) {
+  for (int k = 0; k < 2 * FLOOR_SITES; ++k) {
+    const unsigned long long v = f[FLOOR_CUR + k];
+    f[FLOOR_TOT + k] += v;
+    if (v > f[FLOOR_MAX + k]) f[FLOOR_MAX + k] = v;
+    f[FLOOR_CUR + k] = 0ull;
+  }
+}
+
+// apply_floors to a 7-variable state, counting the floored values at `site`
+template <class T>
+PMHD_FI void floors_counted(const Ctx<T>& c, int site, T v[7], T min_last) {
+  const bool lo0 = v[0] < c.min_dens;
+  const bool lo1 = v[6] < min_last;
+  apply_floors(v, c.min_dens, min_last);
+  if (c.fcnt && (lo0 || lo1)) {
+    if (lo0) count_add(c.fcnt + FLOOR_CUR + 2 * site, 1ull);
+    if (lo1) count_add(c.fcnt + FLOOR_CUR + 2 * site + 1, 1ull);
+  }
+}
 PMHD_FI int64_t co(const I3& p, int d) { return d == 0 ? p.i : (d == 1 ? p.j : p.k); }
 PMHD_FI I3 with(I3 p, int d, int64_t v) {
   if (d == 0) p.i = v; else if (d == 1) p.j = v; else p.k = v;
@@ -484,8 +527,8 @@ PMHD_FI void predict_face(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7]) {
   const int64_t NF = c.nface(d);
   const int64_t idx = c.fid(d, f.j, f.i, f.k);
   const int64_t e = co(f, d);
-  apply_floors(wl, c.min_dens, c.min_pres);
-  apply_floors(wr, c.min_dens, c.min_pres);
+  floors_counted(c, 2 * d, wl, c.min_pres);
+  floors_counted(c, 2 * d + 1, wr, c.min_pres);
   T bn;
   int ke = 0;
   if (c.ndim == 3) {
@@ -974,7 +1017,7 @@ PMHD_FI void ctu_states(const Ctx<T>& c, int d, I3 f, T ul[7], T ur[7]) {
       const T* s = side == 0 ? sl : sr;
 #pragma unroll
       for (int v = 0; v < 7; ++v) u[v] = u[v] + ((-h) * dr[v] + c.hdt * s[v]);
-      apply_floors(u, c.min_dens, c.min_energy);
+      floors_counted(c, 6 + 2 * d + side, u, c.min_energy);
     }
   }
 }

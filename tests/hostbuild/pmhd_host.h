@@ -69,6 +69,9 @@ struct HostExec {
   template <class Ph, class T>
   void run_max(const Ctx<T>& c) {
     for (int d = 0; d < 3; ++d) last[d] = 0;
+    if constexpr (Ph::id == K_UPDATE) {
+      if (c.fcnt) fold_floor_counts(c.fcnt);
+    }
     host_each_max<Ph>(c, last);
   }
   void max_partials(double out[3]) {
@@ -136,6 +139,12 @@ struct HostBatchExec {
       lay->tuned_ms[Ph::id] = 0.f;
     }
     (void)c0;
+    if constexpr (Ph::id == K_UPDATE) {
+      for (int b = 0; b < lay->nbox; ++b) {
+        const Ctx<T>& c = ctxs<T>()[b];
+        if (c.active && c.fcnt) fold_floor_counts(c.fcnt);
+      }
+    }
     walk<Ph, T, true>();
   }
 
@@ -233,6 +242,27 @@ struct HostBackend {
   void d2h(void* d, const void* s, size_t b) { std::memcpy(d, s, b); }
   void zero(void* d, size_t b) { std::memset(d, 0, b); }
   void d2d(void* d, const void* s, size_t b) { std::memcpy(d, s, b); }
+  // "device" float64 arrays of the device entry points are host memory here
+  template <class T>
+  void dev_in(T* dst, const double* src, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) dst[i] = T(src[i]);
+  }
+  template <class T>
+  void dev_out(double* dst, const T* src, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) dst[i] = double(src[i]);
+  }
+  template <class T>
+  void dev_split(T* hi, T* lo, const double* src, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+      hi[i] = T(src[i]);
+      lo[i] = T(src[i] - double(hi[i]));
+    }
+  }
+  template <class T>
+  void dev_join(double* dst, const T* hi, const T* lo, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) dst[i] = double(hi[i]) + double(lo[i]);
+  }
+  void set_stream(void*) {}
   bool overflow = false;
   template <class Ph, class T>
   void adjoint_phase(const Ctx<T>& c, const Ctx<double>& a) {

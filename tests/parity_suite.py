@@ -668,3 +668,113 @@ def test_config_kh_batch(lib):
             same(a, e)
         same(t[k], ref[3])
     b.close()
+
+
+# ---------------------------------------------------------------------------
+# floors in their active regime: the reference's floor-firing trajectories
+# (tests/golden/floors.npz), bit for bit in FP64, with the device floor counters
+# against the number of values each of the reference's applyFloors calls floored
+# ---------------------------------------------------------------------------
+from tests.test_oracle_floors import HYDRO_CASES as FLOOR_HYDRO, MHD_CASES as FLOOR_MHD  # noqa: E402
+from tests.test_oracle_floors import golden as floor_golden, mhd_expected, mhd_inputs  # noqa: E402
+from tests.test_oracle_floors import widths_of as floor_widths  # noqa: E402
+
+
+def _check_floor_counts(st, counts):
+    """Device counters against the fixture's per-cycle, per-site counts: totals
+    summed over the cycles, and the most one call floored in one cycle."""
+    want_tot = counts.sum(axis=0)
+    want_max = counts.max(axis=0) if len(counts) else np.zeros_like(want_tot)
+    assert np.array_equal(st['floored'], want_tot), (st['floored'].tolist(), want_tot.tolist())
+    assert np.array_equal(st['floored_max'], want_max), (st['floored_max'].tolist(), want_max.tolist())
+    assert st['floored_total'] == int(want_tot.sum())
+
+
+@pytest.mark.parametrize('tag,nd,gamma,cfl,order,efix,bcs,t_end,ncyc', FLOOR_MHD)
+def test_floors_mhd_golden(lib, tag, nd, gamma, cfl, order, efix, bcs, t_end, ncyc):
+    d = floor_golden()
+    prim, cons, faces = mhd_inputs(d, tag, nd)
+    if cons is None:
+        cons = M.prim_to_cons_all(prim, gamma)
+    floors = [float(x) for x in d[tag + '_floors']]
+    shape = prim.shape[1:]
+    print('case: floors MHD', tag, flush=True)
+    s = Solver()(shape, floor_widths(shape), bcs, gamma, cfl, *floors, order=order, entropy_fix=efix, lib=lib)
+    s.set_state(prim, cons, faces)
+    t, cyc, t2 = s.advance(0.0, t_end, 0, ncyc)
+    p, c, f = s.get_state(0)
+    p2, c2, f2 = s.get_state(1)
+    st = s.stats()
+    s.close()
+    want = mhd_expected(d, tag, nd)
+    for a, b in ((p, want[0]), (c, want[1]), (t, want[3]), (cyc, want[4]), (p2, want[5]), (c2, want[6]),
+                 (t2, want[8])):
+        same(a, b)
+    for a, b in zip(f, want[2]):
+        same(a, b)
+    for a, b in zip(f2, want[7]):
+        same(a, b)
+    assert st['cycles'] == cyc and st['nonfinite'] == 0
+    _check_floor_counts(st, d[tag + '_counts'])
+
+
+@pytest.mark.parametrize('tag,gamma,cfl,order,efix,bcs,t_end,ncyc', FLOOR_HYDRO)
+def test_floors_hydro_golden(lib, tag, gamma, cfl, order, efix, bcs, t_end, ncyc):
+    from pythonmhd_amd._lib import HYDRO
+    d = floor_golden()
+    p0 = d[tag + '_in']
+    floors = [float(x) for x in d[tag + '_floors']]
+    shape = p0.shape[1:]
+    print('case: floors hydro', tag, flush=True)
+    s = Solver()(shape, floor_widths(shape), bcs, gamma, cfl, *floors, order=order, entropy_fix=efix, lib=lib,
+                 physics=HYDRO)
+    s.set_state(p0, H.prim_to_cons(p0, gamma, 0))
+    t, cyc, t2 = s.advance(0.0, t_end, 0, ncyc)
+    p, c, _ = s.get_state(0)
+    p2, c2, _ = s.get_state(1)
+    st = s.stats()
+    s.close()
+    for a, k in ((p, 'prim'), (c, 'cons'), (t, 't'), (cyc, 'cyc'), (p2, 'prim2'), (c2, 'cons2'), (t2, 't2')):
+        same(a, d[tag + '_' + k])
+    _check_floor_counts(st, d[tag + '_counts'])
+
+
+@pytest.mark.parametrize('tag', ['m2o2', 'm3o2', 'm1o2'])
+def test_floors_fp32_against_golden(lib, tag):
+    """The FP32-bulk step on the floor-firing trajectories: density L1 within 1e-5
+    of the reference's FP64 result (relative to the mean density)."""
+    case = next(c for c in FLOOR_MHD if c[0] == tag)
+    _, nd, gamma, cfl, order, efix, bcs, t_end, ncyc = case
+    d = floor_golden()
+    prim, cons, faces = mhd_inputs(d, tag, nd)
+    if cons is None:
+        cons = M.prim_to_cons_all(prim, gamma)
+    floors = [float(x) for x in d[tag + '_floors']]
+    shape = prim.shape[1:]
+    print('case: floors fp32', tag, flush=True)
+    s = Solver()(shape, floor_widths(shape), bcs, gamma, cfl, *floors, order=order, entropy_fix=efix, lib=lib,
+                 precision=32)
+    s.set_state(prim, cons, faces)
+    s.advance(0.0, t_end, 0, ncyc)
+    _, c, _ = s.get_state(0)
+    s.close()
+    want = mhd_expected(d, tag, nd)[1]
+    l1 = float(np.mean(np.abs(c[0] - want[0])) / np.mean(np.abs(want[0])))
+    print('fp32 floors %s: L1(rho) = %.3e' % (tag, l1))
+    assert l1 < 1e-5
+
+
+def test_floors_reference_indexerror_case_is_flagged(lib):
+    """The 3D state on which the reference's applyFloors raises IndexError (three
+    floored pressures in one call): the counters show a per-call maximum of three or
+    more at a 3D site, which the drop-in front-end turns into the same IndexError."""
+    d = floor_golden()
+    names = ('prim', 'cons', 'fbx', 'fby', 'fbz')
+    prim, cons, fbx, fby, fbz = (d['m3raise_in_' + n] for n in names)
+    floors = [float(x) for x in d['m3raise_floors']]
+    s = Solver()((8, 8, 8), (1 / 8,) * 3, (1, 1, 1), GAMMA, 0.4, *floors, order=0, entropy_fix=True, lib=lib)
+    s.set_state(prim, cons, (fbx, fby, fbz))
+    s.advance(0.0, 10.0, 0, 1)
+    st = s.stats()
+    s.close()
+    assert st['floored_max'][:6].max() >= 3

@@ -44,3 +44,31 @@ def test_autograd_function_matches_adjoint():
     want = gp[0] + gc[0] + v[0] * gc[1] + v[1] * gc[2] + v[2] * gc[3] + 0.5 * (v * v).sum(0) * gc[7]
     assert np.allclose(rho.grad.numpy(), want, rtol=1e-12, atol=1e-15)
     run.close()
+
+
+def test_reused_run_raises_in_backward():
+    """Two fixed_dt_steps calls on one AdjointRun in one graph: the backward of the
+    first raises (its checkpoints were overwritten by the second forward)."""
+    from pythonmhd_amd._lib import load
+    from pythonmhd_amd.adjoint import AdjointRun, fixed_dt_steps
+    lib = load(HOST_LIB)
+    shape = (6, 8)
+    prim, faces = _generic_state(np.random.default_rng(3), shape)
+    cons = M.prim_to_cons_all(prim, GAMMA)
+    run = AdjointRun(shape, widths_of(shape), (1, 1), GAMMA, order=1, precision=64, checkpoint_every=2, lib=lib)
+    p = torch.tensor(prim, requires_grad=True)
+    c = torch.tensor(cons)
+    f = [torch.tensor(x) for x in faces]
+    o1 = fixed_dt_steps(run, 0.004, 2, p, c, *f)
+    o2 = fixed_dt_steps(run, 0.004, 2, p * 1.0, c, *f)
+    ke = lambda u: (0.5 * (u[1] ** 2 + u[2] ** 2 + u[3] ** 2) / u[0]).sum()  # noqa: E731
+    with pytest.raises(RuntimeError):
+        (ke(o1[1]) + ke(o2[1])).backward()
+    run.close()
+
+
+def test_adjoint_rejects_1d_mhd():
+    from pythonmhd_amd._lib import PmhdError, load
+    from pythonmhd_amd.adjoint import AdjointRun
+    with pytest.raises(PmhdError):
+        AdjointRun((16,), (1 / 16,), (1,), GAMMA, order=1, precision=64, lib=load(HOST_LIB))
