@@ -91,6 +91,25 @@ int pmhd_set_stream(pmhd_handle* h, void* stream);
  * Source/TimestepCalculations_HelperModule.py:183-361). */
 int pmhd_compute_dt(pmhd_handle* h, double* dt);
 
+/* pmhd_compute_dt without a handle and without allocating: the CFL timestep
+ * (calcDt_{1,2,3}D_mhd / _hydro, Source/TimestepCalculations_HelperModule.py:24-361)
+ * of a state held in the caller's DEVICE float64 arrays in the reference layout
+ * (cons and the faces the grid has), written to the caller's device double *dt.
+ * No counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops).  It calls no hipMalloc,
+ * hipFree or allocating routine and does not synchronise: the two launches are
+ * queued on `stream` (NULL: the default stream).  Its only device memory beyond
+ * the inputs and *dt is a 24 KiB scratch array (1024 partial maxima) that each GPU
+ * the library runs on holds until the process ends; every call on one GPU shares
+ * it, so calls that overlap on one GPU race.  Computes in FP64 (params->precision
+ * must be 64) and equals pmhd_compute_dt of a precision-64 handle on the same
+ * state bit for bit.  Grids pmhd_create rejects (beyond 2^24 cells along an axis
+ * or 2^29 cells with ghosts) are rejected with PMHD_EINVAL; the message is
+ * pmhd_last_error(NULL). */
+int pmhd_cfl_dt_noalloc(const pmhd_grid* grid, const pmhd_params* params, const double* cons,
+                        const double* fbx, const double* fby, const double* fbz, double* dt,
+                        void* stream);
+
 /* One cycle with the given dt (the body of the evolve loop). */
 int pmhd_step(pmhd_handle* h, double dt);
 
@@ -162,7 +181,8 @@ int pmhd_set_profiling(pmhd_handle* h, int32_t enable);
 int pmhd_get_profile(pmhd_handle* h, int32_t max_entries, char* names, double* total_ms,
                      int64_t* launches, int64_t* elements, int32_t* n_entries);
 
-/* Last error message of the handle (or of the last failed pmhd_create when h is NULL). */
+/* Last error message of the handle (or, when h is NULL, of the last failed
+ * pmhd_create, pmhd_batch_create or pmhd_cfl_dt_noalloc). */
 const char* pmhd_last_error(const pmhd_handle* h);
 
 /* ------------------------------------------------------------------------

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_LIB = os.path.join(_HERE, 'libpmhd.so')
 
 EXPORTS = ('pmhd_create', 'pmhd_destroy', 'pmhd_set_state', 'pmhd_get_state', 'pmhd_set_state_device',
-           'pmhd_get_state_device', 'pmhd_set_stream', 'pmhd_compute_dt',
+           'pmhd_get_state_device', 'pmhd_set_stream', 'pmhd_cfl_dt_noalloc', 'pmhd_compute_dt',
            'pmhd_step', 'pmhd_set_sweep', 'pmhd_advance', 'pmhd_synchronize', 'pmhd_diagnostics',
            'pmhd_set_profiling',
            'pmhd_get_profile', 'pmhd_get_stats', 'pmhd_last_error',
@@ -82,6 +82,7 @@ def _declare(lib):
     lib.pmhd_set_state_device.argtypes = [H, P, P, P, P, P]
     lib.pmhd_get_state_device.argtypes = [H, ctypes.c_int32, P, P, P, P, P]
     lib.pmhd_set_stream.argtypes = [H, P]
+    lib.pmhd_cfl_dt_noalloc.argtypes = [ctypes.POINTER(PmhdGrid), ctypes.POINTER(PmhdParams), P, P, P, P, P, P]
     lib.pmhd_compute_dt.argtypes = [H, ctypes.POINTER(ctypes.c_double)]
     lib.pmhd_step.argtypes = [H, ctypes.c_double]
     lib.pmhd_set_sweep.argtypes = [H, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
@@ -248,6 +249,29 @@ def _state_args(shape, prim, cons, faces, physics=MHD):
             raise ValueError('face shape %s, expected %s' % (f.shape, s))
     keep = [prim, cons] + fs
     return keep, [_ptr(prim), _ptr(cons)] + [_ptr(f) for f in fs] + [None] * (3 - len(fs))
+
+
+def cfl_dt_noalloc(shape, widths, bcs, gamma, cfl, cons, faces=(), dt_out=None, physics=MHD, stream=None,
+                   lib=None):
+    """pmhd_cfl_dt_noalloc: the CFL timestep of a state in float64 tensors (GPU
+    tensors for the HIP library, CPU tensors for the host test build) into the
+    one-element float64 tensor dt_out (allocated here when None), without any
+    allocation inside the library.  Queued on ``stream`` (a hipStream_t integer;
+    None: the default stream); returns dt_out without synchronising."""
+    import torch
+    lib = lib if lib is not None else load()
+    g = make_grid(tuple(int(s) for s in shape), widths, bcs)
+    p = make_params(gamma, cfl, precision=64, physics=physics)
+    gpu = on_gpu(lib)
+    if dt_out is None:
+        dt_out = torch.zeros(1, dtype=torch.float64, device='cuda' if gpu else 'cpu')
+    fp = [_dptr(f, None, 'face array', gpu) for f in faces] + [None] * (3 - len(faces))
+    rc = lib.pmhd_cfl_dt_noalloc(ctypes.byref(g), ctypes.byref(p), _dptr(cons, None, 'cons', gpu), *fp,
+                                 _dptr(dt_out, (1,), 'dt_out', gpu),
+                                 None if stream is None else ctypes.c_void_p(int(stream)))
+    if rc != 0:
+        raise PmhdError('pmhd_cfl_dt_noalloc failed (%d): %s' % (rc, lib.pmhd_last_error(None).decode()), rc)
+    return dt_out
 
 
 class Solver:

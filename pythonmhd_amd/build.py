@@ -36,9 +36,11 @@ IEEE = ['-ffp-contract=off']
 FAST32 = ['-ffp-contract=fast', '-fno-hip-fp32-correctly-rounded-divide-sqrt', '-fno-signed-zeros',
           '-fgpu-flush-denormals-to-zero']
 
-GPU_UNITS = [('pmhd_hip_f64.hip', IEEE), ('pmhd_hip_f32.hip', FAST32), ('pmhd_hip_adj.hip', IEEE)]
+GPU_UNITS = [('pmhd_hip_f64.hip', IEEE), ('pmhd_hip_f32.hip', FAST32), ('pmhd_hip_adj.hip', IEEE),
+             ('pmhd_noalloc.hip', IEEE)]
 HOST_UNITS = [(os.path.join(HOST_DIR, 'pmhd_host_f64.cpp'), []), (os.path.join(HOST_DIR, 'pmhd_host_f32.cpp'), []),
-              (os.path.join(HOST_DIR, 'pmhd_host_adj.cpp'), []), (os.path.join(CSRC, 'pmhd_abi.cpp'), [])]
+              (os.path.join(HOST_DIR, 'pmhd_host_adj.cpp'), []), (os.path.join(HOST_DIR, 'pmhd_host_noalloc.cpp'), []),
+              (os.path.join(CSRC, 'pmhd_abi.cpp'), [])]
 GXX_COMMON = ['-O2', '-std=c++17', '-fPIC', '-ffp-contract=off', '-Wno-unknown-pragmas']
 
 

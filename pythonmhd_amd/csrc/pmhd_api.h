@@ -43,6 +43,11 @@ struct pmhd_handle {
   }
 };
 
+// The message pmhd_last_error(NULL) returns after a failed call without a handle
+// (pmhd_cfl_dt_noalloc): a string literal, recorded without allocating.  Defined
+// with the C ABI.
+void pmhd_note_null_error(const char* literal);
+
 // Constructors of the handles of one precision (FP64 or FP32 bulk), each defined
 // in the translation unit compiled for that precision; the result still needs
 // init().  nullptr when out of host memory.
@@ -867,28 +872,32 @@ struct BatchHandle : public pmhd_batch {
   }
 };
 
-inline int check_grid(const pmhd_grid* g, std::string& err) {
-  if (g->ndim < 1 || g->ndim > 3) { err = "ndim must be 1, 2 or 3"; return PMHD_EINVAL; }
-  if (g->nx < 1 || (g->ndim >= 2 && g->ny < 1) || (g->ndim == 3 && g->nz < 1)) {
-    err = "number of cells must be positive"; return PMHD_EINVAL;
-  }
-  if (g->nx > (1 << 24) || g->ny > (1 << 24) || g->nz > (1 << 24)) {
-    err = "number of cells along an axis must not exceed 2^24"; return PMHD_EINVAL;
-  }
+// the first problem of a grid, as a string literal (nullptr: none)
+inline const char* grid_problem(const pmhd_grid* g) {
+  if (g->ndim < 1 || g->ndim > 3) return "ndim must be 1, 2 or 3";
+  if (g->nx < 1 || (g->ndim >= 2 && g->ny < 1) || (g->ndim == 3 && g->nz < 1)) return "number of cells must be positive";
+  if (g->nx > (1 << 24) || g->ny > (1 << 24) || g->nz > (1 << 24))
+    return "number of cells along an axis must not exceed 2^24";
   {
     // every variable plane of a state, work or edge array stays below 4 GiB: the
     // kernels form byte offsets within a plane in 32 bits
     const int64_t ext = (g->nx + 2) * (g->ndim >= 2 ? g->ny + 2 : 1) * (g->ndim == 3 ? g->nz + 2 : 1);
-    if (ext > (int64_t(1) << 29)) {
-      err = "grid too large: (nx+2)(ny+2)(nz+2) must not exceed 2^29 cells"; return PMHD_EINVAL;
-    }
+    if (ext > (int64_t(1) << 29)) return "grid too large: (nx+2)(ny+2)(nz+2) must not exceed 2^29 cells";
   }
-  if (!(g->dx > 0) || (g->ndim >= 2 && !(g->dy > 0)) || (g->ndim == 3 && !(g->dz > 0))) {
-    err = "cell widths must be positive"; return PMHD_EINVAL;
-  }
+  if (!(g->dx > 0) || (g->ndim >= 2 && !(g->dy > 0)) || (g->ndim == 3 && !(g->dz > 0)))
+    return "cell widths must be positive";
   const int bcs[3] = {g->bcx, g->bcy, g->bcz};
   for (int d = 0; d < g->ndim; ++d)
-    if (bcs[d] != 0 && bcs[d] != 1) { err = "boundary condition must be 0 (outflow) or 1 (periodic)"; return PMHD_EINVAL; }
+    if (bcs[d] != 0 && bcs[d] != 1) return "boundary condition must be 0 (outflow) or 1 (periodic)";
+  return nullptr;
+}
+
+inline int check_grid(const pmhd_grid* g, std::string& err) {
+  const char* m = grid_problem(g);
+  if (m) {
+    err = m;
+    return PMHD_EINVAL;
+  }
   return PMHD_OK;
 }
 
@@ -918,12 +927,19 @@ inline int check_launch_extents(const pmhd_grid* g, std::string& err, int thread
   return PMHD_OK;
 }
 
+inline const char* params_problem(const pmhd_params* p) {
+  if (p->reconstruct_order < 0 || p->reconstruct_order > 2) return "reconstruct_order must be 0, 1 or 2";
+  if (p->precision != 64 && p->precision != 32) return "precision must be 64 or 32";
+  if (p->physics != 0 && p->physics != 1) return "physics must be 0 (MHD) or 1 (hydrodynamics)";
+  return nullptr;
+}
+
 inline int check_params(const pmhd_params* p, std::string& err) {
-  if (p->reconstruct_order < 0 || p->reconstruct_order > 2) {
-    err = "reconstruct_order must be 0, 1 or 2"; return PMHD_EINVAL;
+  const char* m = params_problem(p);
+  if (m) {
+    err = m;
+    return PMHD_EINVAL;
   }
-  if (p->precision != 64 && p->precision != 32) { err = "precision must be 64 or 32"; return PMHD_EINVAL; }
-  if (p->physics != 0 && p->physics != 1) { err = "physics must be 0 (MHD) or 1 (hydrodynamics)"; return PMHD_EINVAL; }
   return PMHD_OK;
 }
 
@@ -1024,7 +1040,10 @@ PMHD_NS_END
 // C ABI bodies (one translation unit; the handles come from the constructors above).
 #define PMHD_DEFINE_C_ABI()                                                                 \
   static std::string g_pmhd_create_error;                                                   \
+  static const char* g_pmhd_null_error = nullptr;                                           \
+  void pmhd_note_null_error(const char* literal) { g_pmhd_null_error = literal; }           \
   extern "C" int pmhd_create(const pmhd_grid* g, const pmhd_params* p, pmhd_handle** out) { \
+    g_pmhd_null_error = nullptr;                                                            \
     try {                                                                                   \
       return pmhd::create_handle(g, p, out, g_pmhd_create_error);                           \
     } catch (...) {                                                                         \
@@ -1109,10 +1128,12 @@ PMHD_NS_END
     catch (...) { return h->fail(PMHD_EINVAL, "exception in pmhd_get_stats"); }             \
   }                                                                                         \
   extern "C" const char* pmhd_last_error(const pmhd_handle* h) {                            \
-    return h ? h->err.c_str() : g_pmhd_create_error.c_str();                                \
+    if (h) return h->err.c_str();                                                           \
+    return g_pmhd_null_error ? g_pmhd_null_error : g_pmhd_create_error.c_str();             \
   }                                                                                         \
   extern "C" int pmhd_batch_create(int32_t nbox, const pmhd_grid* grids,                    \
                                    const pmhd_params* p, pmhd_batch** out) {                \
+    g_pmhd_null_error = nullptr;                                                            \
     try {                                                                                   \
       return pmhd::create_batch(nbox, grids, p, out, g_pmhd_create_error);                  \
     } catch (...) {                                                                         \
@@ -1176,5 +1197,6 @@ PMHD_NS_END
     catch (...) { return b->fail(PMHD_EINVAL, "exception in pmhd_batch_get_stats"); }       \
   }                                                                                         \
   extern "C" const char* pmhd_batch_last_error(const pmhd_batch* b) {                       \
-    return b ? b->err.c_str() : g_pmhd_create_error.c_str();                                \
+    if (b) return b->err.c_str();                                                           \
+    return g_pmhd_null_error ? g_pmhd_null_error : g_pmhd_create_error.c_str();             \
   }
