@@ -7,8 +7,12 @@ Each function reads the duck-typed grid attributes the reference reads
 tStart until tEnd or maxCycles with the reference's clipping of the last dt, and
 returns the final state, the time and cycle reached, and the second-last state.
 
-Extension: ``reconstructOrder=1`` selects PLM (absent from the reference) and the
-keyword ``precision=32`` runs the FP32-bulk step.
+Extensions (keywords the reference does not have): ``reconstructOrder=1``
+selects PLM (absent from the reference); ``precision=32`` runs the FP32-bulk step;
+``printCycles=True`` prints the reference's per-cycle line "Cycle = .. Time = ..
+Dt = .." (Source/EvolveFunctions_MHD.py:317, 692, 1152; the loop then runs cycle by
+cycle from the host, with the same results); ``riemannSolver`` picks the flux
+(C.ROE, the reference's).
 """
 import numpy as np
 
@@ -16,6 +20,9 @@ from . import constants as C
 from ._lib import HYDRO, MHD, Solver
 
 _solver_cache = {}
+
+# values of the riemannSolver keyword / simPar key
+RIEMANN_SOLVERS = (C.ROE,)
 
 
 def _solver(key, make):
@@ -48,8 +55,30 @@ def _check_common(tStart, tEnd, startCycle):
     assert startCycle >= 0
 
 
+def _loop_printing(s, ndim, physics, tStart, tEnd, startCycle, maxCycles):
+    """The evolve loop driven from the host, printing each cycle as the reference
+    does; the same sequence of timesteps and clipping as pmhd_advance."""
+    t, cyc, t2 = tStart, startCycle, tStart
+    while t < tEnd and cyc < maxCycles:
+        t2 = t
+        dt = s.compute_dt()
+        if ndim == 3 and physics == MHD:
+            if tEnd - t < dt:
+                dt = tEnd - t
+        elif t + dt > tEnd:
+            dt = tEnd - t
+        s.step(dt)
+        t += dt
+        cyc += 1
+        print("Cycle = " + str(cyc) + " Time = " + str(t) + " Dt = " + str(dt))
+    return t, cyc, t2
+
+
 def _run(shape, widths, bcs, prim, cons, faces, tStart, tEnd, startCycle, maxCycles, gamma, cfl,
-         minDens, minPres, minEnergy, reconstructOrder, entropyFix, epsilon, precision, physics=MHD):
+         minDens, minPres, minEnergy, reconstructOrder, entropyFix, epsilon, precision, physics=MHD,
+         printCycles=False, riemannSolver=C.ROE):
+    if riemannSolver not in RIEMANN_SOLVERS:
+        raise RuntimeError('riemannSolver must be one of %s' % (RIEMANN_SOLVERS,))
     key = (shape, tuple(widths), tuple(bcs), float(gamma), float(cfl), float(minDens), float(minPres),
            float(minEnergy), int(reconstructOrder), bool(entropyFix), float(epsilon), int(precision),
            int(physics))
@@ -58,7 +87,10 @@ def _run(shape, widths, bcs, prim, cons, faces, tStart, tEnd, startCycle, maxCyc
     s.set_state(prim, cons, faces)
     # a state that turns non-finite ends the loop with a NaN clock, as the
     # reference's does; the results are returned all the same
-    t, cyc, t2 = s.advance(tStart, tEnd, startCycle, maxCycles, allow_nonfinite=True)
+    if printCycles:
+        t, cyc, t2 = _loop_printing(s, len(shape), physics, tStart, tEnd, startCycle, maxCycles)
+    else:
+        t, cyc, t2 = s.advance(tStart, tEnd, startCycle, maxCycles, allow_nonfinite=True)
     p, c, f = s.get_state(0)
     p2, c2, f2 = s.get_state(1)
     return p, c, f, t, cyc, p2, c2, f2, t2
@@ -66,7 +98,7 @@ def _run(shape, widths, bcs, prim, cons, faces, tStart, tEnd, startCycle, maxCyc
 
 def evolve_1D_mhd(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
                   minEnergy, reconstructOrder=0, entropyFix=False, epsilon=0.5,
-                  precision=C.DEFAULT_PRECISION):
+                  precision=C.DEFAULT_PRECISION, printCycles=False, riemannSolver=C.ROE):
     prim = np.asarray(simulationGrid.primVars, dtype=np.float64)
     assert prim.ndim == 2 and prim.shape[0] == 8 and prim.shape[1] > 0
     cons = np.asarray(simulationGrid.consVars, dtype=np.float64)
@@ -78,13 +110,14 @@ def evolve_1D_mhd(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cf
     _check_common(tStart, tEnd, startCycle)
     p, c, _, t, cyc, p2, c2, _, t2 = _run(
         (prim.shape[1],), (dx,), (bcx,), prim, cons, (), tStart, tEnd, startCycle, maxCycles, gamma, cfl,
-        minDens, minPres, minEnergy, reconstructOrder, entropyFix, epsilon, precision)
+        minDens, minPres, minEnergy, reconstructOrder, entropyFix, epsilon, precision,
+        printCycles=printCycles, riemannSolver=riemannSolver)
     return p, c, t, cyc, p2, c2, t2
 
 
 def evolve_2D_mhd(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
                   minEnergy, reconstructOrder=0, entropyFix=False, epsilon=0.5,
-                  precision=C.DEFAULT_PRECISION):
+                  precision=C.DEFAULT_PRECISION, printCycles=False, riemannSolver=C.ROE):
     prim = np.asarray(simulationGrid.primVars, dtype=np.float64)
     bcx, bcy = simulationGrid.BcX, simulationGrid.BcY
     assert bcx == C.OUTFLOW or bcx == C.PERIODIC
@@ -99,13 +132,14 @@ def evolve_2D_mhd(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cf
     faces = (simulationGrid.faceBx, simulationGrid.faceBy)
     p, c, f, t, cyc, p2, c2, f2, t2 = _run(
         prim.shape[1:], (dx, dy), (bcx, bcy), prim, cons, faces, tStart, tEnd, startCycle, maxCycles,
-        gamma, cfl, minDens, minPres, minEnergy, reconstructOrder, entropyFix, epsilon, precision)
+        gamma, cfl, minDens, minPres, minEnergy, reconstructOrder, entropyFix, epsilon, precision,
+        printCycles=printCycles, riemannSolver=riemannSolver)
     return p, c, f[0], f[1], t, cyc, p2, c2, f2[0], f2[1], t2
 
 
 def evolve_3D_mhd(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
                   minEnergy, reconstructOrder=0, entropyFix=True, epsilon=0.5,
-                  precision=C.DEFAULT_PRECISION):
+                  precision=C.DEFAULT_PRECISION, printCycles=False, riemannSolver=C.ROE):
     prim = np.asarray(simulationGrid.primVars, dtype=np.float64)
     bcx, bcy, bcz = simulationGrid.BcX, simulationGrid.BcY, simulationGrid.BcZ
     for b in (bcx, bcy, bcz):
@@ -122,7 +156,7 @@ def evolve_3D_mhd(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cf
     p, c, f, t, cyc, p2, c2, f2, t2 = _run(
         prim.shape[1:], (dx, dy, dz), (bcx, bcy, bcz), prim, cons, faces, tStart, tEnd, startCycle,
         maxCycles, gamma, cfl, minDens, minPres, minEnergy, reconstructOrder, entropyFix, epsilon,
-        precision)
+        precision, printCycles=printCycles, riemannSolver=riemannSolver)
     return p, c, f[0], f[1], f[2], t, cyc, p2, c2, f2[0], f2[1], f2[2], t2
 
 
@@ -131,7 +165,7 @@ def evolve_3D_mhd(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cf
 # same tuple in 1D, 2D and 3D (Source/EvolveFunctions_Hydro.py:305, 516, 806)
 # ---------------------------------------------------------------------------
 def _hydro(simulationGrid, nd, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
-           minEnergy, reconstructOrder, entropyFix, epsilon, precision):
+           minEnergy, reconstructOrder, entropyFix, epsilon, precision, printCycles=False, riemannSolver=C.ROE):
     prim = np.asarray(simulationGrid.primVars, dtype=np.float64)
     assert prim.ndim == nd + 1 and prim.shape[0] == 5
     cons = np.asarray(simulationGrid.consVars, dtype=np.float64)
@@ -155,26 +189,27 @@ def _hydro(simulationGrid, nd, tStart, tEnd, startCycle, maxCycles, gamma, cfl, 
     shape = prim.shape[1:]
     p, c, _, t, cyc, p2, c2, _, t2 = _run(
         shape, widths, bcs, prim, cons, (), tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens,
-        minPres, minEnergy, reconstructOrder, entropyFix, epsilon, precision, physics=HYDRO)
+        minPres, minEnergy, reconstructOrder, entropyFix, epsilon, precision, physics=HYDRO,
+        printCycles=printCycles, riemannSolver=riemannSolver)
     return p, c, t, cyc, p2, c2, t2
 
 
 def evolve_1D_hydro(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
                     minEnergy, reconstructOrder=0, entropyFix=False, epsilon=0.5,
-                    precision=C.DEFAULT_PRECISION):
+                    precision=C.DEFAULT_PRECISION, printCycles=False, riemannSolver=C.ROE):
     return _hydro(simulationGrid, 1, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
-                  minEnergy, reconstructOrder, entropyFix, epsilon, precision)
+                  minEnergy, reconstructOrder, entropyFix, epsilon, precision, printCycles, riemannSolver)
 
 
 def evolve_2D_hydro(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
                     minEnergy, reconstructOrder=0, entropyFix=False, epsilon=0.5,
-                    precision=C.DEFAULT_PRECISION):
+                    precision=C.DEFAULT_PRECISION, printCycles=False, riemannSolver=C.ROE):
     return _hydro(simulationGrid, 2, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
-                  minEnergy, reconstructOrder, entropyFix, epsilon, precision)
+                  minEnergy, reconstructOrder, entropyFix, epsilon, precision, printCycles, riemannSolver)
 
 
 def evolve_3D_hydro(simulationGrid, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
                     minEnergy, reconstructOrder=0, entropyFix=False, epsilon=0.5,
-                    precision=C.DEFAULT_PRECISION):
+                    precision=C.DEFAULT_PRECISION, printCycles=False, riemannSolver=C.ROE):
     return _hydro(simulationGrid, 3, tStart, tEnd, startCycle, maxCycles, gamma, cfl, minDens, minPres,
-                  minEnergy, reconstructOrder, entropyFix, epsilon, precision)
+                  minEnergy, reconstructOrder, entropyFix, epsilon, precision, printCycles, riemannSolver)
