@@ -48,7 +48,33 @@ typedef struct pmhd_params {
                                             1 hydrodynamics (evolve_*D_hydro,
                                             Source/EvolveFunctions_Hydro.py:232, 350, 559):
                                             cells (5, ...) = (rho, v/M, p/E), no faces */
+  int32_t riemann;                       /* PMHD_RIEMANN_*: the flux of MHD faces (hydro: Roe only) */
+  int32_t fix_quirks;                    /* PMHD_FIX_* bits: reference behaviours to correct instead
+                                            of reproduce (0, the default, reproduces them all) */
 } pmhd_params;
+
+/* Riemann solvers.  ROE is the reference's (riemannSolver{X,Y,Z}_mhd_roe,
+ * Source/RiemannSolvers_MHD_Roe_HelperModule.py:74-1559).  HLLE (Einfeldt's bounds
+ * from the states' and the Roe average's fast speeds) and ROE_HLLE (Roe, switching
+ * to HLLE at a face whose Roe intermediate states have a non-positive density or
+ * pressure) are the solvers the reference's header names as missing (:24-30). */
+#define PMHD_RIEMANN_ROE 0
+#define PMHD_RIEMANN_HLLE 1
+#define PMHD_RIEMANN_ROE_HLLE 2
+
+/* Reference quirks (see the parity notes) that a grid may correct:
+ * Z_BC: the 3D z reconstruction source terms and first-order z states use BcZ, not
+ *   BcX (Source/EvolveFunctions_MHD.py:898, 904, 951);
+ * OUTFLOW_GHOSTS: the outflow ghosts of the CTU terms follow the reference's own
+ *   x-direction convention (edge cell, edge face) in every direction; the reference
+ *   takes cell and face row 1 for the 2D y top ghost and face row ny-1 for its bottom
+ *   (Source/CTU_HelperModule.py:377-387), face nz-1 for the 3D z forward ghost
+ *   (:897-898) and flux row nx-1 for a 3D y bottom ghost (:724-725);
+ * DTDY_80BIT: the x-direction 3D CTU update uses 0.5*dt/dy in double, not the
+ *   reference's 80-bit long double (Source/CTU_HelperModule.py:455). */
+#define PMHD_FIX_Z_BC 1
+#define PMHD_FIX_OUTFLOW_GHOSTS 2
+#define PMHD_FIX_DTDY_80BIT 4
 
 typedef struct pmhd_handle pmhd_handle;
 
@@ -166,6 +192,21 @@ typedef struct pmhd_stats {
   int64_t floored_total;       /* sum of floored[][] */
   int64_t nonfinite;           /* non-finite conserved values of the current state */
 } pmhd_stats;
+
+/* The MHD Riemann flux of n faces: riemannSolverX/Y/Z_mhd_roe
+ * (Source/RiemannSolvers_MHD_Roe_HelperModule.py:74, 595, 1093) as one call, or
+ * the HLLE / Roe-with-HLLE-fallback solvers (riemann = PMHD_RIEMANN_*).  Host
+ * float64 arrays in the reference's directional 7-vector order (x: rho, vx, vy,
+ * vz, By, Bz, p; y: ..., Bz, Bx, p; z: ..., Bx, By, p), each 7 planes of n faces:
+ * wl, wr primitive, ul, ur conserved (E last), bn the n face-normal fields; flux
+ * receives 7 planes.  direction 0/1/2 = x/y/z.  Computed with the step's own
+ * flux code in the precision's arithmetic (64: bit for bit the reference's; 32:
+ * the FP32 unit with its FP64 wave-speed islands), on the GPU through temporary
+ * device buffers.  Errors: pmhd_last_error(NULL). */
+int pmhd_riemann_flux(int32_t precision, int32_t direction, int32_t riemann, int64_t n,
+                      const double* wl, const double* wr, const double* ul, const double* ur,
+                      const double* bn, double gamma, int32_t entropy_fix, double epsilon,
+                      double* flux);
 
 /* Statistics of the current state (synchronises the handle's stream).  No
  * counterpart in the reference. */

@@ -479,9 +479,46 @@ def interface_states(w_dir, bn_cell, gamma, dt, dn, bc, d, axis, order):
 # Roe flux for ideal MHD in the rotated frame
 # (Source/RiemannSolvers_MHD_Roe_HelperModule.py:74-576 and rotations)
 # ---------------------------------------------------------------------------
-def roe_flux(WL, WR, UL, UR, bn, gamma, entropy_fix, eps, d):
-    """Directional prim/cons 7-vectors at faces -> directional flux 7-vector."""
-    lam, Lm, R, FL, FR, ul, ur = roe_parts(WL, WR, UL, UR, bn, gamma, entropy_fix, eps, d)
+def fast_speed(w, bn, gamma):
+    """Fast magnetosonic speed of rotated primitive states (normal field bn)."""
+    inv = 1.0 / w[0]
+    btsq = w[4] * w[4] + w[5] * w[5]
+    cs2 = gamma * w[6] * inv
+    bsq = bn * bn + btsq
+    ssum = bsq * inv + cs2
+    sdiff = bsq * inv - cs2
+    return np.sqrt(0.5 * (ssum + np.sqrt(sdiff * sdiff + 4.0 * cs2 * btsq * inv)))
+
+
+def hlle_flux(wl, wr, ul, ur, FL, FR, bn, gamma, lam_lo, lam_hi):
+    """HLLE flux in the rotated frame (Harten-Lax-van Leer with Einfeldt's bounds:
+    the smaller of the left state's and the Roe average's left-going fast speeds,
+    the larger of the right state's and the Roe average's right-going ones; the
+    upwind flux when both bounds have one sign).  New in the build (the reference
+    names HLLE as missing, Source/RiemannSolvers_MHD_Roe_HelperModule.py:24-30)."""
+    sl = np.minimum(wl[1] - fast_speed(wl, bn, gamma), lam_lo)
+    sr = np.maximum(wr[1] + fast_speed(wr, bn, gamma), lam_hi)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        inv = 1.0 / (sr - sl)
+        F = (sr * FL - sl * FR + sl * sr * (ur - ul)) * inv
+    F = np.where(sr <= 0.0, FR, F)
+    return np.where(sl >= 0.0, FL, F)
+
+
+def _cons_pressure(u, bn, gamma):
+    ke = 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / u[0]
+    return (gamma - 1.0) * (u[6] - ke - 0.5 * (bn * bn + u[4] * u[4] + u[5] * u[5]))
+
+
+def roe_flux(WL, WR, UL, UR, bn, gamma, entropy_fix, eps, d, solver=0):
+    """Directional prim/cons 7-vectors at faces -> directional flux 7-vector.
+    solver 0: the reference's Roe flux; 1: HLLE; 2: Roe, with HLLE at the faces
+    whose Roe intermediate states U_L + sum_{j<=k} (L_j . dU) R_j (k = 0..5) have a
+    non-positive density or pressure."""
+    lam, Lm, R, FL, FR, ul, ur, lam0 = roe_parts(WL, WR, UL, UR, bn, gamma, entropy_fix, eps, d, raw=True)
+    if solver == 1:
+        wl, wr = _rot(WL, d), _rot(WR, d)
+        return _unrot(hlle_flux(wl, wr, ul, ur, FL, FR, bn, gamma, lam0[0], lam0[6]), d)
     dU = ur - ul
     proj = _project(Lm, dU)
     amps = [0.5 * np.abs(lam[k]) * proj[k] for k in range(7)]
@@ -489,6 +526,15 @@ def roe_flux(WL, WR, UL, UR, bn, gamma, entropy_fix, eps, d):
     for i in range(7):
         for k in range(7):
             F[i] = F[i] - amps[k] * R[i, k]
+    if solver == 2:
+        us = ul.copy()
+        bad = np.zeros(ul.shape[1:], dtype=bool)
+        for k in range(6):
+            us = us + proj[k] * R[:, k]
+            with np.errstate(divide='ignore', invalid='ignore'):
+                bad = bad | ~(us[0] > 0.0) | ~(_cons_pressure(us, bn, gamma) > 0.0)
+        wl, wr = _rot(WL, d), _rot(WR, d)
+        F = np.where(bad, hlle_flux(wl, wr, ul, ur, FL, FR, bn, gamma, lam0[0], lam0[6]), F)
     up = lam[0] >= 0.0
     F = np.where(up, FL, F)
     dn = lam[6] <= 0.0
@@ -496,7 +542,7 @@ def roe_flux(WL, WR, UL, UR, bn, gamma, entropy_fix, eps, d):
     return _unrot(F, d)
 
 
-def roe_parts(WL, WR, UL, UR, bn, gamma, entropy_fix, eps, d):
+def roe_parts(WL, WR, UL, UR, bn, gamma, entropy_fix, eps, d, raw=False):
     """Roe eigen-system in the rotated frame: (lam[7], L[7,7], R[7,7], F_L, F_R, U_L, U_R)."""
     wl = _rot(WL, d)
     wr = _rot(WR, d)
@@ -542,6 +588,7 @@ def roe_parts(WL, WR, UL, UR, bn, gamma, entropy_fix, eps, d):
     csl2 = cs2 * ca2 / cf2
     csl = np.sqrt(csl2)
     lam = [vn - cf, vn - ca, vn - csl, vn, vn + csl, vn + ca, vn + cf]
+    lam_raw = list(lam)
     if entropy_fix:
         lam = [l + np.sign(l) * (np.abs(l) < 2.0 * eps) * (eps + l * l / (4.0 * eps)) for l in lam]
     with np.errstate(divide='ignore', invalid='ignore'):
@@ -684,6 +731,8 @@ def roe_parts(WL, WR, UL, UR, bn, gamma, entropy_fix, eps, d):
     Lm[6, 5] = Lm[0, 5]
     FL = _cell_flux(wl, ul, bn, hbl)
     FR = _cell_flux(wr, ur, bn, hbr)
+    if raw:
+        return lam, Lm, R, FL, FR, ul, ur, lam_raw
     return lam, Lm, R, FL, FR, ul, ur
 
 
@@ -955,12 +1004,19 @@ def _remap_x_for_y(f):
     return f
 
 
-def ctu_transverse_2d_y(prim_x, prim_y, F, fby, dt, dx, dy, bcy):
+def ctu_transverse_2d_y(prim_x, prim_y, F, fby, dt, dx, dy, bcy, fix=0):
     inv = 1.0 / dy
     hdx = 0.5 * (dt / dx)
     hdt = 0.5 * dt
     ny, nx = prim_x.shape[1:]
-    if bcy == OUTFLOW:
+    if bcy == OUTFLOW and (fix & 2):
+        # corrected: the x-direction convention (edge cell, edge faces)
+        ce = [0] + list(range(ny)) + [ny - 1]
+        tb = _take(fby, [0] + list(range(ny + 1)), 0)
+        bb = _take(fby, list(range(ny + 1)) + [ny], 0)
+        fe_t = [0] + list(range(ny))
+        fe_b = list(range(ny)) + [ny - 1]
+    elif bcy == OUTFLOW:
         # the reference's top ghost is cell row 1 (it raises IndexError when ny == 1;
         # here row 0, the only row, is used instead)
         ce = [min(1, ny - 1)] + list(range(ny)) + [ny - 1]
@@ -1008,7 +1064,7 @@ def _minmod_ctu(t, nrm):
     return out
 
 
-def ctu_transverse_3d(d, UL, UR, prim_x, prim_y, flux, faces, emf, dt, dx, dy, dz, bc):
+def ctu_transverse_3d(d, UL, UR, prim_x, prim_y, flux, faces, emf, dt, dx, dy, dz, bc, fix=0):
     """3D CTU update of the d-interface conserved states.
     flux: dict of the predictor fluxes of the two transverse directions;
     faces: (fBx, fBy, fBz); emf: the corner EMF along d."""
@@ -1026,7 +1082,7 @@ def ctu_transverse_3d(d, UL, UR, prim_x, prim_y, flux, faces, emf, dt, dx, dy, d
     # the cell columns left (i-1) and right (i) of each d-face
     rows_l = ce[0:n + 1]
     rows_r = ce[1:n + 2]
-    if d == 'y' and bc == OUTFLOW:
+    if d == 'y' and bc == OUTFLOW and not (fix & 2):
         # reference quirk: row index nx-1 (Source/CTU_HelperModule.py:724); it raises
         # IndexError when nx > ny, where the last row ny-1 is used instead
         rows_r_fx = list(range(ny)) + [min(nx - 1, ny - 1)]
@@ -1055,7 +1111,8 @@ def ctu_transverse_3d(d, UL, UR, prim_x, prim_y, flux, faces, emf, dt, dx, dy, d
         for a_ in (loL, hiL, loR, hiR):
             a_[4] = 0.0
             a_[5] = 0.0
-        if d == 'x' and c == 'y':
+        ext80 = d == 'x' and c == 'y' and not (fix & 4)
+        if ext80:
             h = 0.5 * (np.longdouble(dt) / np.longdouble(dy))
             newL = (newL - h * (hiL - loL)).astype(np.float64)
             newR = (newR - h * (hiR - loR)).astype(np.float64)
@@ -1069,7 +1126,7 @@ def ctu_transverse_3d(d, UL, UR, prim_x, prim_y, flux, faces, emf, dt, dx, dy, d
             # component t2 gets + h/2 (E(+t1,0) - E(0,0) + E(+t1,+t2) - E(0,+t2))
             termL = eL['10'] - eL['00'] + eL['11'] - eL['01']
             termR = eR['10'] - eR['00'] + eR['11'] - eR['01']
-            if d == 'x':
+            if ext80:
                 newL[5] = (newL[5] + h * 0.5 * termL).astype(np.float64)
                 newR[5] = (newR[5] + h * 0.5 * termR).astype(np.float64)
             else:
@@ -1081,7 +1138,7 @@ def ctu_transverse_3d(d, UL, UR, prim_x, prim_y, flux, faces, emf, dt, dx, dy, d
             newL[4] = newL[4] - h * 0.5 * termL
             newR[4] = newR[4] - h * 0.5 * termR
     # source terms
-    dB = _ctu_db(d, faces, n, bc, width, nax)
+    dB = _ctu_db(d, faces, n, bc, width, nax, fix)
     bn_ = {'x': 'bx', 'y': 'by', 'z': 'bz'}
     mm_t1 = _minmod_ctu(dB[t1], dB[d])
     mm_t2 = _minmod_ctu(dB[t2], dB[d])
@@ -1131,7 +1188,7 @@ def _emf_stencil(d, emf, n, bc):
     return out
 
 
-def _ctu_db(d, faces, n, bc, width, nax):
+def _ctu_db(d, faces, n, bc, width, nax, fix=0):
     """Cell-centred face-B derivatives on the cells -1..n along d (one ghost)."""
     fbx, fby, fbz = faces
     f = {'x': fbx, 'y': fby, 'z': fbz}
@@ -1144,7 +1201,7 @@ def _ctu_db(d, faces, n, bc, width, nax):
             m = f[c].shape[a]           # n+1 faces
             if bc == OUTFLOW:
                 lo_idx = [0] + list(range(n + 1))
-                hi_idx = list(range(n + 1)) + [n if d != 'z' else n - 1]
+                hi_idx = list(range(n + 1)) + [n if (d != 'z' or fix & 2) else n - 1]
             else:
                 lo_idx = [n - 1] + list(range(n + 1))
                 hi_idx = list(range(n + 1)) + [1]
@@ -1165,8 +1222,13 @@ def _ctu_db(d, faces, n, bc, width, nax):
 # one cycle of each evolve function
 # ---------------------------------------------------------------------------
 class Params:
+    """Step parameters; riemann selects the flux (0 the reference's Roe, 1 HLLE,
+    2 Roe with the HLLE fallback) and fix_quirks the reference behaviours to
+    correct (bits 1 z BC, 2 outflow CTU ghosts, 4 80-bit factor); both new in the
+    build, 0 is the reference."""
+
     def __init__(self, gamma, cfl, min_dens=1e-12, min_pres=1e-12, min_energy=1e-12,
-                 order=2, entropy_fix=False, eps=0.5):
+                 order=2, entropy_fix=False, eps=0.5, riemann=0, fix_quirks=0):
         self.gamma = gamma
         self.cfl = cfl
         self.min_dens = min_dens
@@ -1175,6 +1237,8 @@ class Params:
         self.order = order
         self.entropy_fix = entropy_fix
         self.eps = eps
+        self.riemann = riemann
+        self.fix_quirks = fix_quirks
 
 
 def _dirvec(a8, d):
@@ -1194,7 +1258,7 @@ def step_1d(prim, cons, dt, dx, bcx, P):
     wr = apply_floors(wr, True, P.min_dens, P.min_pres, P.min_energy)
     ul = prim_to_cons(wl, fbx, g)
     ur = prim_to_cons(wr, fbx, g)
-    F = roe_flux(wl, wr, ul, ur, fbx, g, P.entropy_fix, P.eps, 'x')
+    F = roe_flux(wl, wr, ul, ur, fbx, g, P.entropy_fix, P.eps, 'x', P.riemann)
     ux = ux - (dt / dx) * (F[:, 1:] - F[:, :-1])
     wx = cons_to_prim(ux, bx, g)
     pa = np.stack([wx[0], wx[1], wx[2], wx[3], bx, wx[4], wx[5], wx[6]])
@@ -1227,13 +1291,13 @@ def step_2d(prim, cons, fbx, fby, dt, dx, dy, bcx, bcy, P):
     ut = prim_to_cons(wt, fby, g)
     ub = prim_to_cons(wb, fby, g)
     Ec = wx[2] * wy[5] - wx[1] * wx[4]
-    F = roe_flux(wl, wr, ul, ur, fbx, g, P.entropy_fix, P.eps, 'x')
-    G = roe_flux(wt, wb, ut, ub, fby, g, P.entropy_fix, P.eps, 'y')
+    F = roe_flux(wl, wr, ul, ur, fbx, g, P.entropy_fix, P.eps, 'x', P.riemann)
+    G = roe_flux(wt, wb, ut, ub, fby, g, P.entropy_fix, P.eps, 'y', P.riemann)
     E = corner_emf_2d(F, G, Ec, bcx, bcy)
     hx = ctu_transverse_2d_x(wx, wy, G, fbx, dt, dx, dy, bcx)
     ul = fl(ul + hx[0], False)
     ur = fl(ur + hx[1], False)
-    vy_ = ctu_transverse_2d_y(wx, wy, F, fby, dt, dx, dy, bcy)
+    vy_ = ctu_transverse_2d_y(wx, wy, F, fby, dt, dx, dy, bcy, P.fix_quirks)
     ut = fl(ut + vy_[0], False)
     ub = fl(ub + vy_[1], False)
     nfbx = fbx - hdy * (E[1:, :] - E[:-1, :])
@@ -1249,8 +1313,8 @@ def step_2d(prim, cons, fbx, fby, dt, dx, dy, bcx, bcy, P):
     nvx = h0[1] / h0[0]
     nvy = h0[2] / h0[0]
     nEc = nvy * ncbx - nvx * ncby
-    F = roe_flux(wl, wr, ul, ur, nfbx, g, P.entropy_fix, P.eps, 'x')
-    G = roe_flux(wt, wb, ut, ub, nfby, g, P.entropy_fix, P.eps, 'y')
+    F = roe_flux(wl, wr, ul, ur, nfbx, g, P.entropy_fix, P.eps, 'x', P.riemann)
+    G = roe_flux(wt, wb, ut, ub, nfby, g, P.entropy_fix, P.eps, 'y', P.riemann)
     E = corner_emf_2d(F, G, nEc, bcx, bcy)
     fbx = fbx - (dt / dy) * (E[1:, :] - E[:-1, :])
     fby = fby + (dt / dx) * (E[:, 1:] - E[:, :-1])
@@ -1286,7 +1350,7 @@ def step_3d(prim, cons, fbx, fby, fbz, dt, dx, dy, dz, bcx, bcy, bcz, P):
     if P.order == 0:
         wl, wr = copy_states(wx, bcx, 1)
         wt, wb = copy_states(wy, bcy, 0)
-        wk, wf = copy_states(wz, bcx, 2)          # reference uses BcX for z here
+        wk, wf = copy_states(wz, bcz if P.fix_quirks & 1 else bcx, 2)   # the reference uses BcX
     else:
         wl, wr = recon_states(wx, wy[5], g, dt, dx, bcx, 'x', 1, P.order)
         wt, wb = recon_states(wy, wx[4], g, dt, dy, bcy, 'y', 0, P.order)
@@ -1307,7 +1371,8 @@ def step_3d(prim, cons, fbx, fby, fbz, dt, dx, dy, dz, bcx, bcy, bcz, P):
     wb[5] = wb[5] + C[1, 1]
     ut = prim_to_cons(wt, fby, g, 'y')
     ub = prim_to_cons(wb, fby, g, 'y')
-    C = recon_b_terms_3d('z', ux, wx[0], fbx, fby, fbz, dt, dx, dy, dz, bcx)  # BcX as in the reference
+    C = recon_b_terms_3d('z', ux, wx[0], fbx, fby, fbz, dt, dx, dy, dz,
+                         bcz if P.fix_quirks & 1 else bcx)   # the reference uses BcX
     wk[4] = wk[4] + C[0, 0]
     wk[5] = wk[5] + C[0, 1]
     wf[4] = wf[4] + C[1, 0]
@@ -1318,17 +1383,18 @@ def step_3d(prim, cons, fbx, fby, fbz, dt, dx, dy, dz, bcx, bcy, bcz, P):
     Eyc = (wx[5] * ux[1] - wy[5] * ux[3]) / ux[0]
     Ezc = (wy[5] * ux[2] - wx[4] * ux[1]) / ux[0]
     ef, eps = P.entropy_fix, P.eps
-    F = roe_flux(wl, wr, ul, ur, fbx, g, ef, eps, 'x')
-    G = roe_flux(wt, wb, ut, ub, fby, g, ef, eps, 'y')
-    H = roe_flux(wk, wf, uk, uf, fbz, g, ef, eps, 'z')
+    F = roe_flux(wl, wr, ul, ur, fbx, g, ef, eps, 'x', P.riemann)
+    G = roe_flux(wt, wb, ut, ub, fby, g, ef, eps, 'y', P.riemann)
+    H = roe_flux(wk, wf, uk, uf, fbz, g, ef, eps, 'z', P.riemann)
     Ex = corner_emf_3d_x(G, H, Exc, bcx, bcy, bcz)
     Ey = corner_emf_3d_y(F, H, Eyc, bcx, bcy, bcz)
     Ez = corner_emf_3d_z(F, G, Ezc, bcx, bcy, bcz)
     faces = (fbx, fby, fbz)
     flux = {'x': F, 'y': G, 'z': H}
-    ul, ur = ctu_transverse_3d('x', ul, ur, wx, wy, flux, faces, Ex, dt, dx, dy, dz, bcx)
-    ut, ub = ctu_transverse_3d('y', ut, ub, wx, wy, flux, faces, Ey, dt, dx, dy, dz, bcy)
-    uk, uf = ctu_transverse_3d('z', uk, uf, wx, wy, flux, faces, Ez, dt, dx, dy, dz, bcz)
+    fq = P.fix_quirks
+    ul, ur = ctu_transverse_3d('x', ul, ur, wx, wy, flux, faces, Ex, dt, dx, dy, dz, bcx, fq)
+    ut, ub = ctu_transverse_3d('y', ut, ub, wx, wy, flux, faces, Ey, dt, dx, dy, dz, bcy, fq)
+    uk, uf = ctu_transverse_3d('z', uk, uf, wx, wy, flux, faces, Ez, dt, dx, dy, dz, bcz, fq)
     nfbx, nfby, nfbz = _ct_update_3d(fbx, fby, fbz, Ex, Ey, Ez, hdx, hdy, hdz)
     wl = cons_to_prim(ul, nfbx, g, 'x')
     wr = cons_to_prim(ur, nfbx, g, 'x')
@@ -1348,9 +1414,9 @@ def step_3d(prim, cons, fbx, fby, fbz, dt, dx, dy, dz, bcx, bcy, bcz, P):
     nExc = (nby * nmz - nbz * nmy) / nd
     nEyc = (nbz * nmx - nbx * nmz) / nd
     nEzc = (nbx * nmy - nby * nmx) / nd
-    F = roe_flux(wl, wr, ul, ur, nfbx, g, ef, eps, 'x')
-    G = roe_flux(wt, wb, ut, ub, nfby, g, ef, eps, 'y')
-    H = roe_flux(wk, wf, uk, uf, nfbz, g, ef, eps, 'z')
+    F = roe_flux(wl, wr, ul, ur, nfbx, g, ef, eps, 'x', P.riemann)
+    G = roe_flux(wt, wb, ut, ub, nfby, g, ef, eps, 'y', P.riemann)
+    H = roe_flux(wk, wf, uk, uf, nfbz, g, ef, eps, 'z', P.riemann)
     Ex = corner_emf_3d_x(G, H, nExc, bcx, bcy, bcz)
     Ey = corner_emf_3d_y(F, H, nEyc, bcx, bcy, bcz)
     Ez = corner_emf_3d_z(F, G, nEzc, bcx, bcy, bcz)

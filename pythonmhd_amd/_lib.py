@@ -16,7 +16,7 @@ EXPORTS = ('pmhd_create', 'pmhd_destroy', 'pmhd_set_state', 'pmhd_get_state', 'p
            'pmhd_get_state_device', 'pmhd_set_stream', 'pmhd_cfl_dt_noalloc', 'pmhd_compute_dt',
            'pmhd_step', 'pmhd_set_sweep', 'pmhd_advance', 'pmhd_synchronize', 'pmhd_diagnostics',
            'pmhd_set_profiling',
-           'pmhd_get_profile', 'pmhd_get_stats', 'pmhd_last_error',
+           'pmhd_get_profile', 'pmhd_get_stats', 'pmhd_riemann_flux', 'pmhd_last_error',
            'pmhd_batch_create', 'pmhd_batch_destroy', 'pmhd_batch_set_state', 'pmhd_batch_get_state',
            'pmhd_batch_advance', 'pmhd_batch_autotune', 'pmhd_batch_get_tiles', 'pmhd_batch_synchronize',
            'pmhd_batch_set_profiling', 'pmhd_batch_get_profile', 'pmhd_batch_get_stats', 'pmhd_batch_last_error',
@@ -38,7 +38,8 @@ class PmhdParams(ctypes.Structure):
                 ('min_dens', ctypes.c_double), ('min_pres', ctypes.c_double),
                 ('min_energy', ctypes.c_double), ('epsilon', ctypes.c_double),
                 ('reconstruct_order', ctypes.c_int32), ('entropy_fix', ctypes.c_int32),
-                ('precision', ctypes.c_int32), ('physics', ctypes.c_int32)]
+                ('precision', ctypes.c_int32), ('physics', ctypes.c_int32),
+                ('riemann', ctypes.c_int32), ('fix_quirks', ctypes.c_int32)]
 
 
 class PmhdStats(ctypes.Structure):
@@ -97,6 +98,8 @@ def _declare(lib):
                                      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int32)]
     lib.pmhd_get_stats.argtypes = [H, ctypes.POINTER(PmhdStats)]
+    lib.pmhd_riemann_flux.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, P, P, P, P,
+                                      P, ctypes.c_double, ctypes.c_int32, ctypes.c_double, P]
     lib.pmhd_batch_get_stats.argtypes = [H, ctypes.c_int32, ctypes.POINTER(PmhdStats)]
     lib.pmhd_last_error.argtypes = [H]
     lib.pmhd_last_error.restype = ctypes.c_char_p
@@ -204,8 +207,13 @@ def make_grid(shape, widths, bcs):
     return g
 
 
+# Riemann solvers (PMHD_RIEMANN_*) and reference quirks a grid may correct (PMHD_FIX_*)
+ROE, HLLE, ROE_HLLE = 0, 1, 2
+FIX_Z_BC, FIX_OUTFLOW_GHOSTS, FIX_DTDY_80BIT = 1, 2, 4
+
+
 def make_params(gamma, cfl, min_dens=1e-12, min_pres=1e-12, min_energy=1e-12, order=2,
-                entropy_fix=False, epsilon=0.5, precision=64, physics=0):
+                entropy_fix=False, epsilon=0.5, precision=64, physics=0, riemann=ROE, fix_quirks=0):
     p = PmhdParams()
     p.gamma, p.cfl = float(gamma), float(cfl)
     p.min_dens, p.min_pres, p.min_energy = float(min_dens), float(min_pres), float(min_energy)
@@ -214,6 +222,8 @@ def make_params(gamma, cfl, min_dens=1e-12, min_pres=1e-12, min_energy=1e-12, or
     p.entropy_fix = 1 if entropy_fix else 0
     p.precision = int(precision)
     p.physics = int(physics)
+    p.riemann = int(riemann)
+    p.fix_quirks = int(fix_quirks)
     return p
 
 
@@ -279,14 +289,14 @@ class Solver:
 
     def __init__(self, shape, widths, bcs, gamma, cfl, min_dens=1e-12, min_pres=1e-12,
                  min_energy=1e-12, order=2, entropy_fix=False, epsilon=0.5, precision=64,
-                 lib=None, physics=MHD):
+                 lib=None, physics=MHD, riemann=ROE, fix_quirks=0):
         self.lib = lib if lib is not None else load()
         self.ndim = len(shape)
         self.shape = tuple(int(s) for s in shape)   # (nx,) / (ny, nx) / (ny, nx, nz)
         self.physics = int(physics)
         g = make_grid(self.shape, widths, bcs)
         p = make_params(gamma, cfl, min_dens, min_pres, min_energy, order, entropy_fix, epsilon, precision,
-                        physics)
+                        physics, riemann, fix_quirks)
         self.grid, self.params = g, p
         h = ctypes.c_void_p()
         rc = self.lib.pmhd_create(ctypes.byref(g), ctypes.byref(p), ctypes.byref(h))
@@ -437,7 +447,7 @@ class Batch:
 
     def __init__(self, shapes, widths, bcs, gamma, cfl, min_dens=1e-12, min_pres=1e-12,
                  min_energy=1e-12, order=2, entropy_fix=False, epsilon=0.5, precision=64, lib=None,
-                 physics=MHD):
+                 physics=MHD, riemann=ROE, fix_quirks=0):
         self.lib = lib if lib is not None else load()
         self.physics = int(physics)
         self.shapes = [tuple(int(x) for x in s) for s in shapes]
@@ -448,7 +458,7 @@ class Batch:
         bcs = list(bcs) if len(bcs) == n and hasattr(bcs[0], '__len__') else [bcs] * n
         grids = (PmhdGrid * n)(*[make_grid(s, w, b) for s, w, b in zip(self.shapes, widths, bcs)])
         p = make_params(gamma, cfl, min_dens, min_pres, min_energy, order, entropy_fix, epsilon, precision,
-                        physics)
+                        physics, riemann, fix_quirks)
         self.grids, self.params = grids, p
         h = ctypes.c_void_p()
         rc = self.lib.pmhd_batch_create(n, grids, ctypes.byref(p), ctypes.byref(h))

@@ -201,3 +201,10 @@ ROE = "roe"
 HLLE = "hlle"
 ROE_HLLE_FALLBACK = "roe-hlle"
 PRINT_CYCLES = "printCycles"
+# reference behaviours to correct (bits, include/pmhd.h PMHD_FIX_*) and the option
+# to raise where the reference raises (pythonmhd_amd.evolve)
+FIX_QUIRKS = "fixQuirks"
+RAISE_LIKE_REFERENCE = "raiseLikeReference"
+FIX_Z_BC = 1
+FIX_OUTFLOW_GHOSTS = 2
+FIX_DTDY_80BIT = 4

@@ -74,6 +74,8 @@ PMHD_FI void make_dual_ctx(const Ctx<V>& c, const Ctx<double>& a, AdState* st, C
   d.efix = c.efix;
   d.ec_next = c.ec_next;
   d.ec_valid = c.ec_valid;
+  d.riemann = c.riemann;
+  d.fixq = c.fixq;
   d.gamma = D(double(c.gamma));
   d.eps = D(double(c.eps));
   d.min_dens = D(double(c.min_dens));

@@ -56,6 +56,13 @@ pmhd_handle* pmhd_new_handle_f32();
 struct pmhd_batch;
 pmhd_batch* pmhd_new_batch_f64();
 pmhd_batch* pmhd_new_batch_f32();
+// the Riemann flux of n faces in the precision's arithmetic (pmhd_riemann_flux)
+int pmhd_riemann_f64(int d, int solver, int64_t n, const double* wl, const double* wr, const double* ul,
+                     const double* ur, const double* bn, double gamma, int efix, double eps, double* F,
+                     std::string& err);
+int pmhd_riemann_f32(int d, int solver, int64_t n, const double* wl, const double* wr, const double* ul,
+                     const double* ur, const double* bn, double gamma, int efix, double eps, double* F,
+                     std::string& err);
 
 struct pmhd_batch {
   virtual ~pmhd_batch() {}
@@ -191,7 +198,10 @@ void init_ctx(Ctx<T>& c, const pmhd_grid& g, const pmhd_params& p) {
   c.bc[0] = g.bcx;
   c.bc[1] = g.bcy;
   c.bc[2] = g.bcz;
-  c.bc_zrec = g.bcx;
+  c.fixq = p.fix_quirks;
+  c.riemann = p.riemann;
+  // the reference's z reconstruction uses the x boundary condition
+  c.bc_zrec = (p.fix_quirks & PMHD_FIX_Z_BC) ? g.bcz : g.bcx;
   c.order = p.reconstruct_order;
   c.efix = p.entropy_fix;
   c.gamma = T(p.gamma);
@@ -931,6 +941,10 @@ inline const char* params_problem(const pmhd_params* p) {
   if (p->reconstruct_order < 0 || p->reconstruct_order > 2) return "reconstruct_order must be 0, 1 or 2";
   if (p->precision != 64 && p->precision != 32) return "precision must be 64 or 32";
   if (p->physics != 0 && p->physics != 1) return "physics must be 0 (MHD) or 1 (hydrodynamics)";
+  if (p->riemann < 0 || p->riemann > 2) return "riemann must be 0 (Roe), 1 (HLLE) or 2 (Roe with HLLE fallback)";
+  if (p->physics == 1 && p->riemann != 0) return "the hydrodynamic step has the Roe solver only (riemann 0)";
+  if (p->fix_quirks & ~7) return "fix_quirks has unknown bits (PMHD_FIX_Z_BC | PMHD_FIX_OUTFLOW_GHOSTS | "
+                                 "PMHD_FIX_DTDY_80BIT)";
   return nullptr;
 }
 
@@ -1035,7 +1049,12 @@ PMHD_NS_END
 // The constructors of one precision, in the translation unit compiled for it.
 #define PMHD_DEFINE_FACTORIES(SUFFIX, T, BACKEND)                                           \
   pmhd_handle* pmhd_new_handle_##SUFFIX() { return new (std::nothrow) pmhd::Handle<T, BACKEND>(); } \
-  pmhd_batch* pmhd_new_batch_##SUFFIX() { return new (std::nothrow) pmhd::BatchHandle<T, BACKEND>(); }
+  pmhd_batch* pmhd_new_batch_##SUFFIX() { return new (std::nothrow) pmhd::BatchHandle<T, BACKEND>(); } \
+  int pmhd_riemann_##SUFFIX(int d, int solver, int64_t n, const double* wl, const double* wr,          \
+                            const double* ul, const double* ur, const double* bn, double gamma,        \
+                            int efix, double eps, double* F, std::string& err) {                      \
+    return pmhd::riemann_run<T>(d, solver, n, wl, wr, ul, ur, bn, gamma, efix, eps, F, err);          \
+  }
 
 // C ABI bodies (one translation unit; the handles come from the constructors above).
 #define PMHD_DEFINE_C_ABI()                                                                 \
@@ -1121,6 +1140,29 @@ PMHD_NS_END
                                   int32_t* n_entries) {                                     \
     if (!h || !names || !total_ms || !launches || !elements || !n_entries) return PMHD_EINVAL; \
     return h->get_profile(max_entries, names, total_ms, launches, elements, n_entries);     \
+  }                                                                                         \
+  extern "C" int pmhd_riemann_flux(int32_t precision, int32_t direction, int32_t riemann,  \
+                                   int64_t n, const double* wl, const double* wr,           \
+                                   const double* ul, const double* ur, const double* bn,    \
+                                   double gamma, int32_t entropy_fix, double epsilon,       \
+                                   double* flux) {                                          \
+    g_pmhd_null_error = nullptr;                                                            \
+    if ((precision != 64 && precision != 32) || direction < 0 || direction > 2 ||           \
+        riemann < 0 || riemann > 2 || n < 0 || (n > 0 && (!wl || !wr || !ul || !ur || !bn || !flux))) { \
+      g_pmhd_create_error = "pmhd_riemann_flux: precision 64/32, direction 0-2, riemann 0-2, n >= 0 " \
+                            "and all arrays are required";                                  \
+      return PMHD_EINVAL;                                                                   \
+    }                                                                                       \
+    try {                                                                                   \
+      return precision == 64                                                                \
+                 ? pmhd_riemann_f64(direction, riemann, n, wl, wr, ul, ur, bn, gamma,       \
+                                    entropy_fix, epsilon, flux, g_pmhd_create_error)        \
+                 : pmhd_riemann_f32(direction, riemann, n, wl, wr, ul, ur, bn, gamma,       \
+                                    entropy_fix, epsilon, flux, g_pmhd_create_error);       \
+    } catch (...) {                                                                         \
+      g_pmhd_create_error = "unexpected exception in pmhd_riemann_flux";                    \
+      return PMHD_EINVAL;                                                                   \
+    }                                                                                       \
   }                                                                                         \
   extern "C" int pmhd_get_stats(pmhd_handle* h, pmhd_stats* out) {                          \
     if (!h || !out) return PMHD_EINVAL;                                                     \

@@ -367,6 +367,49 @@ inline unsigned stride_blocks(int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
+// the Riemann flux of n faces (pmhd_riemann_flux): host float64 arrays staged
+// through device buffers, one thread per face
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) k_riemann(int d, int solver, int64_t n, const double* wl, const double* wr,
+                                                 const double* ul, const double* ur, const double* bn, T g,
+                                                 int efix, T eps, double* F) {
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+    riemann_face<T>(d, solver, n, i, wl, wr, ul, ur, bn, g, efix != 0, eps, F);
+}
+
+template <class T>
+int riemann_run(int d, int solver, int64_t n, const double* wl, const double* wr, const double* ul,
+                const double* ur, const double* bn, double gamma, int efix, double eps, double* F,
+                std::string& err) {
+  if (n <= 0) return PMHD_OK;
+  const size_t plane = size_t(n) * sizeof(double);
+  double* buf = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(&buf), 36 * plane) != hipSuccess) {
+    (void)hipGetLastError();
+    err = "device allocation failed";
+    return PMHD_ENOMEM;
+  }
+  const double* src[5] = {wl, wr, ul, ur, bn};
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < 5 && e == hipSuccess; ++k)
+    e = hipMemcpy(buf + size_t(7 * k) * size_t(n), src[k], (k < 4 ? 7 : 1) * plane, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const double* dv = buf;
+    hipLaunchKernelGGL((k_riemann<T>), dim3(stride_blocks(n)), dim3(256), 0, nullptr, d, solver, n, dv,
+                       dv + 7 * n, dv + 14 * n, dv + 21 * n, dv + 28 * n, T(gamma), efix, T(eps), buf + 29 * n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(F, buf + 29 * n, 7 * plane, hipMemcpyDeviceToHost);
+  (void)hipFree(buf);
+  if (e != hipSuccess) {
+    err = std::string("HIP error: ") + hipGetErrorString(e);
+    return PMHD_EHIP;
+  }
+  return PMHD_OK;
+}
+
+// ---------------------------------------------------------------------------
 // adjoint of one phase: one thread per output element (pmhd_adjoint.h)
 // ---------------------------------------------------------------------------
 struct DevAtomicAdd {

@@ -10,6 +10,7 @@
 #pragma once
 #include <type_traits>
 
+#include "pmhd.h"
 #include "pmhd_math.h"
 
 PMHD_NS_BEGIN
@@ -94,6 +95,8 @@ struct Ctx {
   int efix;
   int ec_next;       // the update also stores the new state's cell E (the next cycle's CellE)
   int ec_valid;      // Ec already holds the current state's cell E (the cycle skips CellE)
+  int riemann;       // RIEMANN_ROE (the reference's), RIEMANN_HLLE or RIEMANN_ROE_HLLE
+  int fixq;          // PMHD_FIX_* bits: reference behaviours corrected instead of reproduced
   int seg[3];        // faces per thread of the fused predictor sweep along x, y, z (0: separate
                      // slope / edge / predictor launches; -1 along z: one workgroup per line)
   T gamma, eps, min_dens, min_pres, min_energy;
@@ -575,7 +578,7 @@ PMHD_FI void predict_face(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7]) {
   to_rot(wr, d, rwr);
   to_rot(ul, d, rul);
   to_rot(ur, d, rur);
-  roe_flux(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf);
+  roe_flux(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf, c.riemann);
   from_rot(rf, d, df);
   store7(c.F[d], NF, idx, df);
 }
@@ -856,7 +859,9 @@ PMHD_FI void ctu_source_3d(const Ctx<T>& c, int d, int bc, I3 f, int64_t e, T sr
   I3 pm = with(f, d, map1(e, n, bc));
   int64_t flo, fhi;
   if (e < 0) flo = bc == OUTFLOW ? 0 : n - 1; else flo = e;
-  if (e + 1 > n) fhi = bc == OUTFLOW ? (d == 2 ? n - 1 : n) : 1; else fhi = e + 1;
+  // the reference's z-outflow forward ghost face is face nz-1 (Source/CTU_HelperModule.py:897-898)
+  const bool zq = d == 2 && !(c.fixq & PMHD_FIX_OUTFLOW_GHOSTS);
+  if (e + 1 > n) fhi = bc == OUTFLOW ? (zq ? n - 1 : n) : 1; else fhi = e + 1;
   T dd = face_val(c, c.fb[d], d, f, fhi) - face_val(c, c.fb[d], d, f, flo);
   T d1 = face_val(c, c.fb[t1], t1, pm, co(pm, t1) + 1) - face_val(c, c.fb[t1], t1, pm, co(pm, t1));
   T d2 = face_val(c, c.fb[t2], t2, pm, co(pm, t2) + 1) - face_val(c, c.fb[t2], t2, pm, co(pm, t2));
@@ -896,7 +901,7 @@ PMHD_FI void ctu_apply_3d(const Ctx<T>& c, int d, int tc, I3 pcell, int64_t hi_o
   T h = c.hdtd[tc];
   const int vs[5] = {0, 1, 2, 3, 6};
   // x faces, y fluxes: the reference's 80-bit factor (FP64 instantiation only)
-  const bool ext80 = is_f64<T>() && d == 0 && tc == 1;
+  const bool ext80 = is_f64<T>() && d == 0 && tc == 1 && !(c.fixq & PMHD_FIX_DTDY_80BIT);
 #pragma unroll
   for (int q = 0; q < 5; ++q) {
     int v = vs[q];
@@ -948,7 +953,8 @@ PMHD_FI void ctu_states(const Ctx<T>& c, int d, I3 f, T ul[7], T ur[7]) {
     // y-outflow: the bottom ghost of the high x-face flux sits in row nx-1
     // (Source/CTU_HelperModule.py:724); where that row does not exist (nx > ny, the
     // reference raises IndexError) the last row ny-1 is used
-    int64_t hov = (d == 1 && bc == OUTFLOW && e == n) ? (c.nx < c.ny ? c.nx - 1 : c.ny - 1) : -1;
+    int64_t hov = (d == 1 && bc == OUTFLOW && e == n && !(c.fixq & PMHD_FIX_OUTFLOW_GHOSTS))
+                      ? (c.nx < c.ny ? c.nx - 1 : c.ny - 1) : -1;
     for (int s = 0; s < 2; ++s) {
       int tc = s == 0 ? first : second;
       ctu_apply_3d(c, d, tc, pl, -1, f, layer_l, ul);
@@ -972,7 +978,7 @@ PMHD_FI void ctu_states(const Ctx<T>& c, int d, I3 f, T ul[7], T ur[7]) {
     auto src2 = [&](int64_t ee, T s[7]) {
       int64_t cm;
       int64_t flo, fhi;
-      if (d == 0) {
+      if (d == 0 || (c.fixq & PMHD_FIX_OUTFLOW_GHOSTS)) {
         cm = map1(ee, n, bc);
         flo = ee < 0 ? (bc == OUTFLOW ? 0 : n - 1) : ee;
         fhi = ee + 1 > n ? (bc == OUTFLOW ? n : 1) : ee + 1;
@@ -1092,7 +1098,7 @@ PMHD_FI void k_correct(const Ctx<T>& c, int d, I3 f) {
   to_rot(wr, d, rwr);
   to_rot(ul, d, rul);
   to_rot(ur, d, rur);
-  roe_flux(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf);
+  roe_flux(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf, c.riemann);
   from_rot(rf, d, df);
   store7(c.Fc[d], NF, idx, df);
 }
@@ -1167,6 +1173,33 @@ PMHD_FI void k_update(const Ctx<T>& c, I3 p) {
   st(c.W2 + 7 * N, idx, w[6]);
   // the next cycle's cell E from the stored values (what CellE would read back)
   if (c.ec_next && c.ndim >= 2) store_cell_e<T>(c, idx, u[0], u[1], u[2], u[3], w[1], w[2], bx, w[4], w[5]);
+}
+
+// ===========================================================================
+// the Riemann flux of one face from float64 arrays of directional 7-vectors
+// (planes of n faces; riemannSolver{X,Y,Z}_mhd_roe's layout,
+// Source/RiemannSolvers_MHD_Roe_HelperModule.py:74, 595, 1093), computed in T
+// ===========================================================================
+template <class T>
+PMHD_FI void riemann_face(int d, int solver, int64_t n, int64_t i, const double* wl, const double* wr,
+                          const double* ul, const double* ur, const double* bn, T g, bool efix, T eps,
+                          double* F) {
+  T a[7], b[7], c[7], e[7], ra[7], rb[7], rc[7], re[7], rf[7], df[7];
+#pragma unroll
+  for (int v = 0; v < 7; ++v) {
+    a[v] = T(wl[v * n + i]);
+    b[v] = T(wr[v * n + i]);
+    c[v] = T(ul[v * n + i]);
+    e[v] = T(ur[v * n + i]);
+  }
+  to_rot(a, d, ra);
+  to_rot(b, d, rb);
+  to_rot(c, d, rc);
+  to_rot(e, d, re);
+  roe_flux(ra, rb, rc, re, T(bn[i]), g, efix, eps, rf, solver);
+  from_rot(rf, d, df);
+#pragma unroll
+  for (int v = 0; v < 7; ++v) F[v * n + i] = double(df[v]);
 }
 
 // ===========================================================================

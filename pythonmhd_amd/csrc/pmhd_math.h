@@ -581,9 +581,58 @@ PMHD_FI T entropy_fixed(T l, T eps) {
   return l + s * (eps + l * l / (T(4) * eps));
 }
 
+// Riemann solvers of the step: the reference's Roe flux; HLLE (no counterpart in
+// the reference, which names it as missing at
+// Source/RiemannSolvers_MHD_Roe_HelperModule.py:24-30); Roe that falls back to HLLE
+// at a face whose Roe intermediate states have a non-positive density or pressure
+// (the switch of Athena's Roe solver that the reference's header describes)
+enum { RIEMANN_ROE = 0, RIEMANN_HLLE = 1, RIEMANN_ROE_HLLE = 2 };
+
+// fast magnetosonic speed of a rotated primitive state with normal field bn
+template <class T>
+PMHD_FI T fast_speed_of(const T w[7], T bn, T g) {
+  const T inv = T(1) / w[0];
+  const T btsq = w[4] * w[4] + w[5] * w[5];
+  const T cs2 = g * w[6] * inv;
+  const T bsq = bn * bn + btsq;
+  const T ssum = bsq * inv + cs2;
+  const T sdiff = bsq * inv - cs2;
+  return sqrt_(T(0.5) * (ssum + sqrt_(sdiff * sdiff + T(4) * cs2 * btsq * inv)));
+}
+
+// HLLE flux with Einfeldt's wave-speed bounds: the slowest of the left state's and
+// the Roe average's left-going fast speeds, the fastest of the right state's and
+// the Roe average's right-going ones; the upwind flux when both bounds have one sign
+template <class T>
+PMHD_FI void hlle_flux(const T wl[7], const T wr[7], const T ul[7], const T ur[7], T bn, T g, T lam_lo,
+                       T lam_hi, const T FL[7], const T FR[7], T F[7]) {
+  const T sl = npmin(wl[1] - fast_speed_of(wl, bn, g), lam_lo);
+  const T sr = npmax(wr[1] + fast_speed_of(wr, bn, g), lam_hi);
+  if (sl >= T(0)) {
+#pragma unroll
+    for (int v = 0; v < 7; ++v) F[v] = FL[v];
+    return;
+  }
+  if (sr <= T(0)) {
+#pragma unroll
+    for (int v = 0; v < 7; ++v) F[v] = FR[v];
+    return;
+  }
+  const T inv = T(1) / (sr - sl);
+#pragma unroll
+  for (int v = 0; v < 7; ++v) F[v] = (sr * FL[v] - sl * FR[v] + sl * sr * (ur[v] - ul[v])) * inv;
+}
+
+// pressure of a rotated conserved state (normal field bn)
+template <class T>
+PMHD_FI T cons_pressure(const T u[7], T bn, T g) {
+  const T ke = T(0.5) * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / u[0];
+  return (g - T(1)) * (u[6] - ke - T(0.5) * (bn * bn + u[4] * u[4] + u[5] * u[5]));
+}
+
 template <class T>
 PMHD_HD inline void roe_flux(const T wl[7], const T wr[7], const T ul[7], const T ur[7], T bn,
-                             T g, bool efix, T eps, T F[7]) {
+                             T g, bool efix, T eps, T F[7], int solver = RIEMANN_ROE) {
   using N = island_t<T>;
   const T rl = wl[0], rr = wr[0];
   // FP32 fast unit: reciprocal roots of the two densities, of the Cargo-Gallice
@@ -678,13 +727,18 @@ PMHD_HD inline void roe_flux(const T wl[7], const T wr[7], const T ul[7], const 
     csl = sqrt_(T(n_csl2));
   }
   T lam[7] = {vn - cf, vn - ca, vn - csl, vn, vn + csl, vn + ca, vn + cf};
+  T FL[7], FR[7];
+  cell_flux(wl, ul, bn, hbl, FL);
+  cell_flux(wr, ur, bn, hbr, FR);
+  if (solver == RIEMANN_HLLE) {
+    hlle_flux(wl, wr, ul, ur, bn, g, lam[0], lam[6], FL, FR, F);
+    return;
+  }
+  const T lam_lo = lam[0], lam_hi = lam[6];
   if (efix) {
 #pragma unroll
     for (int k = 0; k < 7; ++k) lam[k] = entropy_fixed(lam[k], eps);
   }
-  T FL[7], FR[7];
-  cell_flux(wl, ul, bn, hbl, FL);
-  cell_flux(wr, ur, bn, hbr, FR);
   if (lam[0] >= T(0)) {
     // supersonic to the right; the reference still evaluates the Roe sum and then
     // overwrites it, so only the selection matters
@@ -814,6 +868,28 @@ PMHD_HD inline void roe_flux(const T wl[7], const T wr[7], const T ul[7], const 
   T p4 = T(0) + L40 * dU[0] + L41 * dU[1] + L42 * dU[2] + L43 * dU[3] + L46 * dU[6] + L24 * dU[4] + L25 * dU[5];
   T p5 = T(0) + (-L10) * dU[0] + (-L12) * dU[2] + (-L13) * dU[3] + L14 * dU[4] + L15 * dU[5];
   T p6 = T(0) + L60 * dU[0] + L61 * dU[1] + L62 * dU[2] + L63 * dU[3] + L66 * dU[6] + L04 * dU[4] + L05 * dU[5];
+  if (solver == RIEMANN_ROE_HLLE) {
+    // the Roe intermediate states U_L + sum_{j<=k} p_j R_j, k = 0..5: a non-positive
+    // density or pressure in any of them selects the HLLE flux
+    const T pk[6] = {p0, p1, p2, p3, p4, p5};
+    const T col[6][7] = {{Af_, R10, R20, R30, R40, R50, R60}, {T(0), T(0), R21, R31, R41, R51, R61},
+                         {As_, R12, R22, R32, R42, R52, R62}, {T(1), vn, R23, R33, T(0), T(0), R63},
+                         {As_, R14, R24, R34, R42, R52, R64}, {T(0), T(0), R25, R35, R41, R51, R65}};
+    T us[7];
+#pragma unroll
+    for (int v = 0; v < 7; ++v) us[v] = ul[v];
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+#pragma unroll
+      for (int v = 0; v < 7; ++v) us[v] = us[v] + pk[k] * col[k][v];
+      bad = bad || !(us[0] > T(0)) || !(cons_pressure(us, bn, g) > T(0));
+    }
+    if (bad) {
+      hlle_flux(wl, wr, ul, ur, bn, g, lam_lo, lam_hi, FL, FR, F);
+      return;
+    }
+  }
   T a0 = T(0.5) * npabs(lam[0]) * p0;
   T a1 = T(0.5) * npabs(lam[1]) * p1;
   T a2 = T(0.5) * npabs(lam[2]) * p2;

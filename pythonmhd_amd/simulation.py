@@ -99,6 +99,10 @@ def validateSimPar(simPar, primVars):
                 raise RuntimeError("\nYou passed an invalid data type for epsilon; it must be numeric.")
             if simPar[C.EPSILON] <= 0.0:
                 raise RuntimeError("\nepsilon must be greater than 0.")
+    if C.RIEMANN_SOLVER in k and simPar[C.RIEMANN_SOLVER] != C.ROE and not simPar.get(C.IS_MHD, C.DEFAULT_IS_MHD):
+        raise RuntimeError("The hydrodynamic step has the Roe riemannSolver only.")
+    if C.FIX_QUIRKS in k and (type(simPar[C.FIX_QUIRKS]) != int or simPar[C.FIX_QUIRKS] & ~7):  # noqa: E721
+        raise RuntimeError("fixQuirks must be an integer of FIX_Z_BC | FIX_OUTFLOW_GHOSTS | FIX_DTDY_80BIT bits.")
     if C.PRECISION in k and simPar[C.PRECISION] not in (32, 64):
         raise RuntimeError("\nprecision must be 64 (FP64 throughout) or 32 (FP32 bulk with FP64 islands).")
     if C.SAVE_DATA in k:
@@ -199,6 +203,8 @@ class Simulation:
         self.precision = g(C.PRECISION, C.DEFAULT_PRECISION)
         self.riemannSolver = g(C.RIEMANN_SOLVER, C.ROE)
         self.printCycles = bool(g(C.PRINT_CYCLES, False))
+        self.fixQuirks = g(C.FIX_QUIRKS, 0)
+        self.raiseLikeReference = bool(g(C.RAISE_LIKE_REFERENCE, False))
         print("Building Simulation Grid\n")
         self.grid = SimulationGrid(primVars, gridPar, self.isMHD, self.gamma, self.minDens, self.minPres,
                                    self.minEnergy, consVars, faceBx, faceBy, faceBz)
@@ -252,7 +258,8 @@ class Simulation:
         simTime = self.startTime
         numCycles = self.startCycle
         evolve = self._evolve()
-        extra = dict(precision=self.precision, printCycles=self.printCycles, riemannSolver=self.riemannSolver)
+        extra = dict(precision=self.precision, printCycles=self.printCycles, riemannSolver=self.riemannSolver,
+                     fixQuirks=self.fixQuirks, raiseLikeReference=self.raiseLikeReference)
         # the stops: the next plot time and the next save time (Simulation.py:330-346)
         resumed = hasattr(self, 'resumeStops')
         lastPlotTime, lastSaveTime = self.resumeStops if resumed else (0.0, 0.0)

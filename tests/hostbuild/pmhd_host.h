@@ -227,6 +227,14 @@ struct HostBatchExec {
   }
 };
 
+template <class T>
+int riemann_run(int d, int solver, int64_t n, const double* wl, const double* wr, const double* ul,
+                const double* ur, const double* bn, double gamma, int efix, double eps, double* F,
+                std::string&) {
+  for (int64_t i = 0; i < n; ++i) riemann_face<T>(d, solver, n, i, wl, wr, ul, ur, bn, T(gamma), efix != 0, T(eps), F);
+  return PMHD_OK;
+}
+
 struct HostBackend {
   HostExec ex;
   HostBatchExec bex;

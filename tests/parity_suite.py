@@ -778,3 +778,152 @@ def test_floors_reference_indexerror_case_is_flagged(lib):
     st = s.stats()
     s.close()
     assert st['floored_max'][:6].max() >= 3
+
+
+# ---------------------------------------------------------------------------
+# Riemann solvers: the reference's Roe flux as one call (golden vectors of
+# riemannSolver{X,Y,Z}_mhd_roe), its FP32 evaluation at the degeneracies, and the
+# HLLE / Roe-with-HLLE-fallback solvers and the quirk corrections against the oracle
+# ---------------------------------------------------------------------------
+def _roe_golden():
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), 'golden', 'roe.npz'))
+
+
+@pytest.mark.parametrize('axis', ['x', 'y', 'z'])
+def test_riemann_flux_golden(lib, axis):
+    """pmhd_riemann_flux (FP64, Roe) reproduces the reference's Roe flux bit for bit,
+    with and without the entropy fix."""
+    from pythonmhd_amd.riemann import riemann_flux
+    r = _roe_golden()
+    for ef in (0, 1):
+        f = riemann_flux(r[axis + '_wl'], r[axis + '_wr'], r[axis + '_ul'], r[axis + '_ur'], r[axis + '_bn'],
+                         GAMMA, bool(ef), 0.5, 'xyz'.index(axis), lib=lib)
+        same(f, r['%s_ef%d' % (axis, ef)])
+
+
+@pytest.mark.parametrize('axis', ['x', 'y', 'z'])
+def test_riemann_flux_fp32_at_degeneracies(lib, axis):
+    """The FP32 unit's Roe flux against the reference's FP64 one on every face of the
+    golden set, including its degenerate blocks (identical states, supersonic flow
+    both ways, B_t = B_n = 0, exact zeros in B): each flux component within 2e-5 of
+    the face's largest FP64 flux component (relative FP32 rounding of about 6e-8
+    times the conditioning of the eigen-projection)."""
+    from pythonmhd_amd.riemann import riemann_flux
+    r = _roe_golden()
+    for ef in (0, 1):
+        want = r['%s_ef%d' % (axis, ef)]
+        f = riemann_flux(r[axis + '_wl'], r[axis + '_wr'], r[axis + '_ul'], r[axis + '_ur'], r[axis + '_bn'],
+                         GAMMA, bool(ef), 0.5, 'xyz'.index(axis), precision=32, lib=lib)
+        scale = np.maximum(np.max(np.abs(want), axis=0), 1e-30)
+        err = np.max(np.abs(f - want), axis=0) / scale
+        blocks = {'identical': err[:100], 'supersonic': err[100:300], 'zero B': err[300:350], 'random': err[350:]}
+        print('fp32 roe %s ef%d: ' % (axis, ef) + ', '.join('%s %.2e' % (k, float(v.max())) for k, v in blocks.items()))
+        assert float(err.max()) < 2e-5, {k: float(v.max()) for k, v in blocks.items()}
+
+
+@pytest.mark.parametrize('solver', [1, 2])
+def test_riemann_flux_solvers_against_oracle(lib, solver):
+    """HLLE and Roe-with-HLLE-fallback fluxes of the golden face states equal the
+    oracle's bit for bit (FP64); the fallback changes only faces whose Roe
+    intermediate states are unphysical."""
+    from pythonmhd_amd.riemann import riemann_flux
+    r = _roe_golden()
+    for axis in 'xyz':
+        args = [r[axis + k] for k in ('_wl', '_wr', '_ul', '_ur', '_bn')]
+        f = riemann_flux(*args, GAMMA, False, 0.5, 'xyz'.index(axis), solver=solver, lib=lib)
+        same(f, M.roe_flux(*args, GAMMA, False, 0.5, axis, solver))
+
+
+def _rarefaction_1d(n=96):
+    x = (np.arange(n) + 0.5) / n
+    prim = np.zeros((8, n))
+    prim[0] = 1.0
+    prim[7] = 0.4
+    prim[1] = np.where(x < 0.5, -3.0, 3.0)
+    prim[4] = 0.5
+    prim[5] = 0.5
+    return prim
+
+
+@pytest.mark.parametrize('solver,order', [(1, 0), (1, 2), (2, 0), (2, 2)])
+def test_solvers_strong_rarefaction(lib, solver, order):
+    """A strong double rarefaction (v = -+3, rho = 1, p = 0.4, gamma = 1.4) on which
+    the Roe step fails (negative pressure, then NaN): HLLE and Roe with the HLLE
+    fallback run to t = 0.08 with positive density and pressure, bit for bit the
+    oracle's in FP64."""
+    prim = _rarefaction_1d()
+    cons = M.prim_to_cons_all(prim, 1.4)
+    n = prim.shape[1]
+    P = M.Params(1.4, 0.8, order=order, riemann=solver)
+    print('case: rarefaction solver %d order %d' % (solver, order), flush=True)
+    ref = M.evolve_1d(prim, cons, 1.0 / n, 0, 0.0, 0.08, 0, 400, P)
+    s = Solver()((n,), (1.0 / n,), (0,), 1.4, 0.8, order=order, lib=lib, riemann=solver)
+    s.set_state(prim, cons)
+    t, cyc, _ = s.advance(0.0, 0.08, 0, 400)
+    p, c, _ = s.get_state(0)
+    s.close()
+    assert t == ref[2] and cyc == ref[3]
+    same(p, ref[0])
+    same(c, ref[1])
+    assert p[0].min() > 0 and p[7].min() > 0
+
+
+def test_roe_fails_on_strong_rarefaction(lib):
+    """The reference's Roe flux on the same problem: the state turns non-finite and
+    the loop ends with a NaN clock (PMHD_ENONFINITE from the C ABI)."""
+    from pythonmhd_amd._lib import ENONFINITE, PmhdError
+    prim = _rarefaction_1d()
+    cons = M.prim_to_cons_all(prim, 1.4)
+    n = prim.shape[1]
+    s = Solver()((n,), (1.0 / n,), (0,), 1.4, 0.8, order=2, lib=lib)
+    s.set_state(prim, cons)
+    with pytest.raises(PmhdError) as e:
+        s.advance(0.0, 0.08, 0, 400)
+    assert e.value.code == ENONFINITE
+    t, _, _ = s.advance(0.0, 0.08, 0, 400, allow_nonfinite=True)
+    s.close()
+    assert t != t
+
+
+@pytest.mark.parametrize('nd,solver', [(2, 1), (2, 2), (3, 2)])
+def test_solvers_multid_against_oracle(lib, nd, solver):
+    rng = np.random.default_rng(90 + nd + solver)
+    shape = {2: (14, 12), 3: (6, 8, 7)}[nd]
+    prim, faces = smooth_state(rng, shape, vamp=0.9)
+    cons = M.prim_to_cons_all(prim, GAMMA)
+    widths = widths_of(shape)
+    cfl = 0.4 if nd == 3 else 0.8
+    P = M.Params(GAMMA, cfl, order=2, entropy_fix=(nd == 3), riemann=solver)
+    print('case: solver %d nd %d' % (solver, nd), flush=True)
+    ref = oracle_run(shape, widths, (1,) * nd, prim, cons, faces, 10.0, 3, P)
+    got = run_solver(Solver(), lib, shape, widths, (1,) * nd, prim, cons, faces, 10.0, 3, cfl=cfl, order=2,
+                     entropy_fix=(nd == 3), riemann=solver)
+    same(got[1], ref[1])
+    for a, b in zip(got[2], ref[2]):
+        same(a, b)
+
+
+@pytest.mark.parametrize('nd,bcs,order,fix', [(2, (0, 0), 2, 2), (2, (1, 0), 0, 2), (3, (0, 0, 0), 2, 7),
+                                             (3, (1, 0, 0), 0, 1), (3, (0, 1, 1), 2, 4), (3, (1, 1, 0), 2, 3)])
+def test_fix_quirks_against_oracle(lib, nd, bcs, order, fix):
+    """Each quirk correction (z boundary condition, outflow CTU ghosts, the 80-bit
+    factor) against the oracle with the same correction, bit for bit (FP64), and it
+    changes the result where the quirk acts."""
+    rng = np.random.default_rng(60 + nd + fix)
+    shape = {2: (10, 12), 3: (6, 7, 8)}[nd]
+    prim, faces = smooth_state(rng, shape)
+    cons = M.prim_to_cons_all(prim, GAMMA)
+    widths = widths_of(shape)
+    cfl = 0.4 if nd == 3 else 0.8
+    print('case: fix %d nd %d bcs %s order %d' % (fix, nd, bcs, order), flush=True)
+    P = M.Params(GAMMA, cfl, order=order, entropy_fix=True, fix_quirks=fix)
+    ref = oracle_run(shape, widths, bcs, prim, cons, faces, 10.0, 2, P)
+    got = run_solver(Solver(), lib, shape, widths, bcs, prim, cons, faces, 10.0, 2, cfl=cfl, order=order,
+                     entropy_fix=True, fix_quirks=fix)
+    same(got[1], ref[1])
+    for a, b in zip(got[2], ref[2]):
+        same(a, b)
+    plain = oracle_run(shape, widths, bcs, prim, cons, faces, 10.0, 2,
+                       M.Params(GAMMA, cfl, order=order, entropy_fix=True))
+    assert not np.array_equal(plain[1], ref[1])

@@ -249,3 +249,73 @@ def test_2d_and_3d_reduce_to_1d(order):
     p3, c3, _, _, _ = M.step_3d(prim3, cons3, f3x, f3y, f3z, dt, 1 / n, 1 / ny, 1 / nz, 1, 1, 1, P)
     for v in (0, 1, 2, 3, 5, 6, 7):
         assert np.max(np.abs(p3[v] - p1[v][None, :, None])) < 1e-12, v
+
+
+# ---------------------------------------------------------------------------
+# HLLE and Roe with HLLE fallback (new in the build; no reference counterpart):
+# pinned by consistency, upwinding, the Roe property of the fallback and positivity
+# ---------------------------------------------------------------------------
+def _faces(rng, n, d='x'):
+    g = 5.0 / 3.0
+    pl = np.stack([rng.uniform(0.3, 2, n), *rng.uniform(-1, 1, (3, n)), *rng.uniform(-1, 1, (3, n)),
+                   rng.uniform(0.2, 2, n)])
+    pr = np.stack([rng.uniform(0.3, 2, n), *rng.uniform(-1, 1, (3, n)), *rng.uniform(-1, 1, (3, n)),
+                   rng.uniform(0.2, 2, n)])
+    bn = rng.uniform(-1, 1, n)
+    pl[4] = bn
+    pr[4] = bn
+    perm = list(M.DIRVARS['x'])
+    return (pl[perm], pr[perm], M.prim_to_cons_all(pl, g)[perm], M.prim_to_cons_all(pr, g)[perm], bn, g)
+
+
+def test_hlle_consistency_and_upwinding():
+    rng = np.random.default_rng(31)
+    wl, wr, ul, ur, bn, g = _faces(rng, 500)
+    # equal states: the physical flux (to rounding; exactly in supersonic cases)
+    f_same = M.roe_flux(wl, wl, ul, ul, bn, g, False, 0.5, 'x', 1)
+    lam, _, _, FL, _, _, _ = M.roe_parts(wl, wl, ul, ul, bn, g, False, 0.5, 'x')
+    phys = M._unrot(FL, 'x')
+    assert np.allclose(f_same, phys, rtol=1e-13, atol=1e-13)
+    # supersonic to the right: the left flux exactly
+    wl2, wr2 = wl.copy(), wr.copy()
+    wl2[1] += 20.0
+    wr2[1] += 20.0
+    ul2, ur2 = ul.copy(), ur.copy()
+    ul2[1] = wl2[0] * wl2[1]
+    ur2[1] = wr2[0] * wr2[1]
+    ul2[6] = ul[6] + 0.5 * wl2[0] * (wl2[1] ** 2 - wl[1] ** 2)
+    ur2[6] = ur[6] + 0.5 * wr2[0] * (wr2[1] ** 2 - wr[1] ** 2)
+    f = M.roe_flux(wl2, wr2, ul2, ur2, bn, g, False, 0.5, 'x', 1)
+    _, _, _, FL2, _, _, _ = M.roe_parts(wl2, wr2, ul2, ur2, bn, g, False, 0.5, 'x')
+    assert np.array_equal(f, M._unrot(FL2, 'x'))
+
+
+def test_hlle_is_more_diffusive_than_roe_and_bounds_the_fan():
+    """HLLE's bounds contain every Roe wave speed, so on a contact (jump in density
+    only, uniform v and p, no B) its mass flux differs from the exact upwind value
+    while Roe's equals it."""
+    g = 1.4
+    n = 4
+    wl = np.zeros((7, n))
+    wr = np.zeros((7, n))
+    wl[0], wr[0] = 1.0, 0.5
+    wl[1] = wr[1] = 0.2
+    wl[6] = wr[6] = 1.0
+    bn = np.zeros(n)
+    ul = M.prim_to_cons(wl, bn, g)
+    ur = M.prim_to_cons(wr, bn, g)
+    roe = M.roe_flux(wl, wr, ul, ur, bn, g, False, 0.5, 'x', 0)
+    hlle = M.roe_flux(wl, wr, ul, ur, bn, g, False, 0.5, 'x', 1)
+    assert np.allclose(roe[0], 0.2 * 1.0)              # upwind mass flux of the contact
+    assert np.all(np.abs(hlle[0] - 0.2) > 1e-3)        # smeared by HLLE
+
+
+def test_fallback_equals_roe_on_physical_states():
+    rng = np.random.default_rng(32)
+    wl, wr, ul, ur, bn, g = _faces(rng, 400)
+    roe = M.roe_flux(wl, wr, ul, ur, bn, g, False, 0.5, 'x', 0)
+    fb = M.roe_flux(wl, wr, ul, ur, bn, g, False, 0.5, 'x', 2)
+    hl = M.roe_flux(wl, wr, ul, ur, bn, g, False, 0.5, 'x', 1)
+    changed = ~np.all(fb == roe, axis=0)
+    assert changed.sum() < 0.5 * changed.size
+    assert np.array_equal(fb[:, changed], hl[:, changed])
