@@ -500,6 +500,10 @@ inline int create_adjoint(const pmhd_grid* g, const pmhd_params* p, int every, p
   if (rc) return rc;
   rc = check_params(p, err);
   if (rc) return rc;
+  if (p->riemann != 0) {
+    err = "the adjoint differentiates the Roe step only (riemann 0)";
+    return PMHD_EINVAL;
+  }
   if (g->ndim == 1 && p->physics == 0) {
     // the 1D MHD step holds B_x as a scalar of the context (the reference's uniform
     // Bx, Source/EvolveFunctions_MHD.py:209-212), which the dual evaluation cannot

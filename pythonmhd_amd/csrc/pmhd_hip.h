@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(256) k_phase_flat(const Ctx<T> c, unsigned n1,
 // SweepEnds launch, so every thread of every wave does the same work.  The face
 // states are those of k_sweep_run, so the results are identical.  q: the line's
 // first cell.  Every thread of the workgroup calls it (it synchronises).
-template <int D, class T>
+template <int D, int S, class T>
 __device__ inline void sweep_line(const Ctx<T>& c, const I3 q) {
   __shared__ T sra[7][256];
   __shared__ T prev[7];   // right edge state of the last cell of the previous chunk
@@ -81,7 +81,7 @@ __device__ inline void sweep_line(const Ctx<T>& c, const I3 q) {
       T wl[7];
 #pragma unroll
       for (int v = 0; v < 7; ++v) wl[v] = t == 0 ? prev[v] : sra[v][t - 1];
-      predict_face(c, D, with(q, D, k), wl, la);
+      predict_face<S>(c, D, with(q, D, k), wl, la);
     }
     __syncthreads();
     if (k0 + B < n) {
@@ -96,11 +96,11 @@ __device__ inline void sweep_line(const Ctx<T>& c, const I3 q) {
 
 // one workgroup per line: (j, i) = (blockIdx.y, blockIdx.x) along z in 3D, j =
 // blockIdx.x along x in 2D
-template <int D, class T>
+template <int D, int S, class T>
 __global__ void __launch_bounds__(256) k_sweep_line(const Ctx<T> c) {
   static_assert(D == 0 || D == 2, "line sweeps run along the contiguous axis");
   const I3 q = D == 2 ? I3{int64_t(blockIdx.y), int64_t(blockIdx.x), 0} : I3{int64_t(blockIdx.x), 0, 0};
-  sweep_line<D>(c, q);
+  sweep_line<D, S>(c, q);
 }
 
 __device__ inline unsigned long long umax64(unsigned long long a, unsigned long long b) {
@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(256) k_batch(const Ctx<T>* __restrict__ ctxs, 
 }
 
 // line sweeps of a batch: one workgroup per line of one box (prefix: lines per box)
-template <int D, class T>
+template <int D, int S, class T>
 __global__ void __launch_bounds__(256) k_batch_line(const Ctx<T>* __restrict__ ctxs, const int* __restrict__ prefix,
                                                     int nbox) {
   const int total = prefix[nbox];
@@ -256,7 +256,7 @@ __global__ void __launch_bounds__(256) k_batch_line(const Ctx<T>* __restrict__ c
     const TileRef r = locate_tile(ctxs, prefix, nbox, vb, EXT_LINE, D, 1, 1, 0);
     const Ctx<T>& c = ctxs[r.box];
     if (!c.active) continue;
-    sweep_line<D>(c, I3{r.j, r.i, r.k});
+    sweep_line<D, S>(c, I3{r.j, r.i, r.k});
   }
 }
 
@@ -580,7 +580,7 @@ struct HipExec {
       // lines (j, i) along z in 3D, j along x in 2D (nz = 1)
       const dim3 grid = Ph::dir == 2 ? dim3(unsigned(c.nx), unsigned(c.ny)) : dim3(unsigned(c.ny));
       timed(Ph::id, work, [&] {
-        hipLaunchKernelGGL((k_sweep_line<Ph::dir, T>), grid, dim3(b), 0, s, c);
+        hipLaunchKernelGGL((k_sweep_line<Ph::dir, Ph::solver, T>), grid, dim3(b), 0, s, c);
       });
       note(hipGetLastError());
       return;
@@ -718,7 +718,7 @@ struct HipBatchExec {
       const int total = lay->total[Ph::id];
       if (total <= 0) return;
       ex->timed(Ph::id, lay->elems[Ph::id], [&] {
-        hipLaunchKernelGGL((k_batch_line<Ph::dir, T>), dim3(unsigned(total)), dim3(64u), 0, ex->s, dctx, prefix,
+        hipLaunchKernelGGL((k_batch_line<Ph::dir, Ph::solver, T>), dim3(unsigned(total)), dim3(64u), 0, ex->s, dctx, prefix,
                            lay->nbox);
       });
       ex->note(hipGetLastError());

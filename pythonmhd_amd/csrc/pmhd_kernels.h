@@ -524,7 +524,7 @@ PMHD_FI void recon_b_term_3d(const Ctx<T>& c, int d, int bc, I3 p, int64_t e, T 
 
 // predictor of face f from its reconstructed states wl, wr (directional order;
 // modified in place)
-template <class T>
+template <int S, class T>
 PMHD_FI void predict_face(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7]) {
   const int64_t N = c.ncell();
   const int64_t NF = c.nface(d);
@@ -578,19 +578,19 @@ PMHD_FI void predict_face(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7]) {
   to_rot(wr, d, rwr);
   to_rot(ul, d, rul);
   to_rot(ur, d, rur);
-  roe_flux(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf, c.riemann);
+  roe_flux<S>(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf);
   from_rot(rf, d, df);
   store7(c.F[d], NF, idx, df);
 }
 
-template <class T>
+template <int S, class T>
 PMHD_FI void k_predict(const Ctx<T>& c, int d, I3 f) {
   const int64_t NF = c.nface(d);
   const int64_t idx = c.fid(d, f.j, f.i, f.k);
   T wl[7], wr[7];
   load7(c.WL, NF, idx, wl);
   load7(c.WR, NF, idx, wr);
-  predict_face(c, d, f, wl, wr);
+  predict_face<S>(c, d, f, wl, wr);
 }
 
 // ===========================================================================
@@ -606,7 +606,7 @@ PMHD_FI void k_predict(const Ctx<T>& c, int d, I3 f) {
 // reads the slopes of the slope launch; order 1 forms them here.
 // ===========================================================================
 // the faces [f0, f1) of the line through q along d
-template <class T>
+template <int S, class T>
 PMHD_FI void k_sweep_run(const Ctx<T>& c, int d, int bc, I3 q, int64_t f0, int64_t f1) {
   const int64_t n = c.n_along(d);
   const bool from_s = c.order == 2;
@@ -639,7 +639,7 @@ PMHD_FI void k_sweep_run(const Ctx<T>& c, int d, int bc, I3 q, int64_t f0, int64
 #pragma unroll
       for (int v = 0; v < 7; ++v) wr[v] = prev[v];
     }
-    predict_face(c, d, with(q, d, f), wl, wr);
+    predict_face<S>(c, d, with(q, d, f), wl, wr);
   }
 }
 
@@ -676,7 +676,7 @@ PMHD_FI void k_line_ends(const Ctx<T>& c, int d, int bc, I3 q) {
 // (outflow) or the right edge of cell n-1 and the left edge of cell 0 (periodic);
 // face n takes the right edge of cell n-1 on both sides (outflow) or the right edge
 // of cell n-1 and the left edge of cell 0 (periodic)
-template <class T>
+template <int S, class T>
 PMHD_FI void k_sweep_end(const Ctx<T>& c, int d, int bc, I3 q) {
   const int64_t L = line_count(c, d);
   const int64_t l = line_of(c, d, q);
@@ -692,15 +692,15 @@ PMHD_FI void k_sweep_end(const Ctx<T>& c, int d, int bc, I3 q) {
     wl[v] = face0 ? (bc == OUTFLOW ? l0[v] : rn[v]) : rn[v];
     wr[v] = face0 ? l0[v] : (bc == OUTFLOW ? rn[v] : l0[v]);
   }
-  predict_face(c, d, with(q, d, face0 ? 0 : c.n_along(d)), wl, wr);
+  predict_face<S>(c, d, with(q, d, face0 ? 0 : c.n_along(d)), wl, wr);
 }
 
-template <class T>
+template <int S, class T>
 PMHD_FI void k_sweep(const Ctx<T>& c, int d, int bc, I3 q) {
   const int64_t n = c.n_along(d);
   const int64_t len = c.seg[d] > 0 ? c.seg[d] : 1;
   const int64_t f0 = co(q, d) * len;
-  k_sweep_run(c, d, bc, q, f0, f0 + len < n + 1 ? f0 + len : n + 1);
+  k_sweep_run<S>(c, d, bc, q, f0, f0 + len < n + 1 ? f0 + len : n + 1);
 }
 
 // ===========================================================================
@@ -1083,7 +1083,7 @@ PMHD_FI void k_ct(const Ctx<T>& c, int d, const PI& fin, const PI& lin, const PO
 // corrector: cons -> prim with the half-step normal field, Roe flux
 // ===========================================================================
 // CTU-corrected states -> primitive with the half-step normal field -> Roe flux
-template <class T>
+template <int S, class T>
 PMHD_FI void k_correct(const Ctx<T>& c, int d, I3 f) {
   const int64_t NF = c.nface(d);
   const int64_t idx = c.fid(d, f.j, f.i, f.k);
@@ -1098,7 +1098,7 @@ PMHD_FI void k_correct(const Ctx<T>& c, int d, I3 f) {
   to_rot(wr, d, rwr);
   to_rot(ul, d, rul);
   to_rot(ur, d, rur);
-  roe_flux(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf, c.riemann);
+  roe_flux<S>(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf);
   from_rot(rf, d, df);
   store7(c.Fc[d], NF, idx, df);
 }
@@ -1196,7 +1196,7 @@ PMHD_FI void riemann_face(int d, int solver, int64_t n, int64_t i, const double*
   to_rot(b, d, rb);
   to_rot(c, d, rc);
   to_rot(e, d, re);
-  roe_flux(ra, rb, rc, re, T(bn[i]), g, efix, eps, rf, solver);
+  roe_flux_solver(ra, rb, rc, re, T(bn[i]), g, efix, eps, rf, solver);
   from_rot(rf, d, df);
 #pragma unroll
   for (int v = 0; v < 7; ++v) F[v * n + i] = double(df[v]);

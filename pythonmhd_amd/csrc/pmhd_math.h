@@ -630,9 +630,11 @@ PMHD_FI T cons_pressure(const T u[7], T bn, T g) {
   return (g - T(1)) * (u[6] - ke - T(0.5) * (bn * bn + u[4] * u[4] + u[5] * u[5]));
 }
 
-template <class T>
+// S: the solver (RIEMANN_*), fixed at compile time so that each grid's kernels
+// carry the code of its own solver only
+template <int S = RIEMANN_ROE, class T>
 PMHD_HD inline void roe_flux(const T wl[7], const T wr[7], const T ul[7], const T ur[7], T bn,
-                             T g, bool efix, T eps, T F[7], int solver = RIEMANN_ROE) {
+                             T g, bool efix, T eps, T F[7]) {
   using N = island_t<T>;
   const T rl = wl[0], rr = wr[0];
   // FP32 fast unit: reciprocal roots of the two densities, of the Cargo-Gallice
@@ -730,11 +732,13 @@ PMHD_HD inline void roe_flux(const T wl[7], const T wr[7], const T ul[7], const 
   T FL[7], FR[7];
   cell_flux(wl, ul, bn, hbl, FL);
   cell_flux(wr, ur, bn, hbr, FR);
-  if (solver == RIEMANN_HLLE) {
+  if constexpr (S == RIEMANN_HLLE) {
     hlle_flux(wl, wr, ul, ur, bn, g, lam[0], lam[6], FL, FR, F);
     return;
   }
   const T lam_lo = lam[0], lam_hi = lam[6];
+  (void)lam_lo;
+  (void)lam_hi;
   if (efix) {
 #pragma unroll
     for (int k = 0; k < 7; ++k) lam[k] = entropy_fixed(lam[k], eps);
@@ -868,7 +872,7 @@ PMHD_HD inline void roe_flux(const T wl[7], const T wr[7], const T ul[7], const 
   T p4 = T(0) + L40 * dU[0] + L41 * dU[1] + L42 * dU[2] + L43 * dU[3] + L46 * dU[6] + L24 * dU[4] + L25 * dU[5];
   T p5 = T(0) + (-L10) * dU[0] + (-L12) * dU[2] + (-L13) * dU[3] + L14 * dU[4] + L15 * dU[5];
   T p6 = T(0) + L60 * dU[0] + L61 * dU[1] + L62 * dU[2] + L63 * dU[3] + L66 * dU[6] + L04 * dU[4] + L05 * dU[5];
-  if (solver == RIEMANN_ROE_HLLE) {
+  if constexpr (S == RIEMANN_ROE_HLLE) {
     // the Roe intermediate states U_L + sum_{j<=k} p_j R_j, k = 0..5: a non-positive
     // density or pressure in any of them selects the HLLE flux
     const T pk[6] = {p0, p1, p2, p3, p4, p5};
@@ -919,6 +923,15 @@ PMHD_HD inline void roe_flux(const T wl[7], const T wr[7], const T ul[7], const 
   f = T(0.5) * (FL[6] + FR[6]);
   f = f - a0 * R60; f = f - a1 * R61; f = f - a2 * R62; f = f - a3 * R63; f = f - a4 * R64; f = f - a5 * R65; f = f - a6 * R66;
   F[6] = f;
+}
+
+// the flux of a solver chosen at run time (the Riemann-flux entry point)
+template <class T>
+PMHD_FI void roe_flux_solver(const T wl[7], const T wr[7], const T ul[7], const T ur[7], T bn, T g, bool efix,
+                             T eps, T F[7], int solver) {
+  if (solver == RIEMANN_HLLE) roe_flux<RIEMANN_HLLE>(wl, wr, ul, ur, bn, g, efix, eps, F);
+  else if (solver == RIEMANN_ROE_HLLE) roe_flux<RIEMANN_ROE_HLLE>(wl, wr, ul, ur, bn, g, efix, eps, F);
+  else roe_flux<RIEMANN_ROE>(wl, wr, ul, ur, bn, g, efix, eps, F);
 }
 
 // ---------------------------------------------------------------------------

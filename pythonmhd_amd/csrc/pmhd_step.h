@@ -23,6 +23,7 @@ PMHD_NS_BEGIN
   static constexpr int id = (ID);        \
   static constexpr int kind = (KIND);    \
   static constexpr int dir = (DIR);
+// phases that form Riemann fluxes carry their solver (RIEMANN_*) as a parameter
 
 template <int D>
 struct Slopes {
@@ -45,37 +46,41 @@ struct CopyStates {
     k_copy_states(c, D, D == 2 ? c.bc_zrec : c.bc[D], p);
   }
 };
-template <int D>
+template <int D, int S = RIEMANN_ROE>
 struct Predict {
   PMHD_PHASE_TRAITS(K_PREDICT + D, EXT_FACE, D)
+  static constexpr int solver = S;
   template <class T>
-  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_predict(c, D, p); }
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_predict<S>(c, D, p); }
 };
-template <int D>
+template <int D, int S = RIEMANN_ROE>
 struct Sweep {  // fused slopes (PLM) + edges + predictor over runs of faces
   PMHD_PHASE_TRAITS(K_SWEEP + D, EXT_SEG, D)
+  static constexpr int solver = S;
   template <class T>
-  PMHD_FI void operator()(const Ctx<T>& c, I3 q) const { k_sweep(c, D, c.bc[D], q); }
+  PMHD_FI void operator()(const Ctx<T>& c, I3 q) const { k_sweep<S>(c, D, c.bc[D], q); }
 };
 // the fused sweep over whole lines along the contiguous axis D (z in 3D, x in 2D),
 // faces 1..n-1, keeping the edge states of both end cells (one workgroup per line
 // on the GPU; here the reference order of one thread per line)
-template <int D>
+template <int D, int S = RIEMANN_ROE>
 struct SweepLine {
   static_assert(D == 0 || D == 2, "line sweeps run along the contiguous axis");
   PMHD_PHASE_TRAITS(D == 2 ? K_SWEEP_LINE_Z : K_SWEEP_LINE_X, EXT_LINE, D)
+  static constexpr int solver = S;
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 q) const {
-    k_sweep_run(c, D, c.bc[D], q, 1, c.n_along(D));
+    k_sweep_run<S>(c, D, c.bc[D], q, 1, c.n_along(D));
     k_line_ends(c, D, c.bc[D], q);
   }
 };
 // the two end faces (0 and n) of every line along D from the kept end states
-template <int D>
+template <int D, int S = RIEMANN_ROE>
 struct SweepEnds {
   PMHD_PHASE_TRAITS(D == 2 ? K_SWEEP_ENDS_Z : K_SWEEP_ENDS_X, EXT_ENDS, D)
+  static constexpr int solver = S;
   template <class T>
-  PMHD_FI void operator()(const Ctx<T>& c, I3 q) const { k_sweep_end(c, D, c.bc[D], q); }
+  PMHD_FI void operator()(const Ctx<T>& c, I3 q) const { k_sweep_end<S>(c, D, c.bc[D], q); }
 };
 struct CellE {
   PMHD_PHASE_TRAITS(K_CELL_E, EXT_CELL, 0)
@@ -114,11 +119,12 @@ struct HalfE {
   template <class T>
   PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_half_e(c, p); }
 };
-template <int D>
+template <int D, int S = RIEMANN_ROE>
 struct Correct {  // CTU transverse corrections + corrector flux
   PMHD_PHASE_TRAITS(K_CORRECT + D, EXT_FACE, D)
+  static constexpr int solver = S;
   template <class T>
-  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_correct(c, D, p); }
+  PMHD_FI void operator()(const Ctx<T>& c, I3 p) const { k_correct<S>(c, D, p); }
 };
 struct CtFull {  // full-step face field (one launch for all directions, as CtHalf)
   PMHD_PHASE_TRAITS(K_CT_FULL, EXT_EDGE, 0)
@@ -178,7 +184,7 @@ void diagnostics(X& ex, const Ctx<T>& c, double sums[9], double maxes[2]) {
 
 // interface states and predictor fluxes of direction D: with c.seg[D] > 0 the
 // reconstruction and the predictor are one launch (PPM keeps its slope launch)
-template <int D, class T, class X>
+template <int D, int S, class T, class X>
 void predictor_sweep(X& ex, const Ctx<T>& c) {
   // line launches exist along the contiguous axis only (z in 3D, x in 2D)
   constexpr bool can_line = D == 2 || D == 0;
@@ -187,13 +193,13 @@ void predictor_sweep(X& ex, const Ctx<T>& c) {
     if (c.order == 2) ex.template run<Slopes<D>>(c);
     if constexpr (can_line) {
       if (line) {
-        ex.template run<SweepLine<D>>(c);
+        ex.template run<SweepLine<D, S>>(c);
         // the end faces overlap the other directions' sweeps (joined before the EMFs)
-        ex.template run_side<SweepEnds<D>>(c);
+        ex.template run_side<SweepEnds<D, S>>(c);
         return;
       }
     }
-    ex.template run<Sweep<D>>(c);
+    ex.template run<Sweep<D, S>>(c);
     return;
   }
   if (c.order > 0) {
@@ -202,25 +208,18 @@ void predictor_sweep(X& ex, const Ctx<T>& c) {
   } else {
     ex.template run<CopyStates<D>>(c);
   }
-  ex.template run<Predict<D>>(c);
+  ex.template run<Predict<D, S>>(c);
 }
 
-template <template <int> class Ph, class T, class X>
-void each_dir(X& ex, const Ctx<T>& c) {
-  ex.template run<Ph<0>>(c);
-  ex.template run<Ph<1>>(c);
-  if (c.ndim == 3) ex.template run<Ph<2>>(c);
-}
-
-// c supplies the launch-invariant decisions (ndim, order), which a batch shares
-template <class T, class X>
-void cycle(X& ex, const Ctx<T>& c) {
+// c supplies the launch-invariant decisions (ndim, order, solver), which a batch shares
+template <int S, class T, class X>
+void cycle_solver(X& ex, const Ctx<T>& c) {
   const int nd = c.ndim;
   // the contiguous axis first (z in 3D): its line sweep's end-face launch then
   // overlaps the other sweeps
-  if (nd == 3) predictor_sweep<2>(ex, c);
-  predictor_sweep<0>(ex, c);
-  if (nd >= 2) predictor_sweep<1>(ex, c);
+  if (nd == 3) predictor_sweep<2, S>(ex, c);
+  predictor_sweep<0, S>(ex, c);
+  if (nd >= 2) predictor_sweep<1, S>(ex, c);
   ex.join_side();
   if (nd == 1) {
     ex.template run_max<Update>(c);
@@ -230,10 +229,29 @@ void cycle(X& ex, const Ctx<T>& c) {
   ex.template run<EmfPredictor>(c);   // corner EMFs of the predictor fluxes
   ex.template run<CtHalf>(c);         // half-step face field
   ex.template run<HalfE>(c);
-  each_dir<Correct>(ex, c);           // CTU transverse corrections + corrector fluxes
+  ex.template run<Correct<0, S>>(c);  // CTU transverse corrections + corrector fluxes
+  ex.template run<Correct<1, S>>(c);
+  if (nd == 3) ex.template run<Correct<2, S>>(c);
   ex.template run<EmfCorrector>(c);   // corner EMFs of the corrector fluxes
   ex.template run<CtFull>(c);         // full-step face field
   ex.template run_max<Update>(c);
+}
+
+// one cycle with the grid's Riemann solver (a unit that defines PMHD_ROE_ONLY, the
+// adjoint's, instantiates the Roe cycle only)
+template <class T, class X>
+void cycle(X& ex, const Ctx<T>& c) {
+#ifndef PMHD_ROE_ONLY
+  if (c.riemann == RIEMANN_HLLE) {
+    cycle_solver<RIEMANN_HLLE>(ex, c);
+    return;
+  }
+  if (c.riemann == RIEMANN_ROE_HLLE) {
+    cycle_solver<RIEMANN_ROE_HLLE>(ex, c);
+    return;
+  }
+#endif
+  cycle_solver<RIEMANN_ROE>(ex, c);
 }
 
 // ---------------------------------------------------------------------------
