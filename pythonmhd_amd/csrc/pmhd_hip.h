@@ -39,8 +39,12 @@ __device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
   return (per >= 8 && b < per * 8) ? (b % 8) * per + b / 8 : b;
 }
 
+// the run sweeps keep a window of cell vectors in registers: hold them to the
+// 168 registers at which three waves share each SIMD
+template <class Ph>
+constexpr int phase_waves() { return Ph::kind == EXT_SEG ? 3 : 1; }
 template <class Ph, class T>
-__global__ void __launch_bounds__(256) k_phase_flat(const Ctx<T> c, unsigned n1, unsigned n2, unsigned total) {
+__global__ void __launch_bounds__(256, phase_waves<Ph>()) k_phase_flat(const Ctx<T> c, unsigned n1, unsigned n2, unsigned total) {
   // (the sweeps' runs of faces share little between neighbouring threads)
   const unsigned b = Ph::kind == EXT_SEG ? unsigned(blockIdx.x) : xcd_block(blockIdx.x, gridDim.x);
   const unsigned t = b * blockDim.x + threadIdx.x;
@@ -60,34 +64,41 @@ __global__ void __launch_bounds__(256) k_phase_flat(const Ctx<T> c, unsigned n1,
 // first cell.  Every thread of the workgroup calls it (it synchronises).
 template <int D, int S, class T>
 __device__ inline void sweep_line(const Ctx<T>& c, const I3 q) {
-  __shared__ T sra[7][256];
-  __shared__ T prev[7];   // right edge state of the last cell of the previous chunk
+  // right edge states and reconstruction source terms of the chunk's cells; prev:
+  // those of the last cell of the previous chunk
+  __shared__ T sra[9][256];
+  __shared__ T prev[9];
   const int t = int(threadIdx.x), B = int(blockDim.x);
   const int64_t n = c.n_along(D);
   const int bc = c.bc[D];
   const bool from_s = c.order == 2;
   for (int64_t k0 = 0; k0 < n; k0 += B) {
     const int64_t k = k0 + t;
-    T la[7], ra[7];
+    T la[7], ra[7], src[2];
     if (k < n) {
       cell_edge_states(c, D, bc, with(q, D, k), from_s, la, ra);
+      cell_recon_src(c, D, with(q, D, k), k, src);
 #pragma unroll
       for (int v = 0; v < 7; ++v) sra[v][t] = ra[v];
+      sra[7][t] = src[0];
+      sra[8][t] = src[1];
       if (k == 0) store_line_end(c, D, q, 0, la);
       if (k == n - 1) store_line_end(c, D, q, 1, ra);
     }
     __syncthreads();
     if (k > 0 && k < n) {
-      T wl[7];
+      T wl[7], sl[2];
 #pragma unroll
       for (int v = 0; v < 7; ++v) wl[v] = t == 0 ? prev[v] : sra[v][t - 1];
-      predict_face<S>(c, D, with(q, D, k), wl, la);
+      sl[0] = t == 0 ? prev[7] : sra[7][t - 1];
+      sl[1] = t == 0 ? prev[8] : sra[8][t - 1];
+      predict_face_src<S>(c, D, with(q, D, k), wl, la, sl, src);
     }
     __syncthreads();
     if (k0 + B < n) {
       if (t == 0) {
 #pragma unroll
-        for (int v = 0; v < 7; ++v) prev[v] = sra[v][B - 1];
+        for (int v = 0; v < 9; ++v) prev[v] = sra[v][B - 1];
       }
       __syncthreads();
     }

@@ -395,19 +395,12 @@ PMHD_FI void k_slopes(const Ctx<T>& c, int d, int bc, I3 q) {
 // slopes of cells e-1, e, e+1 come from the slope array (PPM); with from_s false
 // (PLM) the cell's own limited slope is formed here from the same eigen-system as
 // the tracing, which is what the slope launch computes for that cell.
+// the edge states of cell p from its own and its neighbours' primitive vectors
+// (rotated) and its normal field bn
 template <class T>
-PMHD_FI void cell_edge_states(const Ctx<T>& c, int d, int bc, I3 p, bool from_s, T dla[7], T dra[7]) {
-  const int64_t N = c.ncell();
-  const int64_t idx = c.cid(p.j, p.i, p.k);
-  const int64_t n = c.n_along(d);
+PMHD_FI void cell_edge_core(const Ctx<T>& c, int d, I3 p, bool from_s, const T wl[7], const T wc[7], const T wr[7],
+                            T bn, T dla[7], T dra[7]) {
   const int64_t e = co(p, d);
-  I3 pl = with(p, d, map2(e - 1, n, bc));
-  I3 pr = with(p, d, map2(e + 1, n, bc));
-  T wl[7], wc[7], wr[7];
-  load_rot8(c.W, N, c.cid(pl.j, pl.i, pl.k), d, wl);
-  load_rot8(c.W, N, idx, d, wc);
-  load_rot8(c.W, N, c.cid(pr.j, pr.i, pr.k), d, wr);
-  T bn = ld(c.W + (4 + d) * N, idx);
   PrimEig<T> E;
   eigen_prim(wc, bn, c.gamma, E);
   T la[7], ra[7];
@@ -436,6 +429,22 @@ PMHD_FI void cell_edge_states(const Ctx<T>& c, int d, int bc, I3 p, bool from_s,
   }
   from_rot(la, d, dla);
   from_rot(ra, d, dra);
+}
+
+template <class T>
+PMHD_FI void cell_edge_states(const Ctx<T>& c, int d, int bc, I3 p, bool from_s, T dla[7], T dra[7]) {
+  const int64_t N = c.ncell();
+  const int64_t idx = c.cid(p.j, p.i, p.k);
+  const int64_t n = c.n_along(d);
+  const int64_t e = co(p, d);
+  I3 pl = with(p, d, map2(e - 1, n, bc));
+  I3 pr = with(p, d, map2(e + 1, n, bc));
+  T wl[7], wc[7], wr[7];
+  load_rot8(c.W, N, c.cid(pl.j, pl.i, pl.k), d, wl);
+  load_rot8(c.W, N, idx, d, wc);
+  load_rot8(c.W, N, c.cid(pr.j, pr.i, pr.k), d, wr);
+  T bn = ld(c.W + (4 + d) * N, idx);
+  cell_edge_core(c, d, p, from_s, wl, wc, wr, bn, dla, dra);
 }
 
 // edges and characteristic tracing of each cell; writes the two faces it owns
@@ -522,23 +531,45 @@ PMHD_FI void recon_b_term_3d(const Ctx<T>& c, int d, int bc, I3 p, int64_t e, T 
   out[1] = c.hdt * (v2 * term2);
 }
 
+// The reconstruction source term of cell m (ghost-mapped) on the line of faces
+// through f along d, added to the edge state of that cell at the face between m
+// and its neighbour: in 3D the two transverse-B terms of recon_b_term_3d; in 2D
+// the B_y (x faces) or B_x (y faces) term v_t dB_n/dn of the cell
+// (getByReconSourceTerms_2D / getBxReconSourceTerms_2D), from the normal field on
+// the cell's two faces.  The same value serves as the left term of face m+1 and
+// the right term of face m, so a sweep forms it once per cell.
+template <class T>
+PMHD_FI void cell_recon_src(const Ctx<T>& c, int d, I3 f, int64_t m, T out[2]) {
+  if (c.ndim == 3) {
+    recon_b_term_3d(c, d, d == 2 ? c.bc_zrec : c.bc[d], f, m, out);
+  } else if (c.ndim == 2) {
+    const int64_t N = c.ncell();
+    const int64_t n = c.n_along(d);
+    const int bc = c.bc[d];
+    I3 pm = with(f, d, map1(m, n, bc));
+    const int vslot = d == 0 ? 2 : 1;  // x faces use v_y, y faces use v_x
+    const T v = ld(c.W + vslot * N, c.cid(pm.j, pm.i, pm.k));
+    const int64_t flo = m < 0 ? (bc == OUTFLOW ? 0 : n - 1) : m;
+    const int64_t fhi = m + 1 > n ? (bc == OUTFLOW ? n : 1) : m + 1;
+    out[0] = c.hdt * v * ((face_val(c, c.fb[d], d, f, fhi) - face_val(c, c.fb[d], d, f, flo)) * c.invd[d]);
+    out[1] = T(0);
+  } else {
+    out[0] = out[1] = T(0);
+  }
+}
+
 // predictor of face f from its reconstructed states wl, wr (directional order;
-// modified in place)
+// modified in place) and the source terms of its left and right cells
+// (cell_recon_src of cells e-1 and e)
 template <int S, class T>
-PMHD_FI void predict_face(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7]) {
-  const int64_t N = c.ncell();
+PMHD_FI void predict_face_src(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7], const T cl[2], const T cr[2]) {
   const int64_t NF = c.nface(d);
   const int64_t idx = c.fid(d, f.j, f.i, f.k);
-  const int64_t e = co(f, d);
   floors_counted(c, 2 * d, wl, c.min_pres);
   floors_counted(c, 2 * d + 1, wr, c.min_pres);
   T bn;
   int ke = 0;
   if (c.ndim == 3) {
-    int bcs = d == 2 ? c.bc_zrec : c.bc[d];
-    T cl[2], cr[2];
-    recon_b_term_3d(c, d, bcs, f, e - 1, cl);
-    recon_b_term_3d(c, d, bcs, f, e, cr);
     wl[4] = wl[4] + cl[0];
     wl[5] = wl[5] + cl[1];
     wr[4] = wr[4] + cr[0];
@@ -546,25 +577,10 @@ PMHD_FI void predict_face(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7]) {
     bn = T(ld(c.fb[d], idx));
     ke = d;
   } else if (c.ndim == 2) {
-    // getByReconSourceTerms_2D / getBxReconSourceTerms_2D
-    const int64_t n = c.n_along(d);
-    const int bc = c.bc[d];
-    I3 pl = with(f, d, map1(e - 1, n, bc));
-    I3 pr = with(f, d, map1(e, n, bc));
-    int vslot = d == 0 ? 2 : 1;  // x faces use v_y, y faces use v_x
-    T lv = ld(c.W + vslot * N, c.cid(pl.j, pl.i, pl.k));
-    T rv = ld(c.W + vslot * N, c.cid(pr.j, pr.i, pr.k));
-    int64_t flo = e - 1 < 0 ? (bc == OUTFLOW ? 0 : n - 1) : e - 1;
-    int64_t fhi = e + 1 > n ? (bc == OUTFLOW ? n : 1) : e + 1;
-    T fm = T(ld(c.fb[d], idx));
-    T lf = face_val(c, c.fb[d], d, f, flo);
-    T rf = face_val(c, c.fb[d], d, f, fhi);
-    T c0 = c.hdt * lv * ((fm - lf) * c.invd[d]);
-    T c1 = c.hdt * rv * ((rf - fm) * c.invd[d]);
-    int slot = d == 0 ? 4 : 5;
-    wl[slot] = wl[slot] + c0;
-    wr[slot] = wr[slot] + c1;
-    bn = fm;
+    const int slot = d == 0 ? 4 : 5;
+    wl[slot] = wl[slot] + cl[0];
+    wr[slot] = wr[slot] + cr[0];
+    bn = T(ld(c.fb[d], idx));
   } else {
     bn = c.bx1d;
   }
@@ -581,6 +597,15 @@ PMHD_FI void predict_face(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7]) {
   roe_flux<S>(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf);
   from_rot(rf, d, df);
   store7(c.F[d], NF, idx, df);
+}
+
+template <int S, class T>
+PMHD_FI void predict_face(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7]) {
+  const int64_t e = co(f, d);
+  T cl[2], cr[2];
+  cell_recon_src(c, d, f, e - 1, cl);
+  cell_recon_src(c, d, f, e, cr);
+  predict_face_src<S>(c, d, f, wl, wr, cl, cr);
 }
 
 template <int S, class T>
@@ -605,41 +630,73 @@ PMHD_FI void k_predict(const Ctx<T>& c, int d, I3 f) {
 // outflow and the left edge of cell 0 on its right when periodic.  order 2
 // reads the slopes of the slope launch; order 1 forms them here.
 // ===========================================================================
-// the faces [f0, f1) of the line through q along d
+// the faces [f0, f1) of the line through q along d.  The run walks the cells
+// (logical indices, ghost-mapped) that those faces touch and keeps a window of
+// primitive vectors in registers: each cell's own and right neighbour's vectors
+// come from the previous step, and the vector two cells ahead is loaded one step
+// early, so its latency overlaps the current cell's arithmetic (the loads of the
+// window are issued before the step's stores, which the compiler could not move
+// them across).
 template <int S, class T>
 PMHD_FI void k_sweep_run(const Ctx<T>& c, int d, int bc, I3 q, int64_t f0, int64_t f1) {
   const int64_t n = c.n_along(d);
+  const int64_t N = c.ncell();
   const bool from_s = c.order == 2;
-  T prev[7];
-  {
+  const bool out = bc == OUTFLOW;
+  // rotated primitive vector (slots 0-6) and normal field (slot 7) of logical cell e
+  auto load_cell = [&](int64_t e, T w[8]) {
+    const I3 pe = with(q, d, map2(e, n, bc));
+    const int64_t id = c.cid(pe.j, pe.i, pe.k);
+    load_rot8(c.W, N, id, d, w);
+    w[7] = ld(c.W + (4 + d) * N, id);
+  };
+  // cells e0..e1: the left cell of face f0 (cell 0 itself for an outflow face 0)
+  // through the right cell of face f1-1 (cell n-1 for an outflow face n)
+  const int64_t e0 = (f0 == 0 && out) ? 0 : f0 - 1;
+  const bool last_out = f1 - 1 == n && out;
+  const int64_t e1 = last_out ? n - 1 : f1 - 1;
+  T wm[8], w0[8], wp[8];
+  load_cell(e0 - 1, wm);
+  load_cell(e0, w0);
+  load_cell(e0 + 1, wp);
+  T prev[7], src_prev[2] = {T(0), T(0)};
+  if (f0 == 0 && out) cell_recon_src(c, d, with(q, d, 0), -1, src_prev);
+  for (int64_t e = e0; e <= e1; ++e) {
+    T wn[8];
+    if (e < e1) load_cell(e + 2, wn);
     T la[7], ra[7];
-    if (f0 == 0 && bc == OUTFLOW) {
-      cell_edge_states(c, d, bc, with(q, d, 0), from_s, la, ra);
+    cell_edge_core(c, d, with(q, d, map2(e, n, bc)), from_s, wm, w0, wp, w0[7], la, ra);
+    T src[2];
+    cell_recon_src(c, d, with(q, d, e < 0 ? 0 : e), e, src);
+    if (e >= f0) {
+      T wl[7];
 #pragma unroll
-      for (int v = 0; v < 7; ++v) prev[v] = la[v];
-    } else {
-      cell_edge_states(c, d, bc, with(q, d, f0 == 0 ? n - 1 : f0 - 1), from_s, la, ra);
+      for (int v = 0; v < 7; ++v) wl[v] = (e == 0 && out) ? la[v] : prev[v];
+      T wr[7];
 #pragma unroll
-      for (int v = 0; v < 7; ++v) prev[v] = ra[v];
+      for (int v = 0; v < 7; ++v) wr[v] = la[v];
+      predict_face_src<S>(c, d, with(q, d, e), wl, wr, src_prev, src);
+    }
+#pragma unroll
+    for (int v = 0; v < 7; ++v) prev[v] = ra[v];
+    src_prev[0] = src[0];
+    src_prev[1] = src[1];
+    if (e < e1) {
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        wm[v] = w0[v];
+        w0[v] = wp[v];
+        wp[v] = wn[v];
+      }
     }
   }
-  for (int64_t f = f0; f < f1; ++f) {
-    T wl[7], wr[7];
+  if (last_out) {
+    // face n with outflow: the right edge state of cell n-1 on both sides
+    T wl[7], wr[7], src[2];
 #pragma unroll
-    for (int v = 0; v < 7; ++v) wl[v] = prev[v];
-    if (f < n || bc != OUTFLOW) {
-      T la[7], ra[7];
-      cell_edge_states(c, d, bc, with(q, d, f < n ? f : 0), from_s, la, ra);
-#pragma unroll
-      for (int v = 0; v < 7; ++v) {
-        wr[v] = la[v];
-        prev[v] = ra[v];
-      }
-    } else {
-#pragma unroll
-      for (int v = 0; v < 7; ++v) wr[v] = prev[v];
-    }
-    predict_face<S>(c, d, with(q, d, f), wl, wr);
+    for (int v = 0; v < 7; ++v) wl[v] = wr[v] = prev[v];
+    cell_recon_src(c, d, with(q, d, n), n, src);
+    predict_face_src<S>(c, d, with(q, d, n), wl, wr, src_prev, src);
   }
 }
 

@@ -387,6 +387,44 @@ PMHD_FI void eig_add_col(const PrimEig<T>& E, int i, T f, T x[7]) {
   }
 }
 
+#ifdef PMHD_FAST_ORDER
+// L . d from the waves' symmetric pairs: rows 0/6, 2/4 and 1/5 share their entries
+// up to the sign of the velocity terms, so each pair is one sum and one difference
+// of two partial dot products (23 operations instead of 34; rounds differently)
+template <class T>
+PMHD_FI void eig_project_pairs(const PrimEig<T>& E, const T d[7], T c[7]) {
+  const T a0 = E.l01 * d[1] + E.l02 * d[2] + E.l03 * d[3];
+  const T b0 = E.l06 * d[6] + E.l04 * d[4] + E.l05 * d[5];
+  const T a1 = E.l12 * d[2] + E.l13 * d[3];
+  const T b1 = E.l14 * d[4] + E.l15 * d[5];
+  const T a2 = E.l21 * d[1] + E.l22 * d[2] + E.l23 * d[3];
+  const T b2 = E.l26 * d[6] + E.l24 * d[4] + E.l25 * d[5];
+  c[0] = b0 + a0;
+  c[1] = b1 + a1;
+  c[2] = b2 + a2;
+  c[3] = d[0] + E.l36 * d[6];
+  c[4] = b2 - a2;
+  c[5] = b1 - a1;
+  c[6] = b0 - a0;
+}
+
+// x += sum_i f[i] * (column i of R), columns 0/6, 2/4 and 1/5 paired like the rows
+// of L (25 operations instead of 37; rounds differently)
+template <class T>
+PMHD_FI void eig_add_cols(const PrimEig<T>& E, const T f[7], T x[7]) {
+  const T s06 = f[0] + f[6], d06 = f[0] - f[6];
+  const T s24 = f[2] + f[4], d24 = f[2] - f[4];
+  const T s15 = f[1] + f[5], d15 = f[1] - f[5];
+  x[0] = x[0] + (s06 * E.r00 + s24 * E.r02 + f[3]);
+  x[1] = x[1] + (d06 * E.r10 + d24 * E.r12);
+  x[2] = x[2] + (d06 * E.r20 + d15 * E.r21 + d24 * E.r22);
+  x[3] = x[3] + (d06 * E.r30 + d15 * E.r31 + d24 * E.r32);
+  x[4] = x[4] + (s06 * E.r40 + s15 * E.r41 + s24 * E.r42);
+  x[5] = x[5] + (s06 * E.r50 + s15 * E.r51 + s24 * E.r52);
+  x[6] = x[6] + (s06 * E.r60 + s24 * E.r62);
+}
+#endif
+
 // characteristic-limited primitive slope of one cell (Colella 1990; Stone+ 2008 sec 4;
 // Source/Reconstruction_PPM_MHD_HelperModule.py:320-389)
 template <class T>
@@ -403,16 +441,17 @@ PMHD_FI void limited_slope(const PrimEig<T>& E, const T wl[7], const T wc[7], co
 #ifdef PMHD_FAST_ORDER
   // dc = dl + dr, so L.dc = L.dl + L.dr (one projection fewer; rounds differently)
   (void)dc;
-  eig_project(E, dl, lc);
-  eig_project(E, dr, rc);
+  eig_project_pairs(E, dl, lc);
+  eig_project_pairs(E, dr, rc);
 #pragma unroll
   for (int v = 0; v < 7; ++v) cc[v] = lc[v] + rc[v];
+  eig_project_pairs(E, wg, wc_);
 #else
   eig_project(E, dc, cc);
   eig_project(E, dl, lc);
   eig_project(E, dr, rc);
-#endif
   eig_project(E, wg, wc_);
+#endif
   T s[7];
 #pragma unroll
   for (int v = 0; v < 7; ++v) {
@@ -420,6 +459,12 @@ PMHD_FI void limited_slope(const PrimEig<T>& E, const T wl[7], const T wc[7], co
     T lim2 = npmin(T(0.5) * npabs(cc[v]), npabs(wc_[v]));
     s[v] = (lc[v] * rc[v] > T(0)) ? npsign(cc[v]) * npmin(T(2) * lim1, lim2) : T(0);
   }
+#ifdef PMHD_FAST_ORDER
+#pragma unroll
+  for (int v = 0; v < 7; ++v) ps[v] = T(0);
+  eig_add_cols(E, s, ps);
+  return;
+#endif
   // R . s, columns in wave order 0..6
   ps[0] = T(0) + s[0] * E.r00 + s[2] * E.r02 + s[3] * T(1) + s[4] * E.r02 + s[6] * E.r00;
   ps[1] = T(0) + s[0] * E.r10 + s[2] * E.r12 + s[4] * (-E.r12) + s[6] * (-E.r10);
@@ -486,34 +531,35 @@ PMHD_FI void trace_cell(const PrimEig<T>& E, const T wl[7], const T wc[7], const
       la[v] = la[v] + c4 * (c1[v] + (T(1) - T(kFourThirds) * c4) * c2[v]);
       ra[v] = ra[v] - c3 * (c1[v] - (T(1) - T(kFourThirds) * c3) * c2[v]);
     }
-    T p2[7];
-    eig_project(E, c1, p1);
-    eig_project(E, c2, p2);
+    T p2[7], fl[7], fr[7];
+    eig_project_pairs(E, c1, p1);
+    eig_project_pairs(E, c2, p2);
 #pragma unroll
     for (int i = 0; i < 7; ++i) {
       const T v2 = hdt * E.lam[i];
       const T v3l = vl - v2, v4l = T(kFourThirds) * (vl * vl - v2 * v2);
       const T v3r = vr - v2, v4r = T(kFourThirds) * (vr * vr - v2 * v2);
-      const T fl = E.lam[i] <= T(0) ? v3l * (p1[i] + p2[i]) + v4l * p2[i] : T(0);
-      const T fr = E.lam[i] >= T(0) ? v3r * (p1[i] - p2[i]) + v4r * p2[i] : T(0);
-      eig_add_col(E, i, fl, la);
-      eig_add_col(E, i, fr, ra);
+      fl[i] = E.lam[i] <= T(0) ? v3l * (p1[i] + p2[i]) + v4l * p2[i] : T(0);
+      fr[i] = E.lam[i] >= T(0) ? v3r * (p1[i] - p2[i]) + v4r * p2[i] : T(0);
     }
+    eig_add_cols(E, fl, la);
+    eig_add_cols(E, fr, ra);
   } else {
 #pragma unroll
     for (int v = 0; v < 7; ++v) {
       la[v] = la[v] + c4 * c1[v];
       ra[v] = ra[v] - c3 * c1[v];
     }
-    eig_project(E, c1, p1);
+    T fl[7], fr[7];
+    eig_project_pairs(E, c1, p1);
 #pragma unroll
     for (int i = 0; i < 7; ++i) {
       const T v2 = hdt * E.lam[i];
-      const T fl = E.lam[i] <= T(0) ? (vl - v2) * p1[i] : T(0);
-      const T fr = E.lam[i] >= T(0) ? (vr - v2) * p1[i] : T(0);
-      eig_add_col(E, i, fl, la);
-      eig_add_col(E, i, fr, ra);
+      fl[i] = E.lam[i] <= T(0) ? (vl - v2) * p1[i] : T(0);
+      fr[i] = E.lam[i] >= T(0) ? (vr - v2) * p1[i] : T(0);
     }
+    eig_add_cols(E, fl, la);
+    eig_add_cols(E, fr, ra);
   }
 #else
 #pragma unroll
@@ -789,6 +835,64 @@ PMHD_HD inline void roe_flux(const T wl[7], const T wr[7], const T ul[7], const 
   }
   T sb = vt1 * b1s + vt2 * b2s;
   T bss = b1s * b1s + b2s * b2s;
+  if constexpr (S == RIEMANN_ROE && fast_f32<T>()) {
+    // FP32 fast unit: the wave pairs (0, 6), (2, 4) and (1, 5) have eigenvectors
+    // that differ only in the sign of their velocity terms, so the projections are
+    // sums and differences of shared partial dot products and the eigenvector sum
+    // runs over the pair sums and differences; the eigenvector entries themselves
+    // are never formed (the same flux in exact arithmetic; rounds differently)
+    T dU[7];
+#pragma unroll
+    for (int v = 0; v < 7; ++v) dU[v] = ur[v] - ul[v];
+    const T nf = T(0.5) / cs2;
+    const T cff = nf * Af_ * cf, css = nf * As_ * csl;
+    const T qfn = qf * nf, qsn = qs * nf;
+    const T a_f = nf * afp * rd, a_s = nf * asp * rd;
+    const T afbn = nf * afp * bts, asbn = nf * asp * bts;
+    const T nfg = nf * (g - T(1));
+    const T afn = Af_ * nfg, asn = As_ * nfg;
+    const T ibss = T(1) / bss;
+    const T b1c = b1s * ibss, b2c = b2s * ibss;
+    const T sbs = vt1 * b1c + vt2 * b2c;
+    const T G = (vsq - hmb) * dU[0] - vn * dU[1] - vt1 * dU[2] - vt2 * dU[3] + dU[6];
+    const T Hd = vn * dU[0] - dU[1];
+    const T K = b1c * dU[2] + b2c * dU[3] - sbs * dU[0];
+    const T P0 = afn * G + (cff * cf - asbn) * dU[0] + (a_s * b1c - afn * bt1) * dU[4] +
+                 (a_s * b2c - afn * bt2) * dU[5];
+    const T Q0 = cff * Hd + qsn * K;
+    const T P2 = asn * G + (css * csl + afbn) * dU[0] + (-a_f * b1c - asn * bt1) * dU[4] +
+                 (-a_f * b2c - asn * bt2) * dU[5];
+    const T Q2 = css * Hd - qfn * K;
+    const T A1 = T(0.5) * (vt1 * b2 - vt2 * b1) * dU[0] + T(-0.5) * b2 * dU[2] + T(0.5) * b1 * dU[3];
+    const T B1 = T(0.5) * srd * sg * (b1 * dU[5] - b2 * dU[4]);
+    const T nf2 = T(2) * nfg;
+    const T r63 = T(0.5) * vsq + (g - T(2)) * btc / (g - T(1));
+    const T p3 = (T(1) - nf2 * (T(0.5) * vsq - (g - T(2)) * btc / (g - T(1)))) * dU[0] +
+                 nf2 * (vn * dU[1] + vt1 * dU[2] + vt2 * dU[3] - dU[6] + bt1 * dU[4] + bt2 * dU[5]);
+    const T a0 = T(0.5) * npabs(lam[0]) * (P0 + Q0);
+    const T a1 = T(0.5) * npabs(lam[1]) * (B1 + A1);
+    const T a2 = T(0.5) * npabs(lam[2]) * (P2 + Q2);
+    const T a3 = T(0.5) * npabs(lam[3]) * p3;
+    const T a4 = T(0.5) * npabs(lam[4]) * (P2 - Q2);
+    const T a5 = T(0.5) * npabs(lam[5]) * (B1 - A1);
+    const T a6 = T(0.5) * npabs(lam[6]) * (P0 - Q0);
+    const T s06 = a0 + a6, d06 = a0 - a6, s24 = a2 + a4, d24 = a2 - a4, s15 = a1 + a5, d15 = a1 - a5;
+    const T X0 = Af_ * s06 + As_ * s24 + a3;
+    const T Y = qs * d06 - qf * d24;
+    const T Z = asp * s06 - afp * s24;
+    const T r41 = -b2 * sg * isrd, r51 = b1 * sg * isrd;
+    const T r61 = -(vt1 * b2 - vt2 * b1);
+    F[0] = T(0.5) * (FL[0] + FR[0]) - X0;
+    F[1] = T(0.5) * (FL[1] + FR[1]) -
+           (a0 * (Af_ * lam[0]) + a2 * (As_ * lam[2]) + a3 * vn + a4 * (As_ * lam[4]) + a6 * (Af_ * lam[6]));
+    F[2] = T(0.5) * (FL[2] + FR[2]) - (vt1 * X0 + b1s * Y - b2 * d15);
+    F[3] = T(0.5) * (FL[3] + FR[3]) - (vt2 * X0 + b2s * Y + b1 * d15);
+    F[4] = T(0.5) * (FL[4] + FR[4]) - (b1s * Z + r41 * s15);
+    F[5] = T(0.5) * (FL[5] + FR[5]) - (b2s * Z + r51 * s15);
+    F[6] = T(0.5) * (FL[6] + FR[6]) - (hmb * (X0 - a3) - vn * (Af_ * cf * d06 + As_ * csl * d24) + sb * Y +
+                                       bts * bss * Z + r61 * d15 + a3 * r63);
+    return;
+  }
   T afb = afp * bts * bss;
   T asb = asp * bts * bss;
   // right eigenvectors (conserved), structurally non-zero entries
