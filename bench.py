@@ -80,11 +80,13 @@ def run_gpu(prim, cons, faces, n, precision, steps, warmup, profile_steps=3, swe
     return s, el, prof, cyc + profile_steps
 
 
-def run_to_time(shape, widths, bcs, gamma, cfl, order, precision, prim, cons, faces, t_end):
-    """One handle run from t = 0 to t_end (state uploaded before the clock starts;
-    the host reads dt each cycle, as the reference's loop does)."""
+def run_to_time(shape, widths, bcs, gamma, cfl, order, precision, prim, cons, faces, t_end, clock='auto'):
+    """One handle run from t = 0 to t_end (state uploaded before the clock starts).
+    clock 'host': the host reads dt each cycle, as the reference's loop does;
+    'device': dt and the loop condition stay on the device."""
     from pythonmhd_amd._lib import Solver
     s = Solver(shape, widths, bcs, gamma, cfl, order=order, entropy_fix=False, precision=precision)
+    s.set_clock(clock)
     s.set_state(prim, cons, faces)
     s.synchronize()
     t0 = time.perf_counter()
@@ -103,24 +105,40 @@ def run_configs():
     out = {}
     n = 512
     prim, cons, _ = brio_wu(n)
-    run_to_time((n,), (1.0 / n,), (0,), 2.0, 0.8, 1, 64, prim, cons, (), 0.01)   # warm-up
-    t, cyc, el, _ = run_to_time((n,), (1.0 / n,), (0,), 2.0, 0.8, 1, 64, prim, cons, (), 0.1)
-    out["brio_wu"] = {"grid": "512 cells, outflow, gamma 2, PLM, CFL 0.8, FP64", "t_end": t, "cycles": cyc,
-                      "seconds": el, "cell_updates_per_s": n * cyc / el}
+    # each configuration with the clock pmhd_set_clock's default picks for it (the
+    # device for grids of at most 2^16 cells) and, beside it, with the other clock
+    auto = lambda cells: 'device' if cells <= 1 << 16 else 'host'
+    bw = {"grid": "512 cells, outflow, gamma 2, PLM, CFL 0.8, FP64", "clock": auto(n)}
+    for clock in ('device', 'host'):
+        run_to_time((n,), (1.0 / n,), (0,), 2.0, 0.8, 1, 64, prim, cons, (), 0.01, clock)   # warm-up
+        t, cyc, el, _ = run_to_time((n,), (1.0 / n,), (0,), 2.0, 0.8, 1, 64, prim, cons, (), 0.1, clock)
+        r = {"t_end": t, "cycles": cyc, "seconds": el, "cell_updates_per_s": n * cyc / el}
+        if clock == bw["clock"]:
+            bw.update(r)
+        else:
+            bw[clock + "_clock"] = r
+    out["brio_wu"] = bw
     prim, cons, faces = orszag_tang(n)
-    ot = {"grid": "512^2 periodic, gamma 5/3, PPM, CFL 0.8", "t_end": 0.5}
+    ot = {"grid": "512^2 periodic, gamma 5/3, PPM, CFL 0.8", "t_end": 0.5, "clock": auto(n * n)}
     rho = {}
     for prec in (32, 64):
-        run_to_time((n, n), (1.0 / n,) * 2, (1, 1), GAMMA, 0.8, 2, prec, prim, cons, faces, 0.005)  # warm-up
-        t, cyc, el, st = run_to_time((n, n), (1.0 / n,) * 2, (1, 1), GAMMA, 0.8, 2, prec, prim, cons, faces, 0.5)
-        ot["fp%d" % prec] = {"cycles": cyc, "seconds": el, "cell_updates_per_s": n * n * cyc / el}
-        rho[prec] = st[1][0]
+        res = {}
+        for clock in ('device', 'host'):
+            run_to_time((n, n), (1.0 / n,) * 2, (1, 1), GAMMA, 0.8, 2, prec, prim, cons, faces, 0.005,
+                        clock)  # warm-up
+            t, cyc, el, st = run_to_time((n, n), (1.0 / n,) * 2, (1, 1), GAMMA, 0.8, 2, prec, prim, cons, faces,
+                                         0.5, clock)
+            res[clock] = {"cycles": cyc, "seconds": el, "cell_updates_per_s": n * n * cyc / el}
+            if clock == ot["clock"]:
+                rho[prec] = st[1][0]
+        other = 'host' if ot["clock"] == 'device' else 'device'
+        ot["fp%d" % prec] = dict(res[ot["clock"]], **{other + "_clock": res[other]})
     ot["l1_rho_fp32_vs_fp64"] = float(np.mean(np.abs(rho[32] - rho[64])))
     out["orszag_tang"] = ot
     return out
 
 
-def run_kh(cycles=200, seq_cycles=50, nbox=256):
+def run_kh(cycles=200, seq_cycles=200, nbox=256):
     """The batch of KH boxes: one batch launch per phase (tile-tuned) vs one
     handle per box advanced box after box (per-box launches and dt reads)."""
     from pythonmhd_amd._lib import Batch, Solver
@@ -145,21 +163,34 @@ def run_kh(cycles=200, seq_cycles=50, nbox=256):
     tiles = b.tiles()
     b.close()
     assert int(cyc.min()) == cycles
-    t0 = time.perf_counter()
+    # the same boxes one handle each, advanced box after box for the same cycle
+    # count, each with its clock on the device (no dt read per cycle); handles are
+    # made and loaded before the clock starts
+    handles = []
     for n, prim, cons, faces in boxes:
         s = Solver((n, n), (1.0 / n, 1.0 / n), (1, 1), GAMMA, 0.8, order=1, entropy_fix=False, precision=32)
+        s.set_clock('device')
         s.set_state(prim, cons, faces)
-        s.advance(0.0, 1e30, 0, seq_cycles)
+        handles.append(s)
+    handles[0].advance(0.0, 1e30, 0, 2)   # warm-up
+    handles[0].set_state(*boxes[0][1:])
+    for s in handles:
         s.synchronize()
-        s.close()
+    t0 = time.perf_counter()
+    for s in handles:
+        s.advance(0.0, 1e30, 0, seq_cycles)
+    for s in handles:
+        s.synchronize()
     el_seq = time.perf_counter() - t0
+    for s in handles:
+        s.close()
     v = cells * cycles / el
     vs = cells * seq_cycles / el_seq
     return {"boxes": nbox, "sides": "U{64..256} seed 7", "cells": cells, "cycles": cycles,
             "value": v, "unit": "cell-updates/s", "seconds": el, "autotune_seconds": tune_s,
             "tiles": {k: list(t) for k, t in sorted(tiles.items())},
             "sequential": {"value": vs, "cycles": seq_cycles, "seconds": el_seq,
-                           "how": "one handle per box, box after box, host dt read per cycle"},
+                           "how": "one handle per box, box after box, each with its clock on the device"},
             "speedup_vs_sequential": v / vs}
 
 
@@ -271,6 +302,33 @@ def cpu_baseline(n=64, cycles=8):
     return {"value": n ** 3 * cycles / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
             "sample": "FP64 NumPy oracle, same workload at %d^3, %d cycles after 1 warm-up cycle, "
                       "1 thread (host: %s, os.cpu_count()=%d)" % (n, cycles, _cpu_model(), os.cpu_count())}
+
+
+def cpu_openmp_baseline(n=128, cycles=3):
+    """The same kernel bodies compiled for the host with g++ -O2 and OpenMP (the
+    build the CPU tests run; FP64, bit for bit the oracle's arithmetic) on all the
+    host threads OpenMP is given: a multi-core CPU figure beside the single-thread
+    NumPy oracle."""
+    from pythonmhd_amd._lib import Solver, load
+    lib_path = os.path.join(ROOT, 'tests', 'hostbuild', 'libpmhd_host.so')
+    if not os.path.exists(lib_path):
+        return None
+    lib = load(lib_path)
+    prim, cons, faces = turbulence(n)
+    h = 1.0 / n
+    s = Solver((n, n, n), (h, h, h), (1, 1, 1), GAMMA, 0.4, order=1, entropy_fix=False, precision=64, lib=lib)
+    s.set_clock('host')
+    s.set_state(prim, cons, faces)
+    s.advance(0.0, 1e30, 0, 1)   # warm-up
+    t0 = time.perf_counter()
+    s.advance(0.0, 1e30, 1, 1 + cycles)
+    el = time.perf_counter() - t0
+    s.close()
+    threads = int(os.environ.get('OMP_NUM_THREADS', '0') or 0) or os.cpu_count()
+    return {"value": n ** 3 * cycles / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": "the step's own kernel bodies built for the host (g++ -O2, OpenMP, FP64), same workload at "
+                      "%d^3, %d cycles after 1 warm-up cycle, %d threads (host: %s)" % (n, cycles, threads,
+                                                                                      _cpu_model())}
 
 
 def _cpu_model():
@@ -428,6 +486,11 @@ def main():
         out["adjoint"] = run_adjoint(n=a.adj_n, steps=a.adj_steps)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+        omp = cpu_openmp_baseline()
+        if omp is not None:
+            out["cpu_baseline_openmp"] = omp
+            print('bench: cpu OpenMP baseline %.3e cell-updates/s on %d threads' % (omp["value"], omp["cores"]),
+                  file=sys.stderr, flush=True)
         print('bench: cpu baseline %.3e cell-updates/s' % out["cpu_baseline"]["value"], file=sys.stderr,
               flush=True)
     print(json.dumps(out))

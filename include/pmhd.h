@@ -147,6 +147,22 @@ int pmhd_step(pmhd_handle* h, double dt);
 int pmhd_advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int64_t max_cycles,
                  double* t_out, int64_t* cycle_out, double* t_second_last);
 
+/* Where pmhd_advance keeps the clock (no counterpart in the reference, whose loop
+ * is Source/EvolveFunctions_MHD.py:252-323, 465-701, 844-1163: a host loop that
+ * reads dt every cycle).  PMHD_CLOCK_HOST: the host reads each cycle's CFL maxima
+ * back and forms dt, one round trip per cycle.  PMHD_CLOCK_DEVICE: dt, its
+ * clipping to t_end, the loop condition, the time and the cycle count stay on the
+ * device (the grid runs as a batch of one box), so cycles are queued without a
+ * round trip and the host looks at the clock every 16 cycles only to stop early;
+ * the results are the same (bit for bit in FP64).  PMHD_CLOCK_AUTO (the default):
+ * the device clock for grids of at most 2^16 cells, whose cycles are short enough
+ * for the round trip to show (larger grids run faster in the single-grid kernels
+ * than as a one-box batch).  PMHD_EINVAL for another mode. */
+#define PMHD_CLOCK_AUTO 0
+#define PMHD_CLOCK_HOST 1
+#define PMHD_CLOCK_DEVICE 2
+int pmhd_set_clock(pmhd_handle* h, int32_t mode);
+
 /* Faces per thread of the fused predictor sweep along x, y and z (no counterpart
  * in the reference; a tuning knob).  With a run length s > 0 one launch per
  * direction reconstructs the cells (PLM slopes, or the PPM slopes of a separate

@@ -14,7 +14,7 @@ DEFAULT_LIB = os.path.join(_HERE, 'libpmhd.so')
 
 EXPORTS = ('pmhd_create', 'pmhd_destroy', 'pmhd_set_state', 'pmhd_get_state', 'pmhd_set_state_device',
            'pmhd_get_state_device', 'pmhd_set_stream', 'pmhd_cfl_dt_noalloc', 'pmhd_compute_dt',
-           'pmhd_step', 'pmhd_set_sweep', 'pmhd_advance', 'pmhd_synchronize', 'pmhd_diagnostics',
+           'pmhd_step', 'pmhd_set_sweep', 'pmhd_set_clock', 'pmhd_advance', 'pmhd_synchronize', 'pmhd_diagnostics',
            'pmhd_set_profiling',
            'pmhd_get_profile', 'pmhd_get_stats', 'pmhd_riemann_flux', 'pmhd_last_error',
            'pmhd_batch_create', 'pmhd_batch_destroy', 'pmhd_batch_set_state', 'pmhd_batch_get_state',
@@ -87,6 +87,7 @@ def _declare(lib):
     lib.pmhd_compute_dt.argtypes = [H, ctypes.POINTER(ctypes.c_double)]
     lib.pmhd_step.argtypes = [H, ctypes.c_double]
     lib.pmhd_set_sweep.argtypes = [H, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    lib.pmhd_set_clock.argtypes = [H, ctypes.c_int32]
     lib.pmhd_advance.argtypes = [H, ctypes.c_double, ctypes.c_double, ctypes.c_int64, ctypes.c_int64,
                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
                                  ctypes.POINTER(ctypes.c_double)]
@@ -375,6 +376,14 @@ class Solver:
         separate slope, edge and predictor launches (same results)."""
         seg = list(seg) + [0] * (3 - len(seg))
         self._check(self.lib.pmhd_set_sweep(self.h, int(seg[0]), int(seg[1]), int(seg[2])), 'pmhd_set_sweep')
+
+    CLOCKS = {'auto': 0, 'host': 1, 'device': 2}
+
+    def set_clock(self, mode):
+        """Where advance() keeps the clock: 'host' (dt read back every cycle),
+        'device' (dt, clipping and the loop condition on the device; same results) or
+        'auto' (the device for grids of at most 2^16 cells)."""
+        self._check(self.lib.pmhd_set_clock(self.h, int(self.CLOCKS[mode])), 'pmhd_set_clock')
 
     def advance(self, t0, t_end, cycle0, max_cycles, allow_nonfinite=False):
         """The evolve loop; returns (t, cycle, t of the second-last state).  A state

@@ -37,6 +37,10 @@ struct pmhd_handle {
   virtual int set_state_device(const double*, const double*, const double*, const double*, const double*) = 0;
   virtual int get_state_device(int, double*, double*, double*, double*, double*) = 0;
   virtual int set_stream(void* stream) = 0;
+  // the evolve loop with the clock on the device (no host round trip per cycle)
+  virtual int advance_clocked(double t0, double t_end, int64_t cycle0, int64_t max_cycles, double* t_out,
+                              int64_t* cycle_out, double* t2_out) = 0;
+  int clock = 0;   // PMHD_CLOCK_AUTO, PMHD_CLOCK_HOST or PMHD_CLOCK_DEVICE
   int fail(int code, const std::string& m) {
     err = m;
     return code;
@@ -124,6 +128,7 @@ struct BatchLayout {
   int total[K_COUNT] = {};
   int64_t elems[K_COUNT] = {};
   float tuned_ms[K_COUNT];
+  int line_block = 64;                // threads per workgroup of the line sweeps
   std::function<void(int)> upload;    // copies hprefix[id] to the device
 
   BatchLayout() {
@@ -423,6 +428,97 @@ void store_state(B& be, const Ctx<T>& c, const pmhd_grid& g, const T* U, const T
 }
 
 // ---------------------------------------------------------------------------
+// device-clocked runs: the launch tables of a batch of boxes (also used for one
+// grid whose clock stays on the device) and the loop that advances them
+// ---------------------------------------------------------------------------
+// Allocate (once: a layout built again for the same boxes keeps its memory) the
+// device contexts, clocks, CFL maxima and tile tables of the boxes hctx (bound to
+// their buffers), fill the tables with the default tiles and copy the contexts to
+// the device.  dalloc(bytes) returns device memory or throws std::bad_alloc.
+// false when a launch would exceed 2^31 tiles.
+template <class T, class B, class A>
+bool init_layout(B& be, BatchLayout& lay, std::vector<Ctx<T>>& hctx, A&& dalloc) {
+  const int n = int(hctx.size());
+  if (!lay.dctx || lay.nbox != n) {
+    lay.dctx = dalloc(sizeof(Ctx<T>) * size_t(n));
+    lay.dclk = static_cast<BoxClock*>(dalloc(sizeof(BoxClock) * size_t(n)));
+    lay.dred = static_cast<unsigned long long*>(dalloc(3 * sizeof(unsigned long long) * size_t(n)));
+    lay.dprefix = static_cast<int*>(dalloc(sizeof(int) * size_t(K_COUNT) * size_t(n + 1)));
+  }
+  lay.nbox = n;
+  be.zero(lay.dred, 3 * sizeof(unsigned long long) * size_t(n));
+  lay.ndim = hctx[0].ndim;
+  lay.cus = be.compute_units();
+  lay.ext.assign(size_t(K_COUNT) * size_t(n) * 3, 0);
+  lay.max_n2 = 1;
+  for (int id = 0; id < K_COUNT; ++id) {
+    int kind, dir;
+    phase_shape(id, kind, dir);
+    for (int b = 0; b < n; ++b) {
+      int64_t e[3];
+      hctx[size_t(b)].extent(kind, dir, e);
+      for (int q = 0; q < 3; ++q) lay.ext[(size_t(id) * size_t(n) + size_t(b)) * 3 + size_t(q)] = e[q];
+      // launches along a direction the boxes do not have never run
+      if (dir < lay.ndim) lay.max_n2 = std::max<int64_t>(lay.max_n2, e[2]);
+    }
+  }
+  lay.hprefix.assign(size_t(K_COUNT) * size_t(n + 1), 0);
+  BatchLayout* L = &lay;
+  B* bp = &be;
+  lay.upload = [L, bp](int id) {
+    bp->h2d(L->dprefix + size_t(id) * size_t(L->nbox + 1), L->hprefix.data() + size_t(id) * size_t(L->nbox + 1),
+            sizeof(int) * size_t(L->nbox + 1));
+  };
+  for (int id = 0; id < K_COUNT; ++id) {
+    int kind, dir;
+    phase_shape(id, kind, dir);
+    // line launches: one tile per line
+    const TileCfg t = kind == EXT_LINE ? TileCfg{1, 1, 0} : default_tile(lay);
+    if (!lay.apply_tile(id, t)) return false;
+  }
+  be.h2d(lay.dctx, hctx.data(), sizeof(Ctx<T>) * size_t(n));
+  return true;
+}
+
+// Cycles of the boxes of lay with their clocks on the device: each cycle's begin
+// launch takes every box's CFL maxima, decides whether the box still runs (the
+// reference's loop condition), clips its dt and forms its scalars, and the end
+// launch swaps its buffers and advances its clock.  The host reads the clocks back
+// every 16 cycles only to stop early.  The clocks hclk are uploaded first and read
+// back at the end; c0 supplies the launch-invariant decisions.  ec_first: the
+// boxes' cell E of the current state is already stored (the first cycle skips it).
+template <class T, class B>
+void clocked_loop(B& be, BatchLayout& lay, Ctx<T>& c0, std::vector<BoxClock>& hclk, double cfl, double t_end,
+                  int64_t max_cycles, int64_t iters, bool ec_first) {
+  be.h2d(lay.dclk, hclk.data(), sizeof(BoxClock) * size_t(lay.nbox));
+  auto& bx = be.batch_exec(&lay);
+  bx.template activate_all<T>();
+  const int ec = c0.ec_valid;
+  auto one_cycle = [&] {
+    bx.template begin<T>(cfl, t_end, max_cycles, 1);
+    run_cycle(bx, c0);
+    bx.template end<T>();
+  };
+  for (int64_t it = 0; it < iters; ++it) {
+    // the first cycle forms the cell E of the set state unless it is stored; later
+    // ones take the one each box's update stored
+    c0.ec_valid = it > 0 || ec_first;
+    // later cycles take their speeds from the previous update's epilogue
+    if (it == 0) bx.cfl(c0);
+    one_cycle();
+    if ((it + 1) % 16 == 0 && it + 1 < iters) {
+      be.d2h(hclk.data(), lay.dclk, sizeof(BoxClock) * size_t(lay.nbox));
+      bool any = false;
+      for (const BoxClock& k : hclk) any = any || (k.t < t_end && k.cyc < max_cycles);
+      if (!any) break;
+    }
+    if (be.error()) break;
+  }
+  c0.ec_valid = ec;
+  be.d2h(hclk.data(), lay.dclk, sizeof(BoxClock) * size_t(lay.nbox));
+}
+
+// ---------------------------------------------------------------------------
 // one grid
 // ---------------------------------------------------------------------------
 template <class T, class B>
@@ -435,6 +531,9 @@ struct Handle : public pmhd_handle {
   int64_t cycles_since_set = 0;
   bool speeds_ready = false;   // the last update left the new state's CFL maxima
   bool ec_ready = false;       // the last update left the new state's cell E
+  // the grid as a one-box batch for the device-clocked loop (built on first use)
+  BatchLayout clay;
+  bool clay_ready = false;
 
   T* dalloc(int64_t n) {
     if (n <= 0) n = 1;
@@ -475,6 +574,7 @@ struct Handle : public pmhd_handle {
                                  "the contiguous axis (z in 3D, x in 2D)");
     }
     for (int d = 0; d < 3; ++d) c.seg[d] = d < g.ndim ? seg[d] : 0;
+    clay_ready = false;
     return PMHD_OK;
   }
 
@@ -535,6 +635,57 @@ struct Handle : public pmhd_handle {
     be.set_stream(stream);
     const char* e = be.error();
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
+  }
+
+  // The evolve loop with the clock on the device: the grid runs as a batch of one
+  // box over its own buffers (the batch kernels read the context from device
+  // memory, where each cycle's begin launch forms dt and its scalars), so cycles
+  // are queued without a host round trip.  The results are those of the host loop
+  // (bit for bit in FP64).
+  int advance_clocked(double t0, double t_end, int64_t cycle0, int64_t max_cycles, double* t_out,
+                      int64_t* cycle_out, double* t2_out) override {
+    bind_bufs(c, sb, cur);
+    c.active = 1;
+    if (!clay_ready) {
+      // (again after pmhd_set_sweep: the run lengths set the launches' index boxes)
+      std::vector<Ctx<T>> one(1, c);
+      try {
+        if (!init_layout(be, clay, one, [&](size_t bytes) { return static_cast<void*>(dalloc(int64_t(
+                                                                  (bytes + sizeof(T) - 1) / sizeof(T)))); }))
+          return fail(PMHD_EINVAL, "grid too large for the device-clocked loop (more than 2^31 tiles)");
+      } catch (const std::bad_alloc&) {
+        return fail(PMHD_ENOMEM, "device allocation failed");
+      }
+      // line sweeps: the single grid's workgroup (the smallest of 64, 128, 256
+      // threads that covers the line, else 256 in chunks), not the batch's 64
+      const int ax = g.ndim == 3 ? 2 : 0;
+      int lb = 64;
+      while (lb < 256 && int64_t(lb) < c.n_along(ax)) lb <<= 1;
+      clay.line_block = lb;
+      clay_ready = true;
+    } else {
+      be.h2d(clay.dctx, &c, sizeof(Ctx<T>));
+    }
+    std::vector<BoxClock> k(1, BoxClock{t0, t0, 0.0, cycle0, 0, 0});
+    clocked_loop(be, clay, c, k, p.cfl, t_end, max_cycles, max_cycles - cycle0, ec_ready);
+    const char* e = be.error();
+    if (e) return fail(PMHD_EHIP, e);
+    const int64_t done = k[0].done;
+    if (done > 0) {
+      cur = int((cur + done) & 1);
+      bind_bufs(c, sb, cur);
+      cycles_since_set += done;
+      // the speeds went to the batch's maxima, not to this grid's partials
+      speeds_ready = false;
+      ec_ready = c.ec_next != 0;
+    }
+    if (t_out) *t_out = k[0].t;
+    if (cycle_out) *cycle_out = k[0].cyc;
+    if (t2_out) *t2_out = k[0].t2;
+    if (k[0].nonfinite)
+      return fail(PMHD_ENONFINITE, "the state became non-finite (NaN timestep); the results are those of the "
+                                   "reference's loop, which ends with a NaN time");
+    return PMHD_OK;
   }
 
   int compute_dt(double* dt) override {
@@ -646,7 +797,7 @@ struct BatchHandle : public pmhd_batch {
     grids.assign(gs, gs + n);
     hctx.resize(size_t(n));
     bufs.resize(size_t(n));
-    hclk.assign(size_t(n), BoxClock{0, 0, 0, 0, 0});
+    hclk.assign(size_t(n), BoxClock{0, 0, 0, 0, 0, 0});
     for (int b = 0; b < n; ++b) init_ctx(hctx[size_t(b)], grids[size_t(b)], p);
     try {
       // one arena for all boxes: a sizing pass, then the carving pass
@@ -668,11 +819,6 @@ struct BatchHandle : public pmhd_batch {
           arena = dalloc_bytes(total);
         }
       }
-      lay.nbox = n;
-      lay.dctx = dalloc_bytes(sizeof(Ctx<T>) * size_t(n));
-      lay.dclk = static_cast<BoxClock*>(dalloc_bytes(sizeof(BoxClock) * size_t(n)));
-      lay.dred = static_cast<unsigned long long*>(dalloc_bytes(3 * sizeof(unsigned long long) * size_t(n)));
-      lay.dprefix = static_cast<int*>(dalloc_bytes(sizeof(int) * size_t(K_COUNT) * size_t(n + 1)));
     } catch (const std::bad_alloc&) {
       return fail(PMHD_ENOMEM, "device allocation failed");
     }
@@ -694,37 +840,13 @@ struct BatchHandle : public pmhd_batch {
       hctx[size_t(b)].ec_next = p.physics == 0 && grids[0].ndim >= 2;
       for (int d = 0; d < 3; ++d) hctx[size_t(b)].seg[d] = seg[d];
     }
-    be.zero(lay.dred, 3 * sizeof(unsigned long long) * size_t(n));
     for (int b = 0; b < n; ++b) be.zero(hctx[size_t(b)].fcnt, FLOOR_WORDS * sizeof(unsigned long long));
-    // tile tables
-    lay.ndim = grids[0].ndim;
-    lay.cus = be.compute_units();
-    lay.ext.assign(size_t(K_COUNT) * size_t(n) * 3, 0);
-    lay.max_n2 = 1;
-    for (int id = 0; id < K_COUNT; ++id) {
-      int kind, dir;
-      phase_shape(id, kind, dir);
-      for (int b = 0; b < n; ++b) {
-        int64_t e[3];
-        hctx[size_t(b)].extent(kind, dir, e);
-        for (int q = 0; q < 3; ++q) lay.ext[(size_t(id) * size_t(n) + size_t(b)) * 3 + size_t(q)] = e[q];
-        // launches along a direction the boxes do not have never run
-        if (dir < lay.ndim) lay.max_n2 = std::max<int64_t>(lay.max_n2, e[2]);
-      }
+    try {
+      if (!init_layout(be, lay, hctx, [&](size_t bytes) { return dalloc_bytes(bytes); }))
+        return fail(PMHD_EINVAL, "batch too large: more than 2^31 tiles");
+    } catch (const std::bad_alloc&) {
+      return fail(PMHD_ENOMEM, "device allocation failed");
     }
-    lay.hprefix.assign(size_t(K_COUNT) * size_t(n + 1), 0);
-    lay.upload = [this](int id) {
-      be.h2d(lay.dprefix + size_t(id) * size_t(nbox + 1), lay.hprefix.data() + size_t(id) * size_t(nbox + 1),
-             sizeof(int) * size_t(nbox + 1));
-    };
-    for (int id = 0; id < K_COUNT; ++id) {
-      int kind, dir;
-      phase_shape(id, kind, dir);
-      // line launches: one tile per line
-      const TileCfg t = kind == EXT_LINE ? TileCfg{1, 1, 0} : default_tile(lay);
-      if (!lay.apply_tile(id, t)) return fail(PMHD_EINVAL, "batch too large: more than 2^31 tiles");
-    }
-    push_ctx();
     const char* e = be.error();
     if (e) return fail(PMHD_EHIP, e);
     return PMHD_OK;
@@ -778,30 +900,9 @@ struct BatchHandle : public pmhd_batch {
       k.t2 = t0;
       k.dt = 0;
       k.cyc = cycle0;
+      k.nonfinite = 0;
     }
-    be.h2d(lay.dclk, hclk.data(), sizeof(BoxClock) * size_t(nbox));
-    auto& bx = be.batch_exec(&lay);
-    bx.template activate_all<T>();
-    const int64_t iters = max_cycles - cycle0;
-    for (int64_t it = 0; it < iters; ++it) {
-      // later cycles take their speeds from the previous update's epilogue
-      if (it == 0) bx.cfl(hctx[0]);
-      bx.template begin<T>(p.cfl, t_end, max_cycles, 1);
-      // the first cycle forms the cell E of the set state; later ones take the one
-      // each box's update stored
-      hctx[0].ec_valid = it > 0;
-      run_cycle(bx, hctx[0]);
-      bx.template end<T>();
-      if ((it + 1) % 16 == 0 && it + 1 < iters) {
-        be.d2h(hclk.data(), lay.dclk, sizeof(BoxClock) * size_t(nbox));
-        bool any = false;
-        for (const BoxClock& k : hclk) any = any || (k.t < t_end && k.cyc < max_cycles);
-        if (!any) break;
-      }
-      const char* e = be.error();
-      if (e) return fail(PMHD_EHIP, e);
-    }
-    be.d2h(hclk.data(), lay.dclk, sizeof(BoxClock) * size_t(nbox));
+    clocked_loop(be, lay, hctx[0], hclk, p.cfl, t_end, max_cycles, max_cycles - cycle0, false);
     pull_ctx();
     for (int b = 0; b < nbox; ++b) {
       if (t_out) t_out[b] = hclk[size_t(b)].t;
@@ -1017,8 +1118,16 @@ inline int create_batch(int nbox, const pmhd_grid* gs, const pmhd_params* p, pmh
   return PMHD_OK;
 }
 
+// grids of at most this many cells run the evolve loop with the device clock by
+// default: their cycles are short enough that a host round trip per cycle shows;
+// larger grids run faster in the single-grid kernels than as a one-box batch
+constexpr int64_t kClockDeviceCells = int64_t(1) << 16;
+
 inline int advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0, int64_t max_cycles,
                    double* t_out, int64_t* cycle_out, double* t2_out) {
+  const int64_t cells = int64_t(h->g.nx) * (h->g.ndim >= 2 ? h->g.ny : 1) * (h->g.ndim == 3 ? h->g.nz : 1);
+  const bool dev = h->clock == PMHD_CLOCK_DEVICE || (h->clock == PMHD_CLOCK_AUTO && cells <= kClockDeviceCells);
+  if (dev) return h->advance_clocked(t0, t_end, cycle0, max_cycles, t_out, cycle_out, t2_out);
   double t = t0, t2 = t0;
   int64_t cyc = cycle0;
   bool finite = true;
@@ -1115,6 +1224,13 @@ PMHD_NS_END
     const int32_t seg[3] = {seg_x, seg_y, seg_z};                                           \
     try { return h->set_sweep(seg); }                                                       \
     catch (...) { return h->fail(PMHD_EINVAL, "exception in pmhd_set_sweep"); }             \
+  }                                                                                         \
+  extern "C" int pmhd_set_clock(pmhd_handle* h, int32_t mode) {                             \
+    if (!h) return PMHD_EINVAL;                                                             \
+    if (mode != PMHD_CLOCK_AUTO && mode != PMHD_CLOCK_HOST && mode != PMHD_CLOCK_DEVICE)    \
+      return h->fail(PMHD_EINVAL, "clock mode must be PMHD_CLOCK_AUTO, _HOST or _DEVICE");  \
+    h->clock = mode;                                                                        \
+    return PMHD_OK;                                                                         \
   }                                                                                         \
   extern "C" int pmhd_advance(pmhd_handle* h, double t0, double t_end, int64_t cycle0,      \
                               int64_t max_cycles, double* t_out, int64_t* cycle_out,        \

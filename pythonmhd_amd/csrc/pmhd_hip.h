@@ -722,15 +722,15 @@ struct HipBatchExec {
   template <class Ph, class T>
   void run(const Ctx<T>&) {
     if constexpr (Ph::kind == EXT_LINE) {
-      // one workgroup of 64 threads per line (lines of uneven boxes waste at most
-      // the tail of their last chunk); the layout's tile is one line
+      // one workgroup per line (64 threads in a batch: lines of uneven boxes waste
+      // at most the tail of their last chunk); the layout's tile is one line
       const Ctx<T>* dctx = static_cast<const Ctx<T>*>(lay->dctx);
       const int* prefix = lay->dprefix + Ph::id * (lay->nbox + 1);
       const int total = lay->total[Ph::id];
       if (total <= 0) return;
       ex->timed(Ph::id, lay->elems[Ph::id], [&] {
-        hipLaunchKernelGGL((k_batch_line<Ph::dir, Ph::solver, T>), dim3(unsigned(total)), dim3(64u), 0, ex->s, dctx, prefix,
-                           lay->nbox);
+        hipLaunchKernelGGL((k_batch_line<Ph::dir, Ph::solver, T>), dim3(unsigned(total)),
+                           dim3(unsigned(lay->line_block)), 0, ex->s, dctx, prefix, lay->nbox);
       });
       ex->note(hipGetLastError());
       return;
@@ -962,6 +962,7 @@ struct HipBackend {
     return h != 0;
   }
   void sync() { ex.note(hipStreamSynchronize(ex.s)); }
+
   void set_profiling(bool on) {
     if (on) ex.reset_profile();
     else ex.collect();

@@ -377,8 +377,9 @@ PMHD_FI TileRef locate_tile(const Ctx<T>* ctxs, const int* prefix, int nbox, int
 // clock of one batch box; the second-last state is the other buffer once done > 0
 struct BoxClock {
   double t, t2, dt;
-  int64_t cyc;   // cycle number
-  int64_t done;  // cycles completed since the box's state was set
+  int64_t cyc;        // cycle number
+  int64_t done;       // cycles completed since the box's state was set
+  int64_t nonfinite;  // cycles whose CFL timestep was not finite (before clipping)
 };
 
 // start of a cycle of one batch box, from its CFL maxima m: decide whether the box
@@ -393,6 +394,9 @@ PMHD_FI void box_begin(Ctx<T>& c, BoxClock& k, const double m[3], double cfl, do
   }
   double dt = dt_of(c, cfl, m);
   if (commit) {
+    // a non-finite state gives a NaN timestep; the reference still runs that cycle
+    // (its clock becomes NaN and its loop ends)
+    if (!(dt - dt == 0.0)) k.nonfinite += 1;
     dt = clip_dt(c.ndim, c.physics, k.t, t_end, dt);
     k.t2 = k.t;
     k.dt = dt;

@@ -927,3 +927,96 @@ def test_fix_quirks_against_oracle(lib, nd, bcs, order, fix):
     plain = oracle_run(shape, widths, bcs, prim, cons, faces, 10.0, 2,
                        M.Params(GAMMA, cfl, order=order, entropy_fix=True))
     assert not np.array_equal(plain[1], ref[1])
+
+
+# the evolve loop with its clock on the host (dt read back every cycle) and on the
+# device (the grid as a one-box batch): the same results, bit for bit in FP64
+def _clock_run(lib, shape, prim, cons, faces, precision, clock, t_end, ncyc, **kw):
+    s = Solver()(shape, widths_of(shape), kw.pop('bcs', (1,) * len(shape)), kw.pop('gamma', GAMMA),
+                 kw.pop('cfl', 0.4 if len(shape) == 3 else 0.8), precision=precision, lib=lib, **kw)
+    s.set_clock(clock)
+    s.set_state(prim, cons, faces)
+    out = []
+    # two calls: the second continues from the first (cell E and speeds of the
+    # stored state), then a cycle limit and a time limit that clips the last dt
+    t, cyc, t2 = s.advance(0.0, t_end, 0, ncyc)
+    out.append((t, cyc, t2))
+    dt = s.compute_dt()
+    t, cyc, t2 = s.advance(t, t + 2.5 * dt, cyc, cyc + 5)
+    out.append((t, cyc, t2))
+    p, c, f = s.get_state(0)
+    p2, c2, f2 = s.get_state(1)
+    st = s.stats()
+    s.close()
+    return out, (p, c, f, p2, c2, f2), st
+
+
+@pytest.mark.parametrize('nd,order,bcs,physics', [(1, 2, (0,), 0), (2, 2, (1, 1), 0), (2, 1, (0, 1), 0),
+                                                  (3, 2, (1, 1, 1), 0), (3, 1, (0, 1, 0), 0), (2, 2, (1, 1), 1)])
+def test_device_clock_matches_host_clock(lib, nd, order, bcs, physics):
+    rng = np.random.default_rng(400 + 10 * nd + order)
+    shape = {1: (40,), 2: (14, 12), 3: (6, 8, 7)}[nd]
+    prim, faces = smooth_state(rng, shape)
+    if physics == 1:
+        prim = prim[[0, 1, 2, 3, 7]]
+        faces = ()
+        cons = H.prim_to_cons(prim, GAMMA, 0)
+    else:
+        cons = M.prim_to_cons_all(prim, GAMMA)
+    print('case: clocks nd', nd, 'order', order, 'physics', physics, flush=True)
+    for precision in (64, 32):
+        kw = dict(order=order, bcs=bcs, physics=physics)
+        ref = _clock_run(lib, shape, prim, cons, faces, precision, 'host', 10.0, 4, **kw)
+        got = _clock_run(lib, shape, prim, cons, faces, precision, 'device', 10.0, 4, **kw)
+        if precision == 64:
+            assert got[0] == ref[0]
+            for a, b in zip(got[1], ref[1]):
+                if isinstance(a, list):
+                    for x, y in zip(a, b):
+                        same(x, y)
+                else:
+                    same(a, b)
+        else:
+            # the batch and single-grid kernels may contract multiply-adds differently
+            for (ta, ca, t2a), (tb, cb, t2b) in zip(got[0], ref[0]):
+                assert ca == cb and abs(ta - tb) <= 1e-5 * abs(tb) and abs(t2a - t2b) <= 1e-5 * abs(t2b)
+            for a, b in zip(got[1][:2], ref[1][:2]):
+                assert np.max(np.abs(a - b)) <= 1e-5 * max(1.0, float(np.max(np.abs(b))))
+        assert got[2]['cycles'] == ref[2]['cycles']
+
+
+def test_device_clock_nonfinite_and_resweep(lib):
+    """The device clock reports a NaN timestep as the host loop does, and a change
+    of sweep plan after a device-clocked run rebuilds its launch tables."""
+    from pythonmhd_amd._lib import ENONFINITE, PmhdError
+    prim = _rarefaction_1d()
+    cons = M.prim_to_cons_all(prim, 1.4)
+    n = prim.shape[1]
+    res = {}
+    for clock in ('host', 'device'):
+        s = Solver()((n,), (1.0 / n,), (0,), 1.4, 0.8, order=2, lib=lib)
+        s.set_clock(clock)
+        s.set_state(prim, cons)
+        with pytest.raises(PmhdError) as e:
+            s.advance(0.0, 0.08, 0, 400)
+        assert e.value.code == ENONFINITE
+        s.set_state(prim, cons)
+        res[clock] = s.advance(0.0, 0.08, 0, 400, allow_nonfinite=True)
+        s.close()
+    assert res['host'][1] == res['device'][1]
+    assert res['host'][0] != res['host'][0] and res['device'][0] != res['device'][0]
+    rng = np.random.default_rng(77)
+    shape = (8, 10, 70)
+    prim, faces = smooth_state(rng, shape)
+    cons = M.prim_to_cons_all(prim, GAMMA)
+    out = []
+    for seg in ((4, 4, -1), (2, 8, 2)):
+        s = Solver()(shape, widths_of(shape), (1, 1, 1), GAMMA, 0.4, precision=32, order=1, lib=lib)
+        s.set_clock('device')
+        s.set_state(prim, cons, faces)
+        s.advance(0.0, 10.0, 0, 2)
+        s.set_sweep(seg)
+        s.advance(0.0, 10.0, 2, 4)
+        out.append(s.get_state(0)[1])
+        s.close()
+    assert np.max(np.abs(out[0] - out[1])) <= 1e-5 * float(np.max(np.abs(out[1])))

@@ -6,7 +6,9 @@ import pytest
 
 from tests.parity_suite import *  # noqa: F401,F403
 
-HOST_LIB = os.path.join(os.path.dirname(__file__), 'hostbuild', 'libpmhd_host.so')
+# PMHD_HOST_LIB: another host build of the same sources (the sanitizer build of
+# tests/hostbuild/asan.py)
+HOST_LIB = os.environ.get('PMHD_HOST_LIB', os.path.join(os.path.dirname(__file__), 'hostbuild', 'libpmhd_host.so'))
 
 
 @pytest.fixture(scope='module')
