@@ -1,11 +1,12 @@
 """Build libpmhd.so (HIP, gfx950) in-tree, and the host test build of the same
 kernel bodies used by tests/ (tests/hostbuild/libpmhd_host.so).
 
-The library has four translation units, compiled in parallel and linked into one
+The library's translation units are compiled in parallel and linked into one
 shared object: the FP64 step (IEEE arithmetic in the reference's operation order,
 no contraction), the FP32-bulk step (contracted multiply-adds, hardware FP32
-division and square root; its double-precision islands stay IEEE), the adjoint
-(IEEE) and the C ABI (host code).
+division and square root; its double-precision islands stay IEEE), the same FP32
+step with non-temporal result stores for large grids, the adjoint (IEEE), the
+allocation-free CFL entry point and the C ABI (host code).
 
     python -m pythonmhd_amd.build            # both
     python -m pythonmhd_amd.build --gpu-only [--only f32,abi]
@@ -36,8 +37,8 @@ IEEE = ['-ffp-contract=off']
 FAST32 = ['-ffp-contract=fast', '-fno-hip-fp32-correctly-rounded-divide-sqrt', '-fno-signed-zeros',
           '-fgpu-flush-denormals-to-zero']
 
-GPU_UNITS = [('pmhd_hip_f64.hip', IEEE), ('pmhd_hip_f32.hip', FAST32), ('pmhd_hip_adj.hip', IEEE),
-             ('pmhd_noalloc.hip', IEEE)]
+GPU_UNITS = [('pmhd_hip_f64.hip', IEEE), ('pmhd_hip_f32.hip', FAST32), ('pmhd_hip_f32nt.hip', FAST32),
+             ('pmhd_hip_adj.hip', IEEE), ('pmhd_noalloc.hip', IEEE)]
 HOST_UNITS = [(os.path.join(HOST_DIR, 'pmhd_host_f64.cpp'), []), (os.path.join(HOST_DIR, 'pmhd_host_f32.cpp'), []),
               (os.path.join(HOST_DIR, 'pmhd_host_adj.cpp'), []), (os.path.join(HOST_DIR, 'pmhd_host_noalloc.cpp'), []),
               (os.path.join(CSRC, 'pmhd_abi.cpp'), [])]

@@ -57,6 +57,9 @@ void pmhd_note_null_error(const char* literal);
 // init().  nullptr when out of host memory.
 pmhd_handle* pmhd_new_handle_f64();
 pmhd_handle* pmhd_new_handle_f32();
+// the FP32 step built with non-temporal result stores, for grids whose launch
+// outputs are far larger than the caches (pmhd_create picks it by size)
+pmhd_handle* pmhd_new_handle_f32nt();
 struct pmhd_batch;
 pmhd_batch* pmhd_new_batch_f64();
 pmhd_batch* pmhd_new_batch_f32();
@@ -1065,6 +1068,17 @@ inline pmhd_grid normal_grid(const pmhd_grid& g) {
   return gg;
 }
 
+// FP32 grids whose interface fluxes along x alone (7 or 5 words per face) exceed
+// the 256 MiB last-level cache: every launch's results stream through HBM anyway,
+// so they are stored past the caches (the non-temporal unit), which leaves the L2
+// to the inputs' neighbour re-reads; smaller grids keep ordinary stores, whose
+// values the next launch may still find cached
+inline bool streaming_grid(const pmhd_grid& g, const pmhd_params& p) {
+  const int64_t ny = g.ndim >= 2 ? g.ny : 1, nz = g.ndim == 3 ? g.nz : 1;
+  const int64_t faces = ny * (int64_t(g.nx) + 1) * nz;
+  return int64_t(face_vars(p.physics)) * faces * int64_t(sizeof(float)) > (int64_t(256) << 20);
+}
+
 inline int create_handle(const pmhd_grid* g, const pmhd_params* p, pmhd_handle** out, std::string& err) {
   if (!g || !p || !out) {
     err = "null argument";
@@ -1078,7 +1092,9 @@ inline int create_handle(const pmhd_grid* g, const pmhd_params* p, pmhd_handle**
   rc = check_params(p, err);
   if (rc) return rc;
   const pmhd_grid gg = normal_grid(*g);
-  pmhd_handle* h = p->precision == 64 ? pmhd_new_handle_f64() : pmhd_new_handle_f32();
+  pmhd_handle* h = p->precision == 64 ? pmhd_new_handle_f64()
+                   : streaming_grid(*g, *p)   ? pmhd_new_handle_f32nt()
+                                             : pmhd_new_handle_f32();
   if (!h) { err = "out of host memory"; return PMHD_ENOMEM; }
   rc = h->init(gg, *p);
   if (rc != PMHD_OK) {

@@ -313,6 +313,25 @@ PMHD_FI void st(const P& base, int64_t i, const V& v) {
   base[i] = v;
 }
 
+// A kernel's result: in a unit that defines PMHD_NT_STORES (built for grids whose
+// launch outputs are far larger than the caches) a non-temporal store, whose value
+// streams past the caches and leaves them to the neighbour re-reads of the launch's
+// inputs; else an ordinary store (the next launch may find the value cached).
+template <class C, class T, class V>
+PMHD_FI void put(const C& c, T* base, int64_t i, const V& v) {
+  (void)c;
+#if defined(__HIP_DEVICE_COMPILE__) && defined(PMHD_NT_STORES)
+  __builtin_nontemporal_store(T(v), reinterpret_cast<T*>(reinterpret_cast<char*>(base) +
+                                                        uint32_t(uint64_t(i) * sizeof(T))));
+#else
+  st(base, i, v);
+#endif
+}
+template <class C, class P, class V, class = typename std::enable_if<std::is_class<P>::value>::type>
+PMHD_FI void put(const C&, const P& base, int64_t i, const V& v) {
+  base[i] = v;
+}
+
 template <class P, class T>
 PMHD_FI void load_rot8(const P& A, int64_t N, int64_t c, int d, T w[7]) {
 #pragma unroll
@@ -333,10 +352,10 @@ PMHD_FI void load7(const P& A, int64_t NF, int64_t f, T v[7]) {
 #pragma unroll
   for (int r = 0; r < 7; ++r) v[r] = ld(A + r * NF, f);
 }
-template <class P, class T>
-PMHD_FI void store7(const P& A, int64_t NF, int64_t f, const T v[7]) {
+template <class C, class P, class T>
+PMHD_FI void store7(const C& c, const P& A, int64_t NF, int64_t f, const T v[7]) {
 #pragma unroll
-  for (int r = 0; r < 7; ++r) st(A + r * NF, f, v[r]);
+  for (int r = 0; r < 7; ++r) put(c, A + r * NF, f, v[r]);
 }
 
 // ===========================================================================
@@ -387,7 +406,7 @@ PMHD_FI void k_slopes(const Ctx<T>& c, int d, int bc, I3 q) {
   eigen_prim(wc, bn, c.gamma, E);
   T ps[7];
   limited_slope(E, wl, wc, wr, ps);
-  store7(c.S, c.nslope(d), idx, ps);
+  store7(c, c.S, c.nslope(d), idx, ps);
 }
 
 // traced edge states of cell p (coordinate e along d, 0 <= e < n) in directional
@@ -457,22 +476,22 @@ PMHD_FI void k_edges(const Ctx<T>& c, int d, int bc, I3 p) {
   const int64_t NF = c.nface(d);
   I3 f0 = with(p, d, e);
   I3 f1 = with(p, d, e + 1);
-  store7(c.WR, NF, c.fid(d, f0.j, f0.i, f0.k), dla);
-  store7(c.WL, NF, c.fid(d, f1.j, f1.i, f1.k), dra);
+  store7(c, c.WR, NF, c.fid(d, f0.j, f0.i, f0.k), dla);
+  store7(c, c.WL, NF, c.fid(d, f1.j, f1.i, f1.k), dra);
   if (e == 0) {
     if (bc == OUTFLOW) {
-      store7(c.WL, NF, c.fid(d, f0.j, f0.i, f0.k), dla);
+      store7(c, c.WL, NF, c.fid(d, f0.j, f0.i, f0.k), dla);
     } else {
       I3 fn = with(p, d, n);
-      store7(c.WR, NF, c.fid(d, fn.j, fn.i, fn.k), dla);
+      store7(c, c.WR, NF, c.fid(d, fn.j, fn.i, fn.k), dla);
     }
   }
   if (e == n - 1) {
     if (bc == OUTFLOW) {
-      store7(c.WR, NF, c.fid(d, f1.j, f1.i, f1.k), dra);
+      store7(c, c.WR, NF, c.fid(d, f1.j, f1.i, f1.k), dra);
     } else {
       I3 fz = with(p, d, 0);
-      store7(c.WL, NF, c.fid(d, fz.j, fz.i, fz.k), dra);
+      store7(c, c.WL, NF, c.fid(d, fz.j, fz.i, fz.k), dra);
     }
   }
 }
@@ -492,8 +511,8 @@ PMHD_FI void k_copy_states(const Ctx<T>& c, int d, int bc, I3 f) {
   from_rot(wl, d, dl);
   from_rot(wr, d, dr);
   const int64_t NF = c.nface(d);
-  store7(c.WL, NF, idx, dl);
-  store7(c.WR, NF, idx, dr);
+  store7(c, c.WL, NF, idx, dl);
+  store7(c, c.WR, NF, idx, dr);
 }
 
 // ===========================================================================
@@ -587,8 +606,8 @@ PMHD_FI void predict_face_src(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7], co
   T ul[7], ur[7];
   prim_to_cons(wl, bn, c.gamma, ke, ul);
   prim_to_cons(wr, bn, c.gamma, ke, ur);
-  store7(c.UL[d], NF, idx, ul);
-  store7(c.UR[d], NF, idx, ur);
+  store7(c, c.UL[d], NF, idx, ul);
+  store7(c, c.UR[d], NF, idx, ur);
   T rwl[7], rwr[7], rul[7], rur[7], rf[7], df[7];
   to_rot(wl, d, rwl);
   to_rot(wr, d, rwr);
@@ -596,7 +615,7 @@ PMHD_FI void predict_face_src(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7], co
   to_rot(ur, d, rur);
   roe_flux<S>(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf);
   from_rot(rf, d, df);
-  store7(c.F[d], NF, idx, df);
+  store7(c, c.F[d], NF, idx, df);
 }
 
 template <int S, class T>
@@ -717,7 +736,7 @@ PMHD_FI void store_line_end(const Ctx<T>& c, int d, I3 q, int side, const T s[7]
   const int64_t L = line_count(c, d);
   const int64_t l = line_of(c, d, q);
 #pragma unroll
-  for (int v = 0; v < 7; ++v) st(c.zend + (7 * side + v) * L, l, s[v]);
+  for (int v = 0; v < 7; ++v) put(c, c.zend + (7 * side + v) * L, l, s[v]);
 }
 template <class T>
 PMHD_FI void k_line_ends(const Ctx<T>& c, int d, int bc, I3 q) {
@@ -769,11 +788,11 @@ template <class T>
 PMHD_FI void store_cell_e(const Ctx<T>& c, int64_t idx, T rho, T mx, T my, T mz, T vx, T vy, T bx, T by, T bz) {
   const int64_t N = c.ncell();
   if (c.ndim == 3) {
-    st(c.Ec, idx, (by * mz - bz * my) / rho);
-    st(c.Ec + N, idx, (bz * mx - bx * mz) / rho);
-    st(c.Ec + 2 * N, idx, (bx * my - by * mx) / rho);
+    put(c, c.Ec, idx, (by * mz - bz * my) / rho);
+    put(c, c.Ec + N, idx, (bz * mx - bx * mz) / rho);
+    put(c, c.Ec + 2 * N, idx, (bx * my - by * mx) / rho);
   } else {
-    st(c.Ec, idx, vy * bx - vx * by);
+    put(c, c.Ec, idx, vy * bx - vx * by);
   }
 }
 
@@ -812,9 +831,9 @@ PMHD_FI void k_half_e(const Ctx<T>& c, I3 p) {
     T bx = T(0.5) * (T(ld(c.nfb[0], fx0)) + T(ld(c.nfb[0], fx1)));
     T by = T(0.5) * (T(ld(c.nfb[1], fy0)) + T(ld(c.nfb[1], fy1)));
     T bz = T(0.5) * (T(ld(c.nfb[2], fz0)) + T(ld(c.nfb[2], fz1)));
-    st(c.Ec, idx, (by * h[3] - bz * h[2]) / h[0]);
-    st(c.Ec + N, idx, (bz * h[1] - bx * h[3]) / h[0]);
-    st(c.Ec + 2 * N, idx, (bx * h[2] - by * h[1]) / h[0]);
+    put(c, c.Ec, idx, (by * h[3] - bz * h[2]) / h[0]);
+    put(c, c.Ec + N, idx, (bz * h[1] - bx * h[3]) / h[0]);
+    put(c, c.Ec + 2 * N, idx, (bx * h[2] - by * h[1]) / h[0]);
   } else {
     T h[3];
 #pragma unroll
@@ -826,7 +845,7 @@ PMHD_FI void k_half_e(const Ctx<T>& c, I3 p) {
     T by = (T(ld(c.nfb[1], fy0)) + T(ld(c.nfb[1], fy1))) / T(2);
     T vx = h[1] / h[0];
     T vy = h[2] / h[0];
-    st(c.Ec, idx, vy * bx - vx * by);
+    put(c, c.Ec, idx, vy * bx - vx * by);
   }
 }
 
@@ -857,7 +876,7 @@ PMHD_FI void k_emf(const Ctx<T>& c, const Ptr<T>* Fs, I3 p) {
     T x2 = upwind(t0, t5 - ul4 - uE, t5 - lf4 - cE, t5 - T(0.5) * (lf4 + ul4 + cE + uE));
     T y1 = upwind(ul0, -ul4 + lt5 - ulE, -ul4 + t5 - uE, -ul4 + T(0.5) * (lt5 + t5 - ulE - uE));
     T y2 = upwind(lf0, -lf4 + lt5 - lE, -lf4 + t5 - cE, -lf4 + T(0.5) * (lt5 + t5 - lE - cE));
-    st(c.E[2], idx, T(0.25) * (x1 + x2 + y1 + y2));
+    put(c, c.E[2], idx, T(0.25) * (x1 + x2 + y1 + y2));
     return;
   }
   const int64_t zl = edge_lo(p.k, c.nz, bz), zr = edge_hi(p.k, c.nz, bz);
@@ -875,7 +894,7 @@ PMHD_FI void k_emf(const Ctx<T>& c, const Ptr<T>* Fs, I3 p) {
     T z2 = upwind(t0, ub5 - uE, b5 - cE, T(0.5) * (ub5 + b5 - uE - cE));
     T y1 = upwind(ub0, -bt4 - ubE, -t4 - uE, T(0.5) * (-bt4 - t4 - ubE - uE));
     T y2 = upwind(b0, -bt4 - bE, -t4 - cE, T(0.5) * (-bt4 - t4 - bE - cE));
-    st(c.E[0], idx, T(0.25) * (b5 + ub5 - t4 - bt4 + y1 + y2 + z1 + z2));
+    put(c, c.E[0], idx, T(0.25) * (b5 + ub5 - t4 - bt4 + y1 + y2 + z1 + z2));
   }
   {  // E_y (getYCornerEMF_3D): faces x (F) and z (H)
     T b0 = fluxv(c, Fs, 2, 0, yr, xr, p.k), b4 = fluxv(c, Fs, 2, 4, yr, xr, p.k);
@@ -888,7 +907,7 @@ PMHD_FI void k_emf(const Ctx<T>& c, const Ptr<T>* Fs, I3 p) {
     T z2 = upwind(lf0, -lb4 - lE, -b4 - cE, T(0.5) * (-lb4 - b4 - lE - cE));
     T x1 = upwind(lb0, blf5 - blE, lf5 - lE, T(0.5) * (blf5 + lf5 - blE - lE));
     T x2 = upwind(b0, blf5 - bE, lf5 - cE, T(0.5) * (blf5 + lf5 - bE - cE));
-    st(c.E[1], idx, T(0.25) * (lf5 + blf5 - b4 - lb4 + x1 + x2 + z1 + z2));
+    put(c, c.E[1], idx, T(0.25) * (lf5 + blf5 - b4 - lb4 + x1 + x2 + z1 + z2));
   }
   {  // E_z (getZCornerEMF_3D): faces x (F) and y (G)
     T t0 = fluxv(c, Fs, 1, 0, p.j, xr, zr), t5 = fluxv(c, Fs, 1, 5, p.j, xr, zr);
@@ -901,7 +920,7 @@ PMHD_FI void k_emf(const Ctx<T>& c, const Ptr<T>* Fs, I3 p) {
     T x2 = upwind(t0, -ul4 - uE, -lf4 - cE, T(-0.5) * (lf4 + ul4 + cE + uE));
     T y1 = upwind(ul0, lt5 - ulE, t5 - uE, T(0.5) * (lt5 + t5 - ulE - uE));
     T y2 = upwind(lf0, lt5 - lE, t5 - cE, T(0.5) * (lt5 + t5 - lE - cE));
-    st(c.E[2], idx, T(0.25) * (lt5 + t5 - ul4 - lf4 + x1 + x2 + y1 + y2));
+    put(c, c.E[2], idx, T(0.25) * (lt5 + t5 - ul4 - lf4 + x1 + x2 + y1 + y2));
   }
 }
 
@@ -1125,6 +1144,7 @@ PMHD_FI void k_ct(const Ctx<T>& c, int d, const PI& fin, const PI& lin, const PO
     }
     res = old + (a - b);
   }
+  // (an ordinary store: the next launch reads the faces at once)
   if constexpr (split_faces<T>()) {
     const T hi = T(res);
     st(fout, idx, hi);
@@ -1157,7 +1177,7 @@ PMHD_FI void k_correct(const Ctx<T>& c, int d, I3 f) {
   to_rot(ur, d, rur);
   roe_flux<S>(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf);
   from_rot(rf, d, df);
-  store7(c.Fc[d], NF, idx, df);
+  store7(c, c.Fc[d], NF, idx, df);
 }
 
 // ===========================================================================
@@ -1212,22 +1232,22 @@ PMHD_FI void k_update(const Ctx<T>& c, I3 p) {
     u[5] = bz;
     cons_to_prim(u, bx, c.gamma, 0, w);
   }
-  st(c.U2, idx, u[0]);
-  st(c.U2 + N, idx, u[1]);
-  st(c.U2 + 2 * N, idx, u[2]);
-  st(c.U2 + 3 * N, idx, u[3]);
-  st(c.U2 + 4 * N, idx, bx);
-  st(c.U2 + 5 * N, idx, u[4]);
-  st(c.U2 + 6 * N, idx, u[5]);
-  st(c.U2 + 7 * N, idx, u[6]);
-  st(c.W2, idx, w[0]);
-  st(c.W2 + N, idx, w[1]);
-  st(c.W2 + 2 * N, idx, w[2]);
-  st(c.W2 + 3 * N, idx, w[3]);
-  st(c.W2 + 4 * N, idx, bx);
-  st(c.W2 + 5 * N, idx, w[4]);
-  st(c.W2 + 6 * N, idx, w[5]);
-  st(c.W2 + 7 * N, idx, w[6]);
+  put(c, c.U2, idx, u[0]);
+  put(c, c.U2 + N, idx, u[1]);
+  put(c, c.U2 + 2 * N, idx, u[2]);
+  put(c, c.U2 + 3 * N, idx, u[3]);
+  put(c, c.U2 + 4 * N, idx, bx);
+  put(c, c.U2 + 5 * N, idx, u[4]);
+  put(c, c.U2 + 6 * N, idx, u[5]);
+  put(c, c.U2 + 7 * N, idx, u[6]);
+  put(c, c.W2, idx, w[0]);
+  put(c, c.W2 + N, idx, w[1]);
+  put(c, c.W2 + 2 * N, idx, w[2]);
+  put(c, c.W2 + 3 * N, idx, w[3]);
+  put(c, c.W2 + 4 * N, idx, bx);
+  put(c, c.W2 + 5 * N, idx, w[4]);
+  put(c, c.W2 + 6 * N, idx, w[5]);
+  put(c, c.W2 + 7 * N, idx, w[6]);
   // the next cycle's cell E from the stored values (what CellE would read back)
   if (c.ec_next && c.ndim >= 2) store_cell_e<T>(c, idx, u[0], u[1], u[2], u[3], w[1], w[2], bx, w[4], w[5]);
 }

@@ -5,3 +5,7 @@
 #include "pmhd_host.h"
 
 PMHD_DEFINE_FACTORIES(f32, float, pmhd::HostBackend)
+
+// the HIP library's non-temporal FP32 unit differs only in its store instructions:
+// on the host it is the same handle
+pmhd_handle* pmhd_new_handle_f32nt() { return pmhd_new_handle_f32(); }
