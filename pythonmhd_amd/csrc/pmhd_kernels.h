@@ -332,6 +332,26 @@ PMHD_FI void put(const C&, const P& base, int64_t i, const V& v) {
   base[i] = v;
 }
 
+// An input only this element reads (no neighbour re-reads it): in the
+// non-temporal unit it streams past the caches like the results
+template <class T>
+PMHD_FI T ld_once(const T* base, int64_t i) {
+#if defined(__HIP_DEVICE_COMPILE__) && defined(PMHD_NT_STORES)
+  return __builtin_nontemporal_load(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                                               uint32_t(uint64_t(i) * sizeof(T))));
+#else
+  return ld(base, i);
+#endif
+}
+template <class T>
+PMHD_FI T ld_once(T* base, int64_t i) {
+  return ld_once(static_cast<const T*>(base), i);
+}
+template <class P, class = typename std::enable_if<std::is_class<P>::value>::type>
+PMHD_FI auto ld_once(const P& base, int64_t i) -> decltype(base[i]) {
+  return base[i];
+}
+
 template <class P, class T>
 PMHD_FI void load_rot8(const P& A, int64_t N, int64_t c, int d, T w[7]) {
 #pragma unroll
@@ -351,6 +371,11 @@ template <class P, class T>
 PMHD_FI void load7(const P& A, int64_t NF, int64_t f, T v[7]) {
 #pragma unroll
   for (int r = 0; r < 7; ++r) v[r] = ld(A + r * NF, f);
+}
+template <class P, class T>
+PMHD_FI void load7_once(const P& A, int64_t NF, int64_t f, T v[7]) {
+#pragma unroll
+  for (int r = 0; r < 7; ++r) v[r] = ld_once(A + r * NF, f);
 }
 template <class C, class P, class T>
 PMHD_FI void store7(const C& c, const P& A, int64_t NF, int64_t f, const T v[7]) {
@@ -824,7 +849,7 @@ PMHD_FI void k_half_e(const Ctx<T>& c, I3 p) {
     T h[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      h[v] = ld(c.U + v * N, idx) - c.hdtd[0] * (ld(c.F[0] + v * NX, fx1) - ld(c.F[0] + v * NX, fx0))
+      h[v] = ld_once(c.U + v * N, idx) - c.hdtd[0] * (ld(c.F[0] + v * NX, fx1) - ld(c.F[0] + v * NX, fx0))
              - c.hdtd[1] * (ld(c.F[1] + v * NY, fy1) - ld(c.F[1] + v * NY, fy0))
              - c.hdtd[2] * (ld(c.F[2] + v * NZ, fz1) - ld(c.F[2] + v * NZ, fz0));
     }
@@ -1016,8 +1041,8 @@ PMHD_FI void ctu_states(const Ctx<T>& c, int d, I3 f, T ul[7], T ur[7]) {
   const int64_t n = c.n_along(d);
   const int bc = c.bc[d];
   const int64_t e = co(f, d);
-  load7(c.UL[d], NF, idx, ul);
-  load7(c.UR[d], NF, idx, ur);
+  load7_once(c.UL[d], NF, idx, ul);
+  load7_once(c.UR[d], NF, idx, ur);
   I3 pl = with(f, d, map1(e - 1, n, bc));
   I3 pr = with(f, d, map1(e, n, bc));
   if (c.ndim == 3) {
@@ -1191,7 +1216,7 @@ PMHD_FI void k_update(const Ctx<T>& c, I3 p) {
   const int dv[7] = {0, 1, 2, 3, 5, 6, 7};
   T u[7];
 #pragma unroll
-  for (int v = 0; v < 7; ++v) u[v] = ld(c.U + dv[v] * N, idx);
+  for (int v = 0; v < 7; ++v) u[v] = ld_once(c.U + dv[v] * N, idx);
   const int64_t NX = c.nface(0);
   int64_t fx0 = c.fid(0, p.j, p.i, p.k), fx1 = c.fid(0, p.j, p.i + 1, p.k);
   T bx, w[7];
