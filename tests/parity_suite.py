@@ -518,6 +518,52 @@ def test_adjoint_against_oracle_differences(lib, nd, order, bcs):
         assert abs(ad - fd) <= 2e-6 * abs(fd) + 1e-9, (arr, v, idx, ad, fd)
 
 
+def test_adjoint_configured_state(lib):
+    """The adjoint configuration's initial state (problems.adjoint_state: rho = 1 +
+    0.05 GRF, p = 1, B = (0.3, 0, 0)) with a smooth velocity field, at 32^2: the
+    gradient of the kinetic energy after 6 fixed-dt steps with respect to the initial
+    density (cons built from prim, v, p and B held) against central differences of
+    the FP64 oracle on 8 cells, and the FP32-forward gradient against the FP64 one."""
+    from pythonmhd_amd.adjoint import AdjointRun
+    from pythonmhd_amd.problems import adjoint_state
+    n, steps = 32, 6
+    prim, cons, faces = adjoint_state(n, velocity=0.2)
+    widths = (1.0 / n, 1.0 / n)
+    P = M.Params(GAMMA, 0.8, order=1, entropy_fix=False)
+    dt = 0.5 * M.calc_dt(cons, faces, GAMMA, 0.8, widths)
+    print('case: adjoint configured state', n, steps, flush=True)
+    grads = {}
+    for prec in (64, 32):
+        run = AdjointRun((n, n), widths, (1, 1), GAMMA, order=1, precision=prec, checkpoint_every=3, lib=lib)
+        _, co, _ = run.forward(prim, cons, faces, dt, steps)
+        gp, gc, _ = run.backward(None, _kinetic_cotangent(co), None)
+        run.close()
+        v = prim[1:4]
+        grads[prec] = gp[0] + gc[0] + v[0] * gc[1] + v[1] * gc[2] + v[2] * gc[3] + 0.5 * (v * v).sum(0) * gc[7]
+
+    def loss(rho):
+        p2 = prim.copy()
+        p2[0] = rho
+        c2 = M.prim_to_cons_all(p2, GAMMA)
+        return _kinetic_loss(_oracle_steps(2, p2, c2, faces, dt, steps, widths, (1, 1), P)[1])
+
+    rng = np.random.default_rng(5)
+    g64 = grads[64]
+    scale = float(np.max(np.abs(g64)))
+    eps = 1e-4
+    errs = []
+    for _ in range(8):
+        j, i = int(rng.integers(0, n)), int(rng.integers(0, n))
+        up, dn = prim[0].copy(), prim[0].copy()
+        up[j, i] += eps
+        dn[j, i] -= eps
+        fd = (loss(up) - loss(dn)) / (2 * eps)
+        errs.append(abs(g64[j, i] - fd))
+        assert abs(g64[j, i] - fd) <= 1e-3 * scale + 1e-4 * abs(fd), (j, i, g64[j, i], fd)
+    assert np.median(errs) <= 1e-5 * scale, errs
+    assert np.max(np.abs(grads[32] - g64)) <= 1e-3 * scale
+
+
 def test_adjoint_checkpoint_spacing_and_fp32(lib):
     """The gradient does not depend on the checkpoint spacing (to FP64 rounding of
     the cotangent sums), and the FP32 adjoint is within 1e-3 of the FP64 one."""

@@ -142,3 +142,26 @@ def test_reduction_2d_to_1d():
     for j in range(6):
         assert np.allclose(b[:, j, :][[0, 1, 3, 4]], a[[0, 1, 3, 4]], rtol=0, atol=1e-14)
     del rng
+
+
+@pytest.mark.parametrize('k', range(5))
+def test_plm_tracing_single_wave_closed_form(k):
+    """Hydro PLM with dt > 0 on linear data along one eigenvector R_k of a uniform
+    state: the traced edge states follow the MUSCL-Hancock closed form of the
+    characteristic tracing (wave k moving toward a face is corrected with its own
+    speed, a wave moving away only with the edge wave's reference state), for the
+    acoustic, entropy and shear waves."""
+    w0 = np.array([1.1, 0.25, -0.3, 0.15, 0.9])
+    lam, L, R = H.eigen(w0[:, None], G, 0)
+    lam, r = lam[:, 0], R[:, k, 0]
+    assert lam[0] < 0 < lam[4]
+    n, eps, dt, dx = 16, 1e-7, 0.03, 0.1
+    nu = dt / dx
+    w = w0[:, None] + eps * np.arange(n)[None, :] * r[:, None]
+    wl, wr = H.recon_states(w, G, dt, dx, 0, 0, 0, 1)
+    a_right = 0.5 - 0.5 * nu * (lam[k] if lam[k] >= 0 else lam[4])
+    a_left = -0.5 - 0.5 * nu * (lam[k] if lam[k] <= 0 else lam[0])
+    tol = 1e-4 * eps
+    for i in range(3, n - 3):
+        assert np.max(np.abs(wl[:, i + 1] - (w[:, i] + eps * a_right * r))) < tol
+        assert np.max(np.abs(wr[:, i] - (w[:, i] + eps * a_left * r))) < tol

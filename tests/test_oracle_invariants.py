@@ -319,3 +319,37 @@ def test_fallback_equals_roe_on_physical_states():
     changed = ~np.all(fb == roe, axis=0)
     assert changed.sum() < 0.5 * changed.size
     assert np.array_equal(fb[:, changed], hl[:, changed])
+
+
+@pytest.mark.parametrize('k', range(7))
+def test_plm_tracing_single_wave_closed_form(k):
+    """Linear data along one eigenvector R_k of a uniform state: the PLM slope is the
+    data's slope and the traced edge states follow the MUSCL-Hancock closed form of
+    the build's characteristic tracing.  Right edge of a cell (left state of the face
+    above it): w + (1/2)(1 - nu lambda_k) dw_k when wave k moves toward that face
+    (lambda_k >= 0), else w + (1/2)(1 - nu max(lambda_6, 0)) dw_k (the reference
+    state of the fastest right-going wave, with no correction for k); the left edge
+    mirrors it with lambda_0.  nu = dt/dx.  An all-waves or wrong-side selection, a
+    lost half or a sign error fails it."""
+    g = G53
+    w0 = np.array([1.2, 0.3, -0.1, 0.2, 0.4, -0.5, 0.8])
+    bn0 = 0.7
+    lam, L, R = M.eigen_prim(w0[:, None], np.array([bn0]), g)
+    lam, r = lam[:, 0], R[:, k, 0]
+    assert lam[0] < 0 < lam[6]
+    n, eps, dt, dx = 16, 1e-7, 0.02, 0.1
+    nu = dt / dx
+    w = w0[:, None] + eps * np.arange(n)[None, :] * r[:, None]
+    wl, wr = M.recon_states(w, np.full(n, bn0), g, dt, dx, 0, 'x', 0, 1)
+    a_right = 0.5 - 0.5 * nu * (lam[k] if lam[k] >= 0 else lam[6])
+    a_left = -0.5 - 0.5 * nu * (lam[k] if lam[k] <= 0 else lam[0])
+    tol = 1e-4 * eps
+    for i in range(3, n - 3):
+        assert np.max(np.abs(wl[:, i + 1] - (w[:, i] + eps * a_right * r))) < tol
+        assert np.max(np.abs(wr[:, i] - (w[:, i] + eps * a_left * r))) < tol
+    # the alternative selection (every wave corrected with its own speed) differs
+    # here by far more than the tolerance unless lambda_k is the edge wave's speed
+    if lam[k] < 0:
+        assert abs(0.5 * nu * (lam[k] - lam[6])) * eps * np.max(np.abs(r)) > 100 * tol
+    if lam[k] > 0:
+        assert abs(0.5 * nu * (lam[k] - lam[0])) * eps * np.max(np.abs(r)) > 100 * tol
