@@ -55,13 +55,13 @@ PMHD_FI void call_phase(const Ctx<T>& c, I3 p) {
 }
 
 template <class V>
-PMHD_FI AdPtr<V> ad_ptr(const V* val, double* adj, AdState* st) {
-  return AdPtr<V>{val, adj, st, 0};
+PMHD_FI AdPtr<V> ad_ptr(const V* val, double* adj, AdState* st, AdHot* hot) {
+  return AdPtr<V>{val, adj, st, hot, 0};
 }
 
 // the dual context of a phase: values from c, cotangents from a
 template <class V>
-PMHD_FI void make_dual_ctx(const Ctx<V>& c, const Ctx<double>& a, AdState* st, Ctx<ADual<V>>& d) {
+PMHD_FI void make_dual_ctx(const Ctx<V>& c, const Ctx<double>& a, AdState* st, AdHot* hot, Ctx<ADual<V>>& d) {
   using D = ADual<V>;
   d.ndim = c.ndim;
   d.active = c.active;
@@ -93,25 +93,25 @@ PMHD_FI void make_dual_ctx(const Ctx<V>& c, const Ctx<double>& a, AdState* st, C
     d.width[q] = D(double(c.width[q]));
     d.wd[q] = c.wd[q];
     d.emfq[q] = D(double(c.emfq[q]));
-    d.fb[q] = ad_ptr<V>(c.fb[q], const_cast<double*>(a.fb[q]), st);
-    d.fb2[q] = ad_ptr<V>(c.fb2[q], a.fb2[q], st);
-    d.nfb[q] = ad_ptr<V>(c.nfb[q], a.nfb[q], st);
-    d.UL[q] = ad_ptr<V>(c.UL[q], a.UL[q], st);
-    d.UR[q] = ad_ptr<V>(c.UR[q], a.UR[q], st);
-    d.F[q] = ad_ptr<V>(c.F[q], a.F[q], st);
-    d.Fc[q] = ad_ptr<V>(c.Fc[q], a.Fc[q], st);
-    d.E[q] = ad_ptr<V>(c.E[q], a.E[q], st);
+    d.fb[q] = ad_ptr<V>(c.fb[q], const_cast<double*>(a.fb[q]), st, hot);
+    d.fb2[q] = ad_ptr<V>(c.fb2[q], a.fb2[q], st, hot);
+    d.nfb[q] = ad_ptr<V>(c.nfb[q], a.nfb[q], st, hot);
+    d.UL[q] = ad_ptr<V>(c.UL[q], a.UL[q], st, hot);
+    d.UR[q] = ad_ptr<V>(c.UR[q], a.UR[q], st, hot);
+    d.F[q] = ad_ptr<V>(c.F[q], a.F[q], st, hot);
+    d.Fc[q] = ad_ptr<V>(c.Fc[q], a.Fc[q], st, hot);
+    d.E[q] = ad_ptr<V>(c.E[q], a.E[q], st, hot);
   }
   d.ldh[0] = c.ldh[0];
   d.ldh[1] = c.ldh[1];
-  d.U = ad_ptr<V>(c.U, const_cast<double*>(a.U), st);
-  d.W = ad_ptr<V>(c.W, const_cast<double*>(a.W), st);
-  d.U2 = ad_ptr<V>(c.U2, a.U2, st);
-  d.W2 = ad_ptr<V>(c.W2, a.W2, st);
-  d.WL = ad_ptr<V>(c.WL, a.WL, st);
-  d.WR = ad_ptr<V>(c.WR, a.WR, st);
-  d.S = ad_ptr<V>(c.S, a.S, st);
-  d.Ec = ad_ptr<V>(c.Ec, a.Ec, st);
+  d.U = ad_ptr<V>(c.U, const_cast<double*>(a.U), st, hot);
+  d.W = ad_ptr<V>(c.W, const_cast<double*>(a.W), st, hot);
+  d.U2 = ad_ptr<V>(c.U2, a.U2, st, hot);
+  d.W2 = ad_ptr<V>(c.W2, a.W2, st, hot);
+  d.WL = ad_ptr<V>(c.WL, a.WL, st, hot);
+  d.WR = ad_ptr<V>(c.WR, a.WR, st, hot);
+  d.S = ad_ptr<V>(c.S, a.S, st, hot);
+  d.Ec = ad_ptr<V>(c.Ec, a.Ec, st, hot);
   d.fcnt = nullptr;   // the dual evaluations floor without counting
 }
 
@@ -119,23 +119,22 @@ PMHD_FI void make_dual_ctx(const Ctx<V>& c, const Ctx<double>& a, AdState* st, C
 // more input or output slots than the state holds
 template <class Ph, class V, class AtomicAdd>
 PMHD_FI bool adjoint_element(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, AtomicAdd add) {
+  AdHot hot{0, nullptr, -1, 0.0};
   Ctx<ADual<V>> d;
-  make_dual_ctx(c, a, &st, d);
-  st.mode = 0;
+  make_dual_ctx(c, a, &st, &hot, d);
   st.nin = 0;
   st.nout = 0;
   st.overflow = 0;
-  st.seed = 0;
-  st.acc = 0.0;
   call_phase<Ph>(d, p);
   if (st.overflow) return false;
   ad_prune(st);
-  st.mode = 1;
+  hot.mode = 1;
   for (int k = 0; k < st.nin; ++k) {
-    st.seed = k;
-    st.acc = 0.0;
+    hot.seed_base = st.in[k].base;
+    hot.seed_i = st.in[k].i;
+    hot.acc = 0.0;
     call_phase<Ph>(d, p);
-    if (st.acc != 0.0) add(st.in[k].adj + st.in[k].i, st.acc);
+    if (hot.acc != 0.0) add(st.in[k].adj + st.in[k].i, hot.acc);
   }
   for (int k = 0; k < st.nout; ++k) st.out[k].adj[st.out[k].i] = 0.0;
   return true;

@@ -79,14 +79,21 @@ struct AdSlot {
   int64_t i;
 };
 
+// the slot tables of one element (indexed at run time: they live in scratch memory)
 struct AdState {
-  int mode;      // 0: record the accessed slots; 1: seeded pass
   int nin, nout;
   int overflow;
-  int seed;      // index into in[] of the seeded slot
-  double acc;    // sum over stores of cotangent * tangent
   AdSlot in[kAdMaxIn];
   AdSlot out[kAdMaxOut];
+};
+
+// what every load and store of a pass reads or updates: kept apart from the slot
+// tables, as a few scalars the compiler can hold in registers
+struct AdHot {
+  int mode;                // 0: record the accessed slots; 1: seeded pass
+  const void* seed_base;   // the seeded slot (array identity and index)
+  int64_t seed_i;
+  double acc;              // sum over stores of cotangent * tangent
 };
 
 PMHD_FI bool slot_has(const AdSlot* s, int n, const void* base, int64_t i) {
@@ -104,6 +111,7 @@ struct AdPtr {
   const V* val;
   double* adj;
   AdState* st;
+  AdHot* hot;
   int64_t off;
   PMHD_FI AdRef<V> operator[](int64_t i) const { return AdRef<V>(*this, off + i); }
   PMHD_FI AdPtr operator+(int64_t n) const {
@@ -118,27 +126,29 @@ struct AdRef : ADual<V> {
   AdPtr<V> p;
   int64_t i;
   PMHD_FI AdRef(const AdPtr<V>& p_, int64_t i_) : ADual<V>(), p(p_), i(i_) {
-    AdState& s = *p.st;
+    const AdHot& h = *p.hot;
     this->v = p.val[i];
     this->t = V(0);
-    if (s.mode == 0) {
+    if (h.mode == 0) {
+      AdState& s = *p.st;
       if (!slot_has(s.in, s.nin, p.val, i)) {
         if (s.nin < kAdMaxIn) s.in[s.nin++] = AdSlot{p.val, p.adj, i};
         else s.overflow = 1;
       }
-    } else if (s.in[s.seed].base == p.val && s.in[s.seed].i == i) {
+    } else if (h.seed_base == p.val && h.seed_i == i) {
       this->t = V(1);
     }
   }
   PMHD_FI AdRef& operator=(const ADual<V>& x) {
-    AdState& s = *p.st;
-    if (s.mode == 0) {
+    AdHot& h = *p.hot;
+    if (h.mode == 0) {
+      AdState& s = *p.st;
       if (!slot_has(s.out, s.nout, p.val, i)) {
         if (s.nout < kAdMaxOut) s.out[s.nout++] = AdSlot{p.val, p.adj, i};
         else s.overflow = 1;
       }
     } else {
-      s.acc += p.adj[i] * double(x.t);
+      h.acc += p.adj[i] * double(x.t);
     }
     return *this;
   }
