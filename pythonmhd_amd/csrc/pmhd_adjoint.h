@@ -54,15 +54,12 @@ PMHD_FI void call_phase(const Ctx<T>& c, I3 p) {
   }
 }
 
-template <class V>
-PMHD_FI AdPtr<V> ad_ptr(const V* val, double* adj, AdState* st, AdHot* hot) {
-  return AdPtr<V>{val, adj, st, hot, 0};
-}
 
-// the dual context of a phase: values from c, cotangents from a
-template <class V>
-PMHD_FI void make_dual_ctx(const Ctx<V>& c, const Ctx<double>& a, AdState* st, AdHot* hot, Ctx<ADual<V>>& d) {
-  using D = ADual<V>;
+
+// the context of a phase on number type D: values from c, cotangents from a, each
+// array wrapped by mk(values, cotangents)
+template <class D, class V, class Mk>
+PMHD_FI void make_ctx_of(const Ctx<V>& c, const Ctx<double>& a, Mk mk, Ctx<D>& d) {
   d.ndim = c.ndim;
   d.active = c.active;
   d.physics = c.physics;
@@ -93,25 +90,25 @@ PMHD_FI void make_dual_ctx(const Ctx<V>& c, const Ctx<double>& a, AdState* st, A
     d.width[q] = D(double(c.width[q]));
     d.wd[q] = c.wd[q];
     d.emfq[q] = D(double(c.emfq[q]));
-    d.fb[q] = ad_ptr<V>(c.fb[q], const_cast<double*>(a.fb[q]), st, hot);
-    d.fb2[q] = ad_ptr<V>(c.fb2[q], a.fb2[q], st, hot);
-    d.nfb[q] = ad_ptr<V>(c.nfb[q], a.nfb[q], st, hot);
-    d.UL[q] = ad_ptr<V>(c.UL[q], a.UL[q], st, hot);
-    d.UR[q] = ad_ptr<V>(c.UR[q], a.UR[q], st, hot);
-    d.F[q] = ad_ptr<V>(c.F[q], a.F[q], st, hot);
-    d.Fc[q] = ad_ptr<V>(c.Fc[q], a.Fc[q], st, hot);
-    d.E[q] = ad_ptr<V>(c.E[q], a.E[q], st, hot);
+    d.fb[q] = mk(c.fb[q], const_cast<double*>(a.fb[q]));
+    d.fb2[q] = mk(c.fb2[q], a.fb2[q]);
+    d.nfb[q] = mk(c.nfb[q], a.nfb[q]);
+    d.UL[q] = mk(c.UL[q], a.UL[q]);
+    d.UR[q] = mk(c.UR[q], a.UR[q]);
+    d.F[q] = mk(c.F[q], a.F[q]);
+    d.Fc[q] = mk(c.Fc[q], a.Fc[q]);
+    d.E[q] = mk(c.E[q], a.E[q]);
   }
   d.ldh[0] = c.ldh[0];
   d.ldh[1] = c.ldh[1];
-  d.U = ad_ptr<V>(c.U, const_cast<double*>(a.U), st, hot);
-  d.W = ad_ptr<V>(c.W, const_cast<double*>(a.W), st, hot);
-  d.U2 = ad_ptr<V>(c.U2, a.U2, st, hot);
-  d.W2 = ad_ptr<V>(c.W2, a.W2, st, hot);
-  d.WL = ad_ptr<V>(c.WL, a.WL, st, hot);
-  d.WR = ad_ptr<V>(c.WR, a.WR, st, hot);
-  d.S = ad_ptr<V>(c.S, a.S, st, hot);
-  d.Ec = ad_ptr<V>(c.Ec, a.Ec, st, hot);
+  d.U = mk(c.U, const_cast<double*>(a.U));
+  d.W = mk(c.W, const_cast<double*>(a.W));
+  d.U2 = mk(c.U2, a.U2);
+  d.W2 = mk(c.W2, a.W2);
+  d.WL = mk(c.WL, a.WL);
+  d.WR = mk(c.WR, a.WR);
+  d.S = mk(c.S, a.S);
+  d.Ec = mk(c.Ec, a.Ec);
   d.fcnt = nullptr;   // the dual evaluations floor without counting
 }
 
@@ -119,16 +116,20 @@ PMHD_FI void make_dual_ctx(const Ctx<V>& c, const Ctx<double>& a, AdState* st, A
 // more input or output slots than the state holds
 template <class Ph, class V, class AtomicAdd>
 PMHD_FI bool adjoint_element(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, AtomicAdd add) {
-  AdHot hot{0, nullptr, -1, 0.0};
-  Ctx<ADual<V>> d;
-  make_dual_ctx(c, a, &st, &hot, d);
-  st.nin = 0;
-  st.nout = 0;
-  st.overflow = 0;
-  call_phase<Ph>(d, p);
+  {
+    // recording pass on plain values
+    Ctx<ARec<V>> r;
+    make_ctx_of<ARec<V>>(c, a, [&](const V* val, double* adj) { return RecPtr<V>{val, adj, &st, 0}; }, r);
+    st.nin = 0;
+    st.nout = 0;
+    st.overflow = 0;
+    call_phase<Ph>(r, p);
+  }
   if (st.overflow) return false;
   ad_prune(st);
-  hot.mode = 1;
+  AdHot hot{nullptr, -1, 0.0};
+  Ctx<ADual<V>> d;
+  make_ctx_of<ADual<V>>(c, a, [&](const V* val, double* adj) { return AdPtr<V>{val, adj, &hot, 0}; }, d);
   for (int k = 0; k < st.nin; ++k) {
     hot.seed_base = st.in[k].base;
     hot.seed_i = st.in[k].i;

@@ -2,13 +2,14 @@
 // kernel bodies form local vector-Jacobian products for the adjoint.
 //
 // A phase of the step computes each output element from a small stencil of
-// input array values.  Run on Ctx<ADual<V>>, every array access of a body goes
-// through an AdPtr: a load yields the stored value with tangent 1 when it is the
-// element's current seed slot (0 otherwise), and in recording mode it notes the
-// (array, index) it read; a store accumulates  cotangent(output) * tangent  into
-// the element's accumulator.  One recording pass lists the element's input slots;
-// one seeded pass per slot then gives  sum_outputs cotangent * d output / d slot,
-// the slot's share of the vector-Jacobian product.  The value part of every
+// input array values.  One recording pass runs the body on Ctx<ARec<V>> (plain
+// values), whose array accesses note the (array, index) slots the element reads
+// and writes.  Then one seeded pass per input slot runs it on Ctx<ADual<V>>, whose
+// array accesses go through an AdPtr: a load yields the stored value with tangent 1
+// when it is the pass's seed slot (0 otherwise), and a store accumulates
+// cotangent(output) * tangent into the pass's accumulator, which gives
+// sum_outputs cotangent * d output / d slot, the slot's share of the
+// vector-Jacobian product.  The value part of every
 // operation is the plain V arithmetic in the forward order, so each pass sees the
 // forward values exactly.
 #pragma once
@@ -87,10 +88,9 @@ struct AdState {
   AdSlot out[kAdMaxOut];
 };
 
-// what every load and store of a pass reads or updates: kept apart from the slot
-// tables, as a few scalars the compiler can hold in registers
+// what every load and store of a seeded pass reads or updates: kept apart from the
+// slot tables, as a few scalars the compiler can hold in registers
 struct AdHot {
-  int mode;                // 0: record the accessed slots; 1: seeded pass
   const void* seed_base;   // the seeded slot (array identity and index)
   int64_t seed_i;
   double acc;              // sum over stores of cotangent * tangent
@@ -110,7 +110,6 @@ template <class V>
 struct AdPtr {
   const V* val;
   double* adj;
-  AdState* st;
   AdHot* hot;
   int64_t off;
   PMHD_FI AdRef<V> operator[](int64_t i) const { return AdRef<V>(*this, off + i); }
@@ -128,28 +127,10 @@ struct AdRef : ADual<V> {
   PMHD_FI AdRef(const AdPtr<V>& p_, int64_t i_) : ADual<V>(), p(p_), i(i_) {
     const AdHot& h = *p.hot;
     this->v = p.val[i];
-    this->t = V(0);
-    if (h.mode == 0) {
-      AdState& s = *p.st;
-      if (!slot_has(s.in, s.nin, p.val, i)) {
-        if (s.nin < kAdMaxIn) s.in[s.nin++] = AdSlot{p.val, p.adj, i};
-        else s.overflow = 1;
-      }
-    } else if (h.seed_base == p.val && h.seed_i == i) {
-      this->t = V(1);
-    }
+    this->t = (h.seed_base == p.val && h.seed_i == i) ? V(1) : V(0);
   }
   PMHD_FI AdRef& operator=(const ADual<V>& x) {
-    AdHot& h = *p.hot;
-    if (h.mode == 0) {
-      AdState& s = *p.st;
-      if (!slot_has(s.out, s.nout, p.val, i)) {
-        if (s.nout < kAdMaxOut) s.out[s.nout++] = AdSlot{p.val, p.adj, i};
-        else s.overflow = 1;
-      }
-    } else {
-      h.acc += p.adj[i] * double(x.t);
-    }
+    p.hot->acc += p.adj[i] * double(x.t);
     return *this;
   }
   PMHD_FI AdRef& operator=(const AdRef& x) { return *this = static_cast<const ADual<V>&>(x); }
@@ -159,6 +140,97 @@ template <class V>
 struct Mem<ADual<V>> {
   using cp = AdPtr<V>;
   using p = AdPtr<V>;
+};
+
+// ---------------------------------------------------------------------------
+// the recording pass: the body runs on plain values (operation for operation the
+// value parts of the seeded passes, so it takes the same branches and reads the
+// same slots), and every array access notes its slot in the element state
+// ---------------------------------------------------------------------------
+template <class V>
+struct ARec {
+  V v;
+  PMHD_HD ARec() : v(0) {}
+  PMHD_HD ARec(double x) : v(V(x)) {}
+  template <class W>
+  PMHD_HD explicit ARec(const ARec<W>& o) : v(V(o.v)) {}
+  PMHD_HD explicit operator double() const { return double(v); }
+  static PMHD_FI ARec of(V x) {
+    ARec r;
+    r.v = x;
+    return r;
+  }
+  friend PMHD_FI ARec operator+(const ARec& a, const ARec& b) { return of(a.v + b.v); }
+  friend PMHD_FI ARec operator-(const ARec& a, const ARec& b) { return of(a.v - b.v); }
+  friend PMHD_FI ARec operator*(const ARec& a, const ARec& b) { return of(a.v * b.v); }
+  friend PMHD_FI ARec operator/(const ARec& a, const ARec& b) { return of(a.v / b.v); }
+  friend PMHD_FI ARec operator-(const ARec& a) { return of(-a.v); }
+  friend PMHD_FI bool operator<(const ARec& a, const ARec& b) { return a.v < b.v; }
+  friend PMHD_FI bool operator<=(const ARec& a, const ARec& b) { return a.v <= b.v; }
+  friend PMHD_FI bool operator>(const ARec& a, const ARec& b) { return a.v > b.v; }
+  friend PMHD_FI bool operator>=(const ARec& a, const ARec& b) { return a.v >= b.v; }
+  friend PMHD_FI bool operator==(const ARec& a, const ARec& b) { return a.v == b.v; }
+  friend PMHD_FI bool operator!=(const ARec& a, const ARec& b) { return a.v != b.v; }
+  PMHD_FI ARec& operator+=(const ARec& b) { return *this = *this + b; }
+  PMHD_FI ARec& operator-=(const ARec& b) { return *this = *this - b; }
+};
+template <class V>
+PMHD_FI ARec<V> sqrt_(const ARec<V>& a) {
+  return ARec<V>::of(sqrt_(a.v));
+}
+template <class V>
+PMHD_FI ARec<V> fabs_(const ARec<V>& a) {
+  return ARec<V>::of(fabs_(a.v));
+}
+template <class V>
+struct Island<ARec<V>> {
+  using type = ARec<double>;
+};
+
+template <class V>
+struct RecRef;
+
+template <class V>
+struct RecPtr {
+  const V* val;
+  double* adj;
+  AdState* st;
+  int64_t off;
+  PMHD_FI RecRef<V> operator[](int64_t i) const { return RecRef<V>(*this, off + i); }
+  PMHD_FI RecPtr operator+(int64_t n) const {
+    RecPtr r = *this;
+    r.off += n;
+    return r;
+  }
+};
+
+template <class V>
+struct RecRef : ARec<V> {
+  RecPtr<V> p;
+  int64_t i;
+  PMHD_FI RecRef(const RecPtr<V>& p_, int64_t i_) : ARec<V>(), p(p_), i(i_) {
+    this->v = p.val[i];
+    AdState& s = *p.st;
+    if (!slot_has(s.in, s.nin, p.val, i)) {
+      if (s.nin < kAdMaxIn) s.in[s.nin++] = AdSlot{p.val, p.adj, i};
+      else s.overflow = 1;
+    }
+  }
+  PMHD_FI RecRef& operator=(const ARec<V>&) {
+    AdState& s = *p.st;
+    if (!slot_has(s.out, s.nout, p.val, i)) {
+      if (s.nout < kAdMaxOut) s.out[s.nout++] = AdSlot{p.val, p.adj, i};
+      else s.overflow = 1;
+    }
+    return *this;
+  }
+  PMHD_FI RecRef& operator=(const RecRef& x) { return *this = static_cast<const ARec<V>&>(x); }
+};
+
+template <class V>
+struct Mem<ARec<V>> {
+  using cp = RecPtr<V>;
+  using p = RecPtr<V>;
 };
 
 // after the recording pass: an element's input slots are the slots it read and
