@@ -8,11 +8,14 @@
 //   of the segment, last first, recomputes that step's intermediates into buffers
 //   that are never reused within a step, and runs the adjoint of every phase in
 //   reverse order.  A phase's adjoint evaluates the phase's own kernel body on dual
-//   numbers (pmhd_dual.h): per output element, one recording pass finds the input
-//   slots, one seeded pass per slot adds that slot's share of the vector-Jacobian
-//   product to the slot's cotangent (FP64 atomics), and the element's output
-//   cotangents are then cleared (the values they belong to are dead before the
-//   phase).  Returns the cotangent of the initial state (prim, cons, faces).
+//   numbers (pmhd_dual.h): per output element, one recording pass (plain values)
+//   finds the input slots, one seeded pass per slot adds that slot's share of the
+//   vector-Jacobian product to the slot's cotangent (FP64 atomics), and the
+//   element's output cotangents are then cleared (the values they belong to are
+//   dead before the phase).  The corrector faces are differentiated in two stages
+//   (adjoint_correct): through the Riemann flux per intermediate value, then through
+//   the CTU correction per input slot.  Returns the cotangent of the initial state
+//   (prim, cons, faces).
 //
 // The loss's dependence on dt is not differentiated: dt is a fixed input.
 #pragma once
@@ -112,33 +115,103 @@ PMHD_FI void make_ctx_of(const Ctx<V>& c, const Ctx<double>& a, Mk mk, Ctx<D>& d
   d.fcnt = nullptr;   // the dual evaluations floor without counting
 }
 
-// adjoint of one output element of phase Ph; returns false when the element has
-// more input or output slots than the state holds
+template <class Ph>
+struct is_correct : std::false_type {};
+template <int D, int S>
+struct is_correct<Correct<D, S>> : std::true_type {};
+
+// Adjoint of one corrector face in two stages.  The face's flux depends on its
+// input slots only through the CTU-corrected states u_L, u_R and the half-step
+// normal field b_n (15 values x).  First one forward pass per x_q through the flux
+// (cons->prim and the Roe solver) gives g_q = sum_v cotangent(F_v) dF_v/dx_q; then
+// one pass per input slot through the CTU correction alone gives the slot's share
+// sum_q g_q dx_q/dslot.  15 passes through the Roe solver instead of one per slot.
 template <class Ph, class V, class AtomicAdd>
-PMHD_FI bool adjoint_element(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, AtomicAdd add) {
+PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, AdState& st, AtomicAdd add) {
+  constexpr int Dir = Ph::dir;
+  using D1 = ADual<V>;
   {
-    // recording pass on plain values
+    // recording pass (input and output slots of the whole body)
     Ctx<ARec<V>> r;
     make_ctx_of<ARec<V>>(c, a, [&](const V* val, double* adj) { return RecPtr<V>{val, adj, &st, 0}; }, r);
     st.nin = 0;
     st.nout = 0;
     st.overflow = 0;
-    call_phase<Ph>(r, p);
+    call_phase<Ph>(r, f);
   }
   if (st.overflow) return false;
   ad_prune(st);
   AdHot hot{nullptr, -1, 0.0};
-  Ctx<ADual<V>> d;
-  make_ctx_of<ADual<V>>(c, a, [&](const V* val, double* adj) { return AdPtr<V>{val, adj, &hot, 0}; }, d);
+  Ctx<D1> d;
+  make_ctx_of<D1>(c, a, [&](const V* val, double* adj) { return AdPtr<V>{val, adj, &hot, 0}; }, d);
+  const int64_t NF = c.nface(Dir);
+  const int64_t idx = c.fid(Dir, f.j, f.i, f.k);
+  // the corrected states and normal field (a pass with no seed: tangents zero)
+  D1 x[15];
+  ctu_states(d, Dir, f, x, x + 7);
+  x[14] = D1(ld(d.nfb[Dir], idx));
+  double gF[7];
+#pragma unroll
+  for (int v = 0; v < 7; ++v) gF[v] = a.Fc[Dir][v * NF + idx];
+  // stage 1: the flux's sensitivities to the 15 intermediate values
+  double g[15];
+  for (int q = 0; q < 15; ++q) {
+    D1 y[15];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) y[k] = D1(x[k].v, k == q ? V(1) : V(0), 0);
+    D1 F[7];
+    correct_flux<Ph::solver>(d, Dir, y, y + 7, y[14], F);
+    double acc = 0.0;
+#pragma unroll
+    for (int v = 0; v < 7; ++v) acc += gF[v] * double(F[v].t);
+    g[q] = acc;
+  }
+  // stage 2: one pass per input slot through the CTU correction alone
   for (int k = 0; k < st.nin; ++k) {
     hot.seed_base = st.in[k].base;
     hot.seed_i = st.in[k].i;
-    hot.acc = 0.0;
-    call_phase<Ph>(d, p);
-    if (hot.acc != 0.0) add(st.in[k].adj + st.in[k].i, hot.acc);
+    D1 y[15];
+    ctu_states(d, Dir, f, y, y + 7);
+    y[14] = D1(ld(d.nfb[Dir], idx));
+    double acc = 0.0;
+    for (int q = 0; q < 15; ++q) acc += g[q] * double(y[q].t);
+    if (acc != 0.0) add(st.in[k].adj + st.in[k].i, acc);
   }
   for (int k = 0; k < st.nout; ++k) st.out[k].adj[st.out[k].i] = 0.0;
   return true;
+}
+
+// adjoint of one output element of phase Ph; returns false when the element has
+// more input or output slots than the state holds
+template <class Ph, class V, class AtomicAdd>
+PMHD_FI bool adjoint_element(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, AtomicAdd add) {
+  if constexpr (is_correct<Ph>::value) {
+    return adjoint_correct<Ph>(c, a, p, st, add);
+  } else {
+    {
+      // recording pass on plain values
+      Ctx<ARec<V>> r;
+      make_ctx_of<ARec<V>>(c, a, [&](const V* val, double* adj) { return RecPtr<V>{val, adj, &st, 0}; }, r);
+      st.nin = 0;
+      st.nout = 0;
+      st.overflow = 0;
+      call_phase<Ph>(r, p);
+    }
+    if (st.overflow) return false;
+    ad_prune(st);
+    AdHot hot{nullptr, -1, 0.0};
+    Ctx<ADual<V>> d;
+    make_ctx_of<ADual<V>>(c, a, [&](const V* val, double* adj) { return AdPtr<V>{val, adj, &hot, 0}; }, d);
+    for (int k = 0; k < st.nin; ++k) {
+      hot.seed_base = st.in[k].base;
+      hot.seed_i = st.in[k].i;
+      hot.acc = 0.0;
+      call_phase<Ph>(d, p);
+      if (hot.acc != 0.0) add(st.in[k].adj + st.in[k].i, hot.acc);
+    }
+    for (int k = 0; k < st.nout; ++k) st.out[k].adj[st.out[k].i] = 0.0;
+    return true;
+  }
 }
 
 // ---------------------------------------------------------------------------

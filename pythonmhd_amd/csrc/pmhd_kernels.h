@@ -1185,23 +1185,32 @@ PMHD_FI void k_ct(const Ctx<T>& c, int d, const PI& fin, const PI& lin, const PO
 // corrector: cons -> prim with the half-step normal field, Roe flux
 // ===========================================================================
 // CTU-corrected states -> primitive with the half-step normal field -> Roe flux
-template <int S, class T>
-PMHD_FI void k_correct(const Ctx<T>& c, int d, I3 f) {
-  const int64_t NF = c.nface(d);
-  const int64_t idx = c.fid(d, f.j, f.i, f.k);
-  T ul[7], ur[7], wl[7], wr[7];
-  ctu_states(c, d, f, ul, ur);
-  T bn = T(ld(c.nfb[d], idx));
-  int ke = c.ndim == 3 ? d : 0;
-  cons_to_prim(ul, bn, c.gamma, ke, wl);
-  cons_to_prim(ur, bn, c.gamma, ke, wr);
-  T rwl[7], rwr[7], rul[7], rur[7], rf[7], df[7];
+// the corrector flux of face f along d from its CTU-corrected states and half-step
+// normal field (directional order)
+template <int S, class T, class U>
+PMHD_FI void correct_flux(const Ctx<U>& c, int d, const T ul[7], const T ur[7], T bn, T df[7]) {
+  T wl[7], wr[7];
+  const int ke = c.ndim == 3 ? d : 0;
+  const T g = T(c.gamma), eps = T(c.eps);
+  cons_to_prim(ul, bn, g, ke, wl);
+  cons_to_prim(ur, bn, g, ke, wr);
+  T rwl[7], rwr[7], rul[7], rur[7], rf[7];
   to_rot(wl, d, rwl);
   to_rot(wr, d, rwr);
   to_rot(ul, d, rul);
   to_rot(ur, d, rur);
-  roe_flux<S>(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf);
+  roe_flux<S>(rwl, rwr, rul, rur, bn, g, c.efix != 0, eps, rf);
   from_rot(rf, d, df);
+}
+
+template <int S, class T>
+PMHD_FI void k_correct(const Ctx<T>& c, int d, I3 f) {
+  const int64_t NF = c.nface(d);
+  const int64_t idx = c.fid(d, f.j, f.i, f.k);
+  T ul[7], ur[7], df[7];
+  ctu_states(c, d, f, ul, ur);
+  T bn = T(ld(c.nfb[d], idx));
+  correct_flux<S>(c, d, ul, ur, bn, df);
   store7(c, c.Fc[d], NF, idx, df);
 }
 
