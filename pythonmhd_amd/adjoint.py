@@ -23,13 +23,14 @@ from ._lib import (MHD, PmhdError, _dptr, _ptr, face_shapes, load, make_grid, ma
 
 class AdjointRun:
     def __init__(self, shape, widths, bcs, gamma, order=1, entropy_fix=False, epsilon=0.5, precision=32,
-                 physics=MHD, checkpoint_every=20, min_dens=1e-12, min_pres=1e-12, min_energy=1e-12, lib=None):
+                 physics=MHD, checkpoint_every=20, min_dens=1e-12, min_pres=1e-12, min_energy=1e-12, lib=None,
+                 fix_quirks=0):
         self.lib = lib if lib is not None else load()
         self.shape = tuple(int(s) for s in shape)
         self.physics = int(physics)
         g = make_grid(self.shape, widths, bcs)
         p = make_params(gamma, 1.0, min_dens, min_pres, min_energy, order, entropy_fix, epsilon, precision,
-                        physics)
+                        physics, fix_quirks=fix_quirks)
         self.grid, self.params = g, p
         self.forward_count = 0   # forward() calls so far (backward reverses the last one)
         h = ctypes.c_void_p()

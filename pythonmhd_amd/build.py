@@ -23,6 +23,7 @@ GPU_LIB = os.path.join(ROOT, 'pythonmhd_amd', 'libpmhd.so')
 GPU_OBJ = os.path.join(ROOT, 'pythonmhd_amd', '_build')
 HOST_DIR = os.path.join(ROOT, 'tests', 'hostbuild')
 HOST_LIB = os.path.join(HOST_DIR, 'libpmhd_host.so')
+HOST_DUAL_LIB = os.path.join(HOST_DIR, 'libpmhd_host_dualc.so')
 HOST_OBJ = os.path.join(HOST_DIR, '_build')
 ARCH = os.environ.get('PYTORCH_ROCM_ARCH', 'gfx950')
 
@@ -84,6 +85,9 @@ def build_gpu(only=None):
 
 
 def build_host():
+    """The host test library, and beside it the same library with every adjoint
+    phase through the per-slot dual-number passes (libpmhd_host_dualc.so, the
+    reference the corrector's hand-written reverse is tested against)."""
     os.makedirs(HOST_OBJ, exist_ok=True)
     inc = ['-I' + INC, '-I' + CSRC]
     cmds, objs = [], []
@@ -91,10 +95,14 @@ def build_host():
         o = _obj(HOST_OBJ, src)
         cmds.append(['g++'] + GXX_COMMON + ['-fopenmp'] + flags + inc + ['-c', src, '-o', o])
         objs.append(o)
+    adj = os.path.join(HOST_DIR, 'pmhd_host_adj.cpp')
+    dual = os.path.join(HOST_OBJ, 'pmhd_host_adj_dualc.o')
+    cmds.append(['g++'] + GXX_COMMON + ['-fopenmp', '-DPMHD_ADJ_DUAL_CORRECTOR'] + inc + ['-c', adj, '-o', dual])
     _parallel(cmds)
-    tmp = HOST_LIB + '.tmp'
-    _run(['g++', '-shared', '-fPIC', '-fopenmp'] + objs + ['-o', tmp])
-    os.replace(tmp, HOST_LIB)
+    for lib, lobjs in ((HOST_LIB, objs), (HOST_DUAL_LIB, [dual if o == _obj(HOST_OBJ, adj) else o for o in objs])):
+        tmp = lib + '.tmp'
+        _run(['g++', '-shared', '-fPIC', '-fopenmp'] + lobjs + ['-o', tmp])
+        os.replace(tmp, lib)
     return HOST_LIB
 
 

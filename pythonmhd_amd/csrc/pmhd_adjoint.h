@@ -115,6 +115,248 @@ PMHD_FI void make_ctx_of(const Ctx<V>& c, const Ctx<double>& a, Mk mk, Ctx<D>& d
   d.fcnt = nullptr;   // the dual evaluations floor without counting
 }
 
+// ---------------------------------------------------------------------------
+// Reverse of the CTU correction of a face (ctu_states, pmhd_kernels.h): given the
+// cotangents gu of the corrected left and right states, add each input's share to
+// its cotangent array.  The corrected states are linear in the interface states,
+// the transverse flux differences and the EMFs, and bilinear in the source terms'
+// fields; the floors (2D) and minmod selections (3D) take the branches the forward
+// values take.  The derivatives are those of the dual-number evaluation (the FP64
+// step's 80-bit factor enters as its double value).
+// ---------------------------------------------------------------------------
+template <class V, class Add>
+PMHD_FI void ctu_apply_vjp(const Ctx<V>& c, const Ctx<double>& a, int d, int tc, I3 pcell, int64_t hi_override,
+                           I3 fedge, int64_t layer, const double gu[7], Add put) {
+  const int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
+  const int64_t NF = c.nface(tc);
+  const I3 lo = pcell;
+  I3 hi = with(pcell, tc, co(pcell, tc) + 1);
+  if (hi_override >= 0) hi = with(hi, 1, hi_override);
+  const int64_t flo = c.fid(tc, lo.j, lo.i, lo.k), fhi = c.fid(tc, hi.j, hi.i, hi.k);
+  const double h = double(c.hdtd[tc]);
+  const int vs[5] = {0, 1, 2, 3, 6};
+  for (int q = 0; q < 5; ++q) {
+    const int v = vs[q];
+    put(a.F[tc], v * NF + fhi, -h * gu[v]);
+    put(a.F[tc], v * NF + flo, h * gu[v]);
+  }
+  const I3 b = with(fedge, d, layer);
+  const I3 b10 = with(b, t1, co(b, t1) + 1);
+  const I3 b01 = with(b, t2, co(b, t2) + 1);
+  const I3 b11 = with(b10, t2, co(b, t2) + 1);
+  const double q = double(c.emfq[tc]);
+  const double* E = a.E[d];
+  if (tc == t1) {
+    const double gy = q * gu[5];   // u5 += q (e10 - e00 + e11 - e01)
+    put(E, c.eid(b10.j, b10.i, b10.k), gy);
+    put(E, c.eid(b.j, b.i, b.k), -gy);
+    put(E, c.eid(b11.j, b11.i, b11.k), gy);
+    put(E, c.eid(b01.j, b01.i, b01.k), -gy);
+  } else {
+    const double gz = -q * gu[4];  // u4 -= q (e01 - e00 + e11 - e10)
+    put(E, c.eid(b01.j, b01.i, b01.k), gz);
+    put(E, c.eid(b.j, b.i, b.k), -gz);
+    put(E, c.eid(b11.j, b11.i, b11.k), gz);
+    put(E, c.eid(b10.j, b10.i, b10.k), -gz);
+  }
+}
+
+// minmod_ctu(t, n)'s choice: 1 the first argument, -1 minus the second, 0 zero
+template <class V>
+PMHD_FI int minmod_ctu_pick(V t, V n) {
+  if (n > V(0) && t < V(0)) return (t > -n) ? 1 : -1;
+  if (n < V(0) && t > V(0)) return (t < -n) ? 1 : -1;
+  return 0;
+}
+
+template <class V, class Add>
+PMHD_FI void ctu_source_vjp(const Ctx<V>& c, const Ctx<double>& a, int d, int bc, I3 f, int64_t e,
+                            const double gu[7], Add put) {
+  const int64_t N = c.ncell();
+  const int64_t n = c.n_along(d);
+  const int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
+  const I3 pm = with(f, d, map1(e, n, bc));
+  int64_t flo, fhi;
+  if (e < 0) flo = bc == OUTFLOW ? 0 : n - 1; else flo = e;
+  const bool zq = d == 2 && !(c.fixq & PMHD_FIX_OUTFLOW_GHOSTS);
+  if (e + 1 > n) fhi = bc == OUTFLOW ? (zq ? n - 1 : n) : 1; else fhi = e + 1;
+  const I3 fh = with(f, d, fhi), fl = with(f, d, flo);
+  const I3 p1h = with(pm, t1, co(pm, t1) + 1), p2h = with(pm, t2, co(pm, t2) + 1);
+  // the forward values, in the forward's arithmetic (for the minmod choices)
+  const V dd = face_val(c, c.fb[d], d, fh, fhi) - face_val(c, c.fb[d], d, fl, flo);
+  const V d1 = face_val(c, c.fb[t1], t1, p1h, co(p1h, t1)) - face_val(c, c.fb[t1], t1, pm, co(pm, t1));
+  const V d2 = face_val(c, c.fb[t2], t2, p2h, co(p2h, t2)) - face_val(c, c.fb[t2], t2, pm, co(pm, t2));
+  V dbn, db1, db2;
+  double sn, s1, s2;   // d(db*)/d(d*)
+  if (d == 0) {
+    dbn = dd * c.invd[d]; db1 = d1 * c.invd[t1]; db2 = d2 * c.invd[t2];
+    sn = double(c.invd[d]); s1 = double(c.invd[t1]); s2 = double(c.invd[t2]);
+  } else {
+    dbn = dd / c.width[d]; db1 = d1 / c.width[t1]; db2 = d2 / c.width[t2];
+    sn = 1.0 / double(c.width[d]); s1 = 1.0 / double(c.width[t1]); s2 = 1.0 / double(c.width[t2]);
+  }
+  const int pick1 = minmod_ctu_pick(db1, dbn), pick2 = minmod_ctu_pick(db2, dbn);
+  const double mm1 = pick1 == 1 ? double(db1) : (pick1 == -1 ? -double(dbn) : 0.0);
+  const double mm2 = pick2 == 1 ? double(db2) : (pick2 == -1 ? -double(dbn) : 0.0);
+  const int64_t cm = c.cid(pm.j, pm.i, pm.k);
+  double cb[3], cv[3];
+  for (int k = 0; k < 3; ++k) {
+    cb[k] = double(ld(c.W + (4 + k) * N, cm));
+    cv[k] = double(ld(c.W + (1 + k) * N, cm));
+  }
+  const double v1 = cv[t1], v2 = cv[t2], b1 = cb[t1], b2 = cb[t2];
+  const double hdt = double(c.hdt);
+  // src1..3 = hdt cb dbn, src4 = -hdt v1 mm2, src5 = -hdt v2 mm1,
+  // src6 = hdt (-b1 v1 mm2 - b2 v2 mm1); each is added to u
+  double gcb[3] = {0, 0, 0}, gcv[3] = {0, 0, 0};
+  double gdbn = 0.0, gmm1 = 0.0, gmm2 = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    gcb[k] += hdt * double(dbn) * gu[1 + k];
+    gdbn += hdt * cb[k] * gu[1 + k];
+  }
+  gcv[t1] += -hdt * mm2 * gu[4];
+  gmm2 += -hdt * v1 * gu[4];
+  gcv[t2] += -hdt * mm1 * gu[5];
+  gmm1 += -hdt * v2 * gu[5];
+  gcb[t1] += -hdt * v1 * mm2 * gu[6];
+  gcv[t1] += -hdt * b1 * mm2 * gu[6];
+  gmm2 += -hdt * b1 * v1 * gu[6];
+  gcb[t2] += -hdt * v2 * mm1 * gu[6];
+  gcv[t2] += -hdt * b2 * mm1 * gu[6];
+  gmm1 += -hdt * b2 * v2 * gu[6];
+  double gdb1 = 0.0, gdb2 = 0.0;
+  if (pick1 == 1) gdb1 += gmm1; else if (pick1 == -1) gdbn -= gmm1;
+  if (pick2 == 1) gdb2 += gmm2; else if (pick2 == -1) gdbn -= gmm2;
+  const double gdd = gdbn * sn, gd1 = gdb1 * s1, gd2 = gdb2 * s2;
+  put(a.fb[d], c.fid(d, fh.j, fh.i, fh.k), gdd);
+  put(a.fb[d], c.fid(d, fl.j, fl.i, fl.k), -gdd);
+  put(a.fb[t1], c.fid(t1, p1h.j, p1h.i, p1h.k), gd1);
+  put(a.fb[t1], c.fid(t1, pm.j, pm.i, pm.k), -gd1);
+  put(a.fb[t2], c.fid(t2, p2h.j, p2h.i, p2h.k), gd2);
+  put(a.fb[t2], c.fid(t2, pm.j, pm.i, pm.k), -gd2);
+  for (int k = 0; k < 3; ++k) {
+    put(a.W, (4 + k) * N + cm, gcb[k]);
+    put(a.W, (1 + k) * N + cm, gcv[k]);
+  }
+}
+
+// g[0..6], g[7..13]: cotangents of the corrected left and right states of face f
+template <class V, class AtomicAdd>
+PMHD_FI void ctu_vjp(const Ctx<V>& c, const Ctx<double>& a, int d, I3 f, const double g[14], AtomicAdd add) {
+  auto put = [&](const double* arr, int64_t i, double v) {
+    if (v != 0.0) add(const_cast<double*>(arr) + i, v);
+  };
+  const int64_t N = c.ncell();
+  const int64_t NF = c.nface(d);
+  const int64_t idx = c.fid(d, f.j, f.i, f.k);
+  const int64_t n = c.n_along(d);
+  const int bc = c.bc[d];
+  const int64_t e = co(f, d);
+  if (c.ndim == 3) {
+    for (int v = 0; v < 7; ++v) {
+      put(a.UL[d], v * NF + idx, g[v]);
+      put(a.UR[d], v * NF + idx, g[7 + v]);
+    }
+    const I3 pl = with(f, d, map1(e - 1, n, bc));
+    const I3 pr = with(f, d, map1(e, n, bc));
+    const int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
+    const int first = d == 1 ? t2 : t1;
+    const int second = d == 1 ? t1 : t2;
+    const int64_t layer_l = e == 0 ? (bc == PERIODIC ? n - 1 : 0) : e - 1;
+    const int64_t layer_r = e;
+    const int64_t hov = (d == 1 && bc == OUTFLOW && e == n && !(c.fixq & PMHD_FIX_OUTFLOW_GHOSTS))
+                            ? (c.nx < c.ny ? c.nx - 1 : c.ny - 1) : -1;
+    for (int s = 0; s < 2; ++s) {
+      const int tc = s == 0 ? first : second;
+      ctu_apply_vjp(c, a, d, tc, pl, -1, f, layer_l, g, put);
+      ctu_apply_vjp(c, a, d, tc, pr, tc == 0 ? hov : -1, f, layer_r, g + 7, put);
+    }
+    ctu_source_vjp(c, a, d, bc, f, e - 1, g, put);
+    ctu_source_vjp(c, a, d, bc, f, e, g + 7, put);
+    return;
+  }
+  // 2D (getXTransverseUpdates_2D / getYTransverseUpdates_2D)
+  const int tc = 1 - d;
+  const int64_t NT = c.nface(tc);
+  const V h = c.hdtd[tc];
+  const int64_t cl = map1(e - 1, n, bc), cr = map1(e, n, bc);
+  for (int side = 0; side < 2; ++side) {
+    const int64_t ee = side == 0 ? e - 1 : e;
+    // the source term's cell and faces (as ctu_states)
+    int64_t cm, flo, fhi;
+    if (d == 0 || (c.fixq & PMHD_FIX_OUTFLOW_GHOSTS)) {
+      cm = map1(ee, n, bc);
+      flo = ee < 0 ? (bc == OUTFLOW ? 0 : n - 1) : ee;
+      fhi = ee + 1 > n ? (bc == OUTFLOW ? n : 1) : ee + 1;
+    } else if (bc == OUTFLOW) {
+      cm = ee < 0 ? (n > 1 ? 1 : 0) : (ee >= n ? n - 1 : ee);
+      flo = ee < 0 ? 1 : ee;
+      fhi = ee + 1 > n ? n - 1 : ee + 1;
+    } else {
+      cm = map1(ee, n, bc);
+      flo = ee < 0 ? n - 1 : ee;
+      fhi = ee + 1 > n ? 1 : ee + 1;
+    }
+    const I3 ps = with(f, d, cm);
+    const int64_t ci = c.cid(ps.j, ps.i, ps.k);
+    const V fvh = face_val(c, c.fb[d], d, f, fhi), fvl = face_val(c, c.fb[d], d, f, flo);
+    const V db = fvh - fvl;
+    const V cbx = ld(c.W + 4 * N, ci), cby = ld(c.W + 5 * N, ci), cbz = ld(c.W + 6 * N, ci);
+    const V cvz = ld(c.W + 3 * N, ci);
+    const V inv = c.invd[d];
+    V s[7];
+    s[0] = V(0);
+    s[1] = cbx * db * inv;
+    s[2] = cby * db * inv;
+    s[3] = cbz * db * inv;
+    s[6] = (cbz * cvz) * db * inv;
+    if (d == 0) { s[4] = V(0); s[5] = cvz * db * inv; }
+    else { s[4] = cvz * db * inv; s[5] = V(0); }
+    // the transverse flux difference of the side's cell, its slots remapped
+    const I3 pc = with(f, d, side == 0 ? cl : cr);
+    const I3 hi = with(pc, tc, co(pc, tc) + 1);
+    const int64_t tlo = c.fid(tc, pc.j, pc.i, pc.k), thi = c.fid(tc, hi.j, hi.i, hi.k);
+    V gflux[7];
+    for (int v = 0; v < 7; ++v) gflux[v] = ld(c.F[tc] + v * NT, thi) - ld(c.F[tc] + v * NT, tlo);
+    V dr[7] = {gflux[0], gflux[1], gflux[2], gflux[3], V(0), V(0), gflux[6]};
+    if (d == 0) dr[5] = gflux[4]; else dr[4] = gflux[5];
+    // the corrected state before its floors (which floors act)
+    const auto& U = side == 0 ? c.UL[d] : c.UR[d];
+    V u0 = ld(U, idx), u6 = ld(U + 6 * NF, idx);
+    u0 = u0 + ((-h) * dr[0] + c.hdt * s[0]);
+    u6 = u6 + ((-h) * dr[6] + c.hdt * s[6]);
+    double gp[7];
+    for (int v = 0; v < 7; ++v) gp[v] = g[7 * side + v];
+    if (u0 < c.min_dens) gp[0] = 0.0;
+    if (u6 < c.min_energy) gp[6] = 0.0;
+    const double* gU = side == 0 ? a.UL[d] : a.UR[d];
+    for (int v = 0; v < 7; ++v) put(gU, v * NF + idx, gp[v]);
+    // u += -h dr
+    const double hd = double(h);
+    const int fvar[7] = {0, 1, 2, 3, d == 0 ? -1 : 5, d == 0 ? 4 : -1, 6};
+    for (int v = 0; v < 7; ++v) {
+      if (fvar[v] < 0) continue;
+      put(a.F[tc], fvar[v] * NT + thi, -hd * gp[v]);
+      put(a.F[tc], fvar[v] * NT + tlo, hd * gp[v]);
+    }
+    // u += hdt s
+    const double hdt = double(c.hdt), id = double(inv), dbd = double(db);
+    const double gs1 = hdt * gp[1], gs2 = hdt * gp[2], gs3 = hdt * gp[3], gs6 = hdt * gp[6];
+    const double gsv = hdt * (d == 0 ? gp[5] : gp[4]);
+    const double bx = double(cbx), by = double(cby), bz = double(cbz), vz = double(cvz);
+    const double gbx = dbd * id * gs1, gby = dbd * id * gs2;
+    const double gbz = dbd * id * gs3 + vz * dbd * id * gs6;
+    const double gvz = bz * dbd * id * gs6 + dbd * id * gsv;
+    const double gdb = (bx * gs1 + by * gs2 + bz * gs3 + bz * vz * gs6 + vz * gsv) * id;
+    put(a.W, 4 * N + ci, gbx);
+    put(a.W, 5 * N + ci, gby);
+    put(a.W, 6 * N + ci, gbz);
+    put(a.W, 3 * N + ci, gvz);
+    put(a.fb[d], c.fid(d, with(f, d, fhi).j, with(f, d, fhi).i, with(f, d, fhi).k), gdb);
+    put(a.fb[d], c.fid(d, with(f, d, flo).j, with(f, d, flo).i, with(f, d, flo).k), -gdb);
+  }
+}
+
 template <class Ph>
 struct is_correct : std::false_type {};
 template <int D, int S>
@@ -124,29 +366,20 @@ struct is_correct<Correct<D, S>> : std::true_type {};
 // input slots only through the CTU-corrected states u_L, u_R and the half-step
 // normal field b_n (15 values x).  First one forward pass per x_q through the flux
 // (cons->prim and the Roe solver) gives g_q = sum_v cotangent(F_v) dF_v/dx_q; then
-// one pass per input slot through the CTU correction alone gives the slot's share
-// sum_q g_q dx_q/dslot.  15 passes through the Roe solver instead of one per slot.
+// the hand-written reverse of the CTU correction (ctu_vjp) distributes g to the
+// inputs.  15 passes through the Roe solver and one reverse sweep, instead of one
+// pass through the whole body per input slot (about 55).
 template <class Ph, class V, class AtomicAdd>
 PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, AdState& st, AtomicAdd add) {
+  (void)st;
   constexpr int Dir = Ph::dir;
   using D1 = ADual<V>;
-  {
-    // recording pass (input and output slots of the whole body)
-    Ctx<ARec<V>> r;
-    make_ctx_of<ARec<V>>(c, a, [&](const V* val, double* adj) { return RecPtr<V>{val, adj, &st, 0}; }, r);
-    st.nin = 0;
-    st.nout = 0;
-    st.overflow = 0;
-    call_phase<Ph>(r, f);
-  }
-  if (st.overflow) return false;
-  ad_prune(st);
   AdHot hot{nullptr, -1, 0.0};
   Ctx<D1> d;
   make_ctx_of<D1>(c, a, [&](const V* val, double* adj) { return AdPtr<V>{val, adj, &hot, 0}; }, d);
   const int64_t NF = c.nface(Dir);
   const int64_t idx = c.fid(Dir, f.j, f.i, f.k);
-  // the corrected states and normal field (a pass with no seed: tangents zero)
+  // the corrected states and normal field (the dual evaluation with no seed)
   D1 x[15];
   ctu_states(d, Dir, f, x, x + 7);
   x[14] = D1(ld(d.nfb[Dir], idx));
@@ -166,16 +399,40 @@ PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, AdStat
     for (int v = 0; v < 7; ++v) acc += gF[v] * double(F[v].t);
     g[q] = acc;
   }
-  // stage 2: one pass per input slot through the CTU correction alone
+  // stage 2: the reverse of the CTU correction, and the normal field's share
+  Ctx<V> cv = c;
+  cv.fcnt = nullptr;
+  ctu_vjp(cv, a, Dir, f, g, add);
+  if (g[14] != 0.0) add(a.nfb[Dir] + idx, g[14]);
+#pragma unroll
+  for (int v = 0; v < 7; ++v) a.Fc[Dir][v * NF + idx] = 0.0;
+  return true;
+}
+
+// the adjoint of one output element through its whole body: a recording pass, then
+// one seeded pass per input slot
+template <class Ph, class V, class AtomicAdd>
+PMHD_FI bool adjoint_generic(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, AtomicAdd add) {
+  {
+    // recording pass on plain values
+    Ctx<ARec<V>> r;
+    make_ctx_of<ARec<V>>(c, a, [&](const V* val, double* adj) { return RecPtr<V>{val, adj, &st, 0}; }, r);
+    st.nin = 0;
+    st.nout = 0;
+    st.overflow = 0;
+    call_phase<Ph>(r, p);
+  }
+  if (st.overflow) return false;
+  ad_prune(st);
+  AdHot hot{nullptr, -1, 0.0};
+  Ctx<ADual<V>> d;
+  make_ctx_of<ADual<V>>(c, a, [&](const V* val, double* adj) { return AdPtr<V>{val, adj, &hot, 0}; }, d);
   for (int k = 0; k < st.nin; ++k) {
     hot.seed_base = st.in[k].base;
     hot.seed_i = st.in[k].i;
-    D1 y[15];
-    ctu_states(d, Dir, f, y, y + 7);
-    y[14] = D1(ld(d.nfb[Dir], idx));
-    double acc = 0.0;
-    for (int q = 0; q < 15; ++q) acc += g[q] * double(y[q].t);
-    if (acc != 0.0) add(st.in[k].adj + st.in[k].i, acc);
+    hot.acc = 0.0;
+    call_phase<Ph>(d, p);
+    if (hot.acc != 0.0) add(st.in[k].adj + st.in[k].i, hot.acc);
   }
   for (int k = 0; k < st.nout; ++k) st.out[k].adj[st.out[k].i] = 0.0;
   return true;
@@ -185,32 +442,17 @@ PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, AdStat
 // more input or output slots than the state holds
 template <class Ph, class V, class AtomicAdd>
 PMHD_FI bool adjoint_element(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, AtomicAdd add) {
-  if constexpr (is_correct<Ph>::value) {
+#ifdef PMHD_ADJ_DUAL_CORRECTOR
+  // every phase through the per-slot dual passes (the reference the corrector's
+  // hand-written reverse is tested against)
+  constexpr bool reverse = false;
+#else
+  constexpr bool reverse = is_correct<Ph>::value;
+#endif
+  if constexpr (reverse) {
     return adjoint_correct<Ph>(c, a, p, st, add);
   } else {
-    {
-      // recording pass on plain values
-      Ctx<ARec<V>> r;
-      make_ctx_of<ARec<V>>(c, a, [&](const V* val, double* adj) { return RecPtr<V>{val, adj, &st, 0}; }, r);
-      st.nin = 0;
-      st.nout = 0;
-      st.overflow = 0;
-      call_phase<Ph>(r, p);
-    }
-    if (st.overflow) return false;
-    ad_prune(st);
-    AdHot hot{nullptr, -1, 0.0};
-    Ctx<ADual<V>> d;
-    make_ctx_of<ADual<V>>(c, a, [&](const V* val, double* adj) { return AdPtr<V>{val, adj, &hot, 0}; }, d);
-    for (int k = 0; k < st.nin; ++k) {
-      hot.seed_base = st.in[k].base;
-      hot.seed_i = st.in[k].i;
-      hot.acc = 0.0;
-      call_phase<Ph>(d, p);
-      if (hot.acc != 0.0) add(st.in[k].adj + st.in[k].i, hot.acc);
-    }
-    for (int k = 0; k < st.nout; ++k) st.out[k].adj[st.out[k].i] = 0.0;
-    return true;
+    return adjoint_generic<Ph>(c, a, p, st, add);
   }
 }
 
