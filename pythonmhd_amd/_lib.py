@@ -22,6 +22,7 @@ EXPORTS = ('pmhd_create', 'pmhd_destroy', 'pmhd_set_state', 'pmhd_get_state', 'p
            'pmhd_batch_set_profiling', 'pmhd_batch_get_profile', 'pmhd_batch_get_stats', 'pmhd_batch_last_error',
            'pmhd_adjoint_create', 'pmhd_adjoint_destroy', 'pmhd_adjoint_forward', 'pmhd_adjoint_backward',
            'pmhd_adjoint_forward_device', 'pmhd_adjoint_backward_device', 'pmhd_adjoint_set_stream',
+           'pmhd_adjoint_set_deterministic',
            'pmhd_adjoint_set_profiling', 'pmhd_adjoint_get_profile', 'pmhd_adjoint_last_error')
 
 
@@ -124,6 +125,7 @@ def _declare(lib):
     lib.pmhd_adjoint_forward_device.argtypes = [H, P, P, P, P, P, D, I64, P, P, P, P, P]
     lib.pmhd_adjoint_backward_device.argtypes = [H] + [P] * 10
     lib.pmhd_adjoint_set_stream.argtypes = [H, P]
+    lib.pmhd_adjoint_set_deterministic.argtypes = [H, I32]
     lib.pmhd_adjoint_set_profiling.argtypes = [H, I32]
     lib.pmhd_adjoint_get_profile.argtypes = lib.pmhd_get_profile.argtypes
     lib.pmhd_adjoint_last_error.argtypes = [H]

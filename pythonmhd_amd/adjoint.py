@@ -24,7 +24,7 @@ from ._lib import (MHD, PmhdError, _dptr, _ptr, face_shapes, load, make_grid, ma
 class AdjointRun:
     def __init__(self, shape, widths, bcs, gamma, order=1, entropy_fix=False, epsilon=0.5, precision=32,
                  physics=MHD, checkpoint_every=20, min_dens=1e-12, min_pres=1e-12, min_energy=1e-12, lib=None,
-                 fix_quirks=0):
+                 fix_quirks=0, deterministic=True):
         self.lib = lib if lib is not None else load()
         self.shape = tuple(int(s) for s in shape)
         self.physics = int(physics)
@@ -39,6 +39,7 @@ class AdjointRun:
             raise PmhdError('pmhd_adjoint_create failed (%d): %s' %
                             (rc, self.lib.pmhd_adjoint_last_error(None).decode()))
         self.h = h
+        self.set_deterministic(deterministic)
 
     def _check(self, rc, what):
         if rc != 0:
@@ -129,6 +130,11 @@ class AdjointRun:
         self._check(self.lib.pmhd_adjoint_backward(self.h, _ptr(gp), _ptr(gc), *gfi, _ptr(p0), _ptr(c0), *f0p),
                     'pmhd_adjoint_backward')
         return p0, c0, f0
+
+    def set_deterministic(self, on):
+        """True (the default): backward() sums each cotangent's contributions in a
+        fixed order, so repeated runs give the same bits; False: FP64 atomics."""
+        self._check(self.lib.pmhd_adjoint_set_deterministic(self.h, 1 if on else 0), 'pmhd_adjoint_set_deterministic')
 
     def set_profiling(self, on):
         self._check(self.lib.pmhd_adjoint_set_profiling(self.h, 1 if on else 0), 'pmhd_adjoint_set_profiling')

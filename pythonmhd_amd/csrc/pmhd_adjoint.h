@@ -32,6 +32,7 @@ struct pmhd_adjoint {
   virtual int backward(const double* gprim, const double* gcons, const double* const gf[3], double* gprim0,
                        double* gcons0, double* const gf0[3], bool dev) = 0;
   virtual int set_stream(void* stream) = 0;
+  virtual int set_deterministic(int on) = 0;
   virtual int set_profiling(int on) = 0;
   virtual int get_profile(int maxn, char* names, double* ms, int64_t* launches, int64_t* elems, int32_t* n) = 0;
   int fail(int code, const std::string& m) {
@@ -241,11 +242,9 @@ PMHD_FI void ctu_source_vjp(const Ctx<V>& c, const Ctx<double>& a, int d, int bc
 }
 
 // g[0..6], g[7..13]: cotangents of the corrected left and right states of face f
-template <class V, class AtomicAdd>
-PMHD_FI void ctu_vjp(const Ctx<V>& c, const Ctx<double>& a, int d, I3 f, const double g[14], AtomicAdd add) {
-  auto put = [&](const double* arr, int64_t i, double v) {
-    if (v != 0.0) add(const_cast<double*>(arr) + i, v);
-  };
+template <class V, class Add>
+PMHD_FI void ctu_vjp(const Ctx<V>& c, const Ctx<double>& a, int d, I3 f, const double g[14], Add add) {
+  auto put = [&](const double* arr, int64_t i, double v) { add(const_cast<double*>(arr) + i, v); };
   const int64_t N = c.ncell();
   const int64_t NF = c.nface(d);
   const int64_t idx = c.fid(d, f.j, f.i, f.k);
@@ -369,8 +368,8 @@ struct is_correct<Correct<D, S>> : std::true_type {};
 // the hand-written reverse of the CTU correction (ctu_vjp) distributes g to the
 // inputs.  15 passes through the Roe solver and one reverse sweep, instead of one
 // pass through the whole body per input slot (about 55).
-template <class Ph, class V, class AtomicAdd>
-PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, AdState& st, AtomicAdd add) {
+template <class Ph, class V, class Add>
+PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, AdState& st, Add add) {
   (void)st;
   constexpr int Dir = Ph::dir;
   using D1 = ADual<V>;
@@ -403,7 +402,7 @@ PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, AdStat
   Ctx<V> cv = c;
   cv.fcnt = nullptr;
   ctu_vjp(cv, a, Dir, f, g, add);
-  if (g[14] != 0.0) add(a.nfb[Dir] + idx, g[14]);
+  add(a.nfb[Dir] + idx, g[14]);
 #pragma unroll
   for (int v = 0; v < 7; ++v) a.Fc[Dir][v * NF + idx] = 0.0;
   return true;
@@ -411,8 +410,8 @@ PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, AdStat
 
 // the adjoint of one output element through its whole body: a recording pass, then
 // one seeded pass per input slot
-template <class Ph, class V, class AtomicAdd>
-PMHD_FI bool adjoint_generic(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, AtomicAdd add) {
+template <class Ph, class V, class Add>
+PMHD_FI bool adjoint_generic(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, Add add) {
   {
     // recording pass on plain values
     Ctx<ARec<V>> r;
@@ -432,16 +431,23 @@ PMHD_FI bool adjoint_generic(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdStat
     hot.seed_i = st.in[k].i;
     hot.acc = 0.0;
     call_phase<Ph>(d, p);
-    if (hot.acc != 0.0) add(st.in[k].adj + st.in[k].i, hot.acc);
+    add(st.in[k].adj + st.in[k].i, hot.acc);
   }
   for (int k = 0; k < st.nout; ++k) st.out[k].adj[st.out[k].i] = 0.0;
   return true;
 }
 
+// at most this many contributions per element (input slots of the generic path;
+// the corrector's reverse adds at most 95)
+constexpr int kAdjMaxContrib = 160;
+
 // adjoint of one output element of phase Ph; returns false when the element has
-// more input or output slots than the state holds
-template <class Ph, class V, class AtomicAdd>
-PMHD_FI bool adjoint_element(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, AtomicAdd add) {
+// more input or output slots than the state holds.  add(p, v) receives each of the
+// element's contributions to the cotangent arrays, zeros included, in an order and
+// to targets that depend only on the element's position and the configuration (the
+// deterministic accumulation relies on this; atomic sinks skip the zeros)
+template <class Ph, class V, class Add>
+PMHD_FI bool adjoint_element(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, Add add) {
 #ifdef PMHD_ADJ_DUAL_CORRECTOR
   // every phase through the per-slot dual passes (the reference the corrector's
   // hand-written reverse is tested against)
@@ -643,12 +649,29 @@ struct AdjointHandle : public pmhd_adjoint {
     try {
       alloc_grid(c, sb, p.reconstruct_order, [&](int64_t n) { return dalloc(n); });
       alloc_step(c, vals, [&](int64_t n) { return dalloc(n); });
-      alloc_step(c, adjs, [&](int64_t n) { return dalloc_d(n); });
-      for (int s = 0; s < 2; ++s) {
-        adj_state[s][0] = dalloc_d(nv() * c.ncell());
-        adj_state[s][1] = dalloc_d(nv() * c.ncell());
-        for (int d = 0; d < 3; ++d) adj_state[s][2 + d] = dalloc_d(nface(d) > 0 ? nface(d) : 1);
-      }
+      // every cotangent array in one zeroed block: a contribution's target is then
+      // an index into it (the deterministic accumulation sorts by that index)
+      int64_t total = 0;
+      double* arena = nullptr;
+      auto sub = [&](int64_t n) {
+        double* ptr = arena ? arena + total : nullptr;
+        total += (std::max<int64_t>(n, 1) + 63) / 64 * 64;
+        return ptr;
+      };
+      auto carve = [&] {
+        alloc_step(c, adjs, sub);
+        for (int s = 0; s < 2; ++s) {
+          adj_state[s][0] = sub(nv() * c.ncell());
+          adj_state[s][1] = sub(nv() * c.ncell());
+          for (int d = 0; d < 3; ++d) adj_state[s][2 + d] = sub(nface(d) > 0 ? nface(d) : 1);
+        }
+      };
+      carve();
+      const int64_t len = total;
+      arena = dalloc_d(len);
+      total = 0;
+      carve();
+      be.set_arena(arena, len);
       for (int k = 0; k < every; ++k) seg.push_back(new_snap());
     } catch (const std::bad_alloc&) {
       return fail(PMHD_ENOMEM, "device allocation failed");
@@ -714,6 +737,11 @@ struct AdjointHandle : public pmhd_adjoint {
     else be.zero(dst, sizeof(double) * size_t(n));
   }
 
+  int set_deterministic(int on) override {
+    be.set_deterministic(on != 0);
+    return PMHD_OK;
+  }
+
   int set_stream(void* stream) override {
     be.set_stream(stream);
     const char* e = be.error();
@@ -769,11 +797,19 @@ struct AdjointHandle : public pmhd_adjoint {
           if (nface(d) > 0) be.d2d(adjs.fb2[d], adj_state[out][2 + d], sizeof(double) * size_t(nface(d)));
         run_cycle(tx, cs);
         if (tx.fused) return fail(PMHD_EINVAL, "the adjoint needs separate predictor launches (seg = 0)");
-        for (size_t t = tx.tape.size(); t-- > 0;) tx.tape[t]();
-        if (be.adjoint_overflow()) return fail(PMHD_EINVAL, "adjoint element stencil larger than supported");
+        for (size_t t = tx.tape.size(); t-- > 0;) {
+          be.adjoint_select(in, t);   // the launch's accumulation plan (its targets depend on in)
+          tx.tape[t]();
+        }
+        if (be.adjoint_overflow())
+          return fail(PMHD_EINVAL, "adjoint element with more contributions than supported, or whose "
+                                   "contributions changed between steps");
+        const char* ae = be.error();
+        if (ae) return fail(PMHD_EHIP, ae);
         out = in;
       }
     }
+    be.adjoint_end();
     // cotangent of the initial state
     std::vector<double> tmp;
     auto down = [&](double* dst, const double* src, int64_t n) {
@@ -910,6 +946,10 @@ PMHD_NS_END
   extern "C" int pmhd_adjoint_set_stream(pmhd_adjoint* a, void* stream) {                           \
     if (!a) return PMHD_EINVAL;                                                                     \
     return a->set_stream(stream);                                                                   \
+  }                                                                                                 \
+  extern "C" int pmhd_adjoint_set_deterministic(pmhd_adjoint* a, int32_t enable) {                  \
+    if (!a) return PMHD_EINVAL;                                                                     \
+    return a->set_deterministic(enable);                                                            \
   }                                                                                                 \
   extern "C" int pmhd_adjoint_set_profiling(pmhd_adjoint* a, int32_t enable) {                      \
     if (!a) return PMHD_EINVAL;                                                                     \

@@ -424,7 +424,9 @@ int riemann_run(int d, int solver, int64_t n, const double* wl, const double* wr
 // adjoint of one phase: one thread per output element (pmhd_adjoint.h)
 // ---------------------------------------------------------------------------
 struct DevAtomicAdd {
-  __device__ void operator()(double* p, double v) const { atomicAdd(p, v); }
+  __device__ void operator()(double* p, double v) const {
+    if (v != 0.0) atomicAdd(p, v);
+  }
 };
 
 template <class Ph, class T>

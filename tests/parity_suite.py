@@ -587,6 +587,39 @@ def test_adjoint_checkpoint_spacing_and_fp32(lib):
     assert np.max(np.abs(grads[3][1] - ref[1])) <= 1e-3 * scale
 
 
+@pytest.mark.parametrize('nd,prec', [(2, 64), (2, 32), (3, 64)])
+def test_adjoint_deterministic_repeats(lib, nd, prec):
+    """Deterministic accumulation (the default): two backward passes of the same
+    forward, and a backward of a second handle, give the same bits; the FP64-atomic
+    accumulation gives the same gradient to rounding."""
+    from pythonmhd_amd.adjoint import AdjointRun
+    shape = (24, 20) if nd == 2 else (8, 10, 6)
+    rng = np.random.default_rng(70 + nd)
+    prim, faces = _generic_state(rng, shape)
+    cons = M.prim_to_cons_all(prim, GAMMA)
+    widths = widths_of(shape)
+    bcs = (1, 0) if nd == 2 else (1, 0, 1)
+    print('case: adjoint determinism', nd, prec, flush=True)
+
+    def grads(run):
+        _, co, _ = run.forward(prim, cons, faces, 0.003, 4)
+        gp, gc, gf = run.backward(None, _kinetic_cotangent(co), None)
+        return [gp, gc] + list(gf)
+    r1 = AdjointRun(shape, widths, bcs, GAMMA, order=1, precision=prec, checkpoint_every=2, lib=lib)
+    a = grads(r1)
+    b = grads(r1)
+    r1.close()
+    r2 = AdjointRun(shape, widths, bcs, GAMMA, order=1, precision=prec, checkpoint_every=3, lib=lib)
+    c = grads(r2)
+    r2.set_deterministic(False)
+    d = grads(r2)
+    r2.close()
+    for x, y, z, w in zip(a, b, c, d):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+        assert np.array_equal(x.view(np.uint64), z.view(np.uint64))
+        assert np.max(np.abs(w - x)) <= 1e-12 * max(np.max(np.abs(x)), 1e-300)
+
+
 def test_adjoint_hydro_1d(lib):
     from pythonmhd_amd._lib import HYDRO
     from pythonmhd_amd.adjoint import AdjointRun
