@@ -61,7 +61,8 @@ PMHD_FI void call_phase(const Ctx<T>& c, I3 p) {
 
 
 // the context of a phase on number type D: values from c, cotangents from a, each
-// array wrapped by mk(values, cotangents)
+// array wrapped by mk(values, cotangents, id) with a fixed id per array member
+// (0 .. kAdIds-1)
 template <class D, class V, class Mk>
 PMHD_FI void make_ctx_of(const Ctx<V>& c, const Ctx<double>& a, Mk mk, Ctx<D>& d) {
   d.ndim = c.ndim;
@@ -94,25 +95,26 @@ PMHD_FI void make_ctx_of(const Ctx<V>& c, const Ctx<double>& a, Mk mk, Ctx<D>& d
     d.width[q] = D(double(c.width[q]));
     d.wd[q] = c.wd[q];
     d.emfq[q] = D(double(c.emfq[q]));
-    d.fb[q] = mk(c.fb[q], const_cast<double*>(a.fb[q]));
-    d.fb2[q] = mk(c.fb2[q], a.fb2[q]);
-    d.nfb[q] = mk(c.nfb[q], a.nfb[q]);
-    d.UL[q] = mk(c.UL[q], a.UL[q]);
-    d.UR[q] = mk(c.UR[q], a.UR[q]);
-    d.F[q] = mk(c.F[q], a.F[q]);
-    d.Fc[q] = mk(c.Fc[q], a.Fc[q]);
-    d.E[q] = mk(c.E[q], a.E[q]);
+    d.fb[q] = mk(c.fb[q], const_cast<double*>(a.fb[q]), 8 * q);
+    d.fb2[q] = mk(c.fb2[q], a.fb2[q], 8 * q + 1);
+    d.nfb[q] = mk(c.nfb[q], a.nfb[q], 8 * q + 2);
+    d.UL[q] = mk(c.UL[q], a.UL[q], 8 * q + 3);
+    d.UR[q] = mk(c.UR[q], a.UR[q], 8 * q + 4);
+    d.F[q] = mk(c.F[q], a.F[q], 8 * q + 5);
+    d.Fc[q] = mk(c.Fc[q], a.Fc[q], 8 * q + 6);
+    d.E[q] = mk(c.E[q], a.E[q], 8 * q + 7);
   }
   d.ldh[0] = c.ldh[0];
   d.ldh[1] = c.ldh[1];
-  d.U = mk(c.U, const_cast<double*>(a.U));
-  d.W = mk(c.W, const_cast<double*>(a.W));
-  d.U2 = mk(c.U2, a.U2);
-  d.W2 = mk(c.W2, a.W2);
-  d.WL = mk(c.WL, a.WL);
-  d.WR = mk(c.WR, a.WR);
-  d.S = mk(c.S, a.S);
-  d.Ec = mk(c.Ec, a.Ec);
+  d.U = mk(c.U, const_cast<double*>(a.U), 24);
+  d.W = mk(c.W, const_cast<double*>(a.W), 25);
+  d.U2 = mk(c.U2, a.U2, 26);
+  d.W2 = mk(c.W2, a.W2, 27);
+  d.WL = mk(c.WL, a.WL, 28);
+  d.WR = mk(c.WR, a.WR, 29);
+  d.S = mk(c.S, a.S, 30);
+  d.Ec = mk(c.Ec, a.Ec, 31);
+  static_assert(kAdIds == 32, "array ids 0..31");
   d.fcnt = nullptr;   // the dual evaluations floor without counting
 }
 
@@ -373,9 +375,9 @@ PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, AdStat
   (void)st;
   constexpr int Dir = Ph::dir;
   using D1 = ADual<V>;
-  AdHot hot{nullptr, -1, 0.0};
+  AdHot hot{-1, -1, 0.0};
   Ctx<D1> d;
-  make_ctx_of<D1>(c, a, [&](const V* val, double* adj) { return AdPtr<V>{val, adj, &hot, 0}; }, d);
+  make_ctx_of<D1>(c, a, [&](const V* val, double* adj, int id) { return AdPtr<V>{val, adj, &hot, 0, id}; }, d);
   const int64_t NF = c.nface(Dir);
   const int64_t idx = c.fid(Dir, f.j, f.i, f.k);
   // the corrected states and normal field (the dual evaluation with no seed)
@@ -408,33 +410,97 @@ PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, AdStat
   return true;
 }
 
-// the adjoint of one output element through its whole body: a recording pass, then
-// one seeded pass per input slot
+// the slot lists of one element kept from an earlier recording pass of the same
+// launch (rows of E elements: inputs in rows 0 .. nin-1, outputs from row rin), so
+// later evaluations skip the recording pass; write: record and keep them
+struct SlotView {
+  uint64_t* slots;
+  int32_t* nin;
+  int32_t* nout;
+  int64_t E, e;
+  int rin;
+  bool write;
+};
+// the cotangent array of array id (the ids make_ctx_of gives)
+PMHD_FI double* adj_array(const Ctx<double>& a, int id) {
+  if (id < 24) {
+    const int q = id >> 3;
+    switch (id & 7) {
+      case 0: return const_cast<double*>(a.fb[q]);
+      case 1: return a.fb2[q];
+      case 2: return a.nfb[q];
+      case 3: return a.UL[q];
+      case 4: return a.UR[q];
+      case 5: return a.F[q];
+      case 6: return a.Fc[q];
+      default: return a.E[q];
+    }
+  }
+  switch (id) {
+    case 24: return const_cast<double*>(a.U);
+    case 25: return const_cast<double*>(a.W);
+    case 26: return a.U2;
+    case 27: return a.W2;
+    case 28: return a.WL;
+    case 29: return a.WR;
+    case 30: return a.S;
+    default: return a.Ec;
+  }
+}
+PMHD_FI uint64_t slot_code(const AdSlot& s) { return (uint64_t(s.id) << 56) | uint64_t(s.i); }
+PMHD_FI AdSlot slot_of(uint64_t v) { return AdSlot{int(v >> 56), int64_t(v & ((uint64_t(1) << 56) - 1))}; }
+
+// the adjoint of one output element through its whole body: a recording pass (or
+// the lists an earlier one kept), then one seeded pass per input slot
 template <class Ph, class V, class Add>
-PMHD_FI bool adjoint_generic(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, Add add) {
-  {
+PMHD_FI bool adjoint_generic(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, Add add,
+                             const SlotView* sv) {
+  const bool kept = sv && !sv->write;
+  if (!kept) {
     // recording pass on plain values
     Ctx<ARec<V>> r;
-    make_ctx_of<ARec<V>>(c, a, [&](const V* val, double* adj) { return RecPtr<V>{val, adj, &st, 0}; }, r);
+    make_ctx_of<ARec<V>>(c, a, [&](const V* val, double*, int id) { return RecPtr<V>{val, &st, 0, id}; }, r);
     st.nin = 0;
     st.nout = 0;
     st.overflow = 0;
     call_phase<Ph>(r, p);
+    if (st.overflow) return false;
+    ad_prune(st);
+    if (sv) {
+      for (int k = 0; k < st.nin; ++k) sv->slots[k * sv->E + sv->e] = slot_code(st.in[k]);
+      for (int k = 0; k < st.nout; ++k) sv->slots[(sv->rin + k) * sv->E + sv->e] = slot_code(st.out[k]);
+      sv->nin[sv->e] = st.nin;
+      sv->nout[sv->e] = st.nout;
+    }
   }
-  if (st.overflow) return false;
-  ad_prune(st);
-  AdHot hot{nullptr, -1, 0.0};
+  const int nin = kept ? sv->nin[sv->e] : st.nin;
+  const int nout = kept ? sv->nout[sv->e] : st.nout;
+  AdHot hot{-1, -1, 0.0};
   Ctx<ADual<V>> d;
-  make_ctx_of<ADual<V>>(c, a, [&](const V* val, double* adj) { return AdPtr<V>{val, adj, &hot, 0}; }, d);
-  for (int k = 0; k < st.nin; ++k) {
-    hot.seed_base = st.in[k].base;
-    hot.seed_i = st.in[k].i;
+  make_ctx_of<ADual<V>>(c, a, [&](const V* val, double* adj, int id) { return AdPtr<V>{val, adj, &hot, 0, id}; }, d);
+  for (int k = 0; k < nin; ++k) {
+    const AdSlot s = kept ? slot_of(sv->slots[k * sv->E + sv->e]) : st.in[k];
+    hot.seed_id = s.id;
+    hot.seed_i = s.i;
     hot.acc = 0.0;
     call_phase<Ph>(d, p);
-    add(st.in[k].adj + st.in[k].i, hot.acc);
+    add(adj_array(a, s.id) + s.i, hot.acc);
   }
-  for (int k = 0; k < st.nout; ++k) st.out[k].adj[st.out[k].i] = 0.0;
+  for (int k = 0; k < nout; ++k) {
+    const AdSlot s = kept ? slot_of(sv->slots[(sv->rin + k) * sv->E + sv->e]) : st.out[k];
+    adj_array(a, s.id)[s.i] = 0.0;
+  }
   return true;
+}
+
+// whether the element adjoint of Ph records slot lists (the generic path)
+template <class Ph>
+constexpr bool uses_slots() {
+#ifdef PMHD_ADJ_DUAL_CORRECTOR
+  return true;
+#else
+  return !is_correct<Ph>::value;
+#endif
 }
 
 // at most this many contributions per element (input slots of the generic path;
@@ -447,7 +513,8 @@ constexpr int kAdjMaxContrib = 160;
 // to targets that depend only on the element's position and the configuration (the
 // deterministic accumulation relies on this; atomic sinks skip the zeros)
 template <class Ph, class V, class Add>
-PMHD_FI bool adjoint_element(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, Add add) {
+PMHD_FI bool adjoint_element(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, Add add,
+                             const SlotView* sv = nullptr) {
 #ifdef PMHD_ADJ_DUAL_CORRECTOR
   // every phase through the per-slot dual passes (the reference the corrector's
   // hand-written reverse is tested against)
@@ -458,7 +525,7 @@ PMHD_FI bool adjoint_element(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdStat
   if constexpr (reverse) {
     return adjoint_correct<Ph>(c, a, p, st, add);
   } else {
-    return adjoint_generic<Ph>(c, a, p, st, add);
+    return adjoint_generic<Ph>(c, a, p, st, add, sv);
   }
 }
 

@@ -74,9 +74,12 @@ struct Island<ADual<V>> {
 constexpr int kAdMaxIn = 160;
 constexpr int kAdMaxOut = 40;
 
+// the arrays of a context carry ids 0 .. kAdIds-1 (make_ctx_of); a slot is an
+// array id and an index into that array
+constexpr int kAdIds = 32;
+
 struct AdSlot {
-  const void* base;   // value array (identity)
-  double* adj;        // its cotangent array
+  int id;
   int64_t i;
 };
 
@@ -91,14 +94,14 @@ struct AdState {
 // what every load and store of a seeded pass reads or updates: kept apart from the
 // slot tables, as a few scalars the compiler can hold in registers
 struct AdHot {
-  const void* seed_base;   // the seeded slot (array identity and index)
+  int seed_id;      // the seeded slot (array id and index)
   int64_t seed_i;
-  double acc;              // sum over stores of cotangent * tangent
+  double acc;       // sum over stores of cotangent * tangent
 };
 
-PMHD_FI bool slot_has(const AdSlot* s, int n, const void* base, int64_t i) {
+PMHD_FI bool slot_has(const AdSlot* s, int n, int id, int64_t i) {
   for (int k = 0; k < n; ++k)
-    if (s[k].base == base && s[k].i == i) return true;
+    if (s[k].id == id && s[k].i == i) return true;
   return false;
 }
 
@@ -112,6 +115,7 @@ struct AdPtr {
   double* adj;
   AdHot* hot;
   int64_t off;
+  int id;
   PMHD_FI AdRef<V> operator[](int64_t i) const { return AdRef<V>(*this, off + i); }
   PMHD_FI AdPtr operator+(int64_t n) const {
     AdPtr r = *this;
@@ -127,7 +131,7 @@ struct AdRef : ADual<V> {
   PMHD_FI AdRef(const AdPtr<V>& p_, int64_t i_) : ADual<V>(), p(p_), i(i_) {
     const AdHot& h = *p.hot;
     this->v = p.val[i];
-    this->t = (h.seed_base == p.val && h.seed_i == i) ? V(1) : V(0);
+    this->t = (h.seed_id == p.id && h.seed_i == i) ? V(1) : V(0);
   }
   PMHD_FI AdRef& operator=(const ADual<V>& x) {
     p.hot->acc += p.adj[i] * double(x.t);
@@ -193,9 +197,9 @@ struct RecRef;
 template <class V>
 struct RecPtr {
   const V* val;
-  double* adj;
   AdState* st;
   int64_t off;
+  int id;
   PMHD_FI RecRef<V> operator[](int64_t i) const { return RecRef<V>(*this, off + i); }
   PMHD_FI RecPtr operator+(int64_t n) const {
     RecPtr r = *this;
@@ -211,15 +215,15 @@ struct RecRef : ARec<V> {
   PMHD_FI RecRef(const RecPtr<V>& p_, int64_t i_) : ARec<V>(), p(p_), i(i_) {
     this->v = p.val[i];
     AdState& s = *p.st;
-    if (!slot_has(s.in, s.nin, p.val, i)) {
-      if (s.nin < kAdMaxIn) s.in[s.nin++] = AdSlot{p.val, p.adj, i};
+    if (!slot_has(s.in, s.nin, p.id, i)) {
+      if (s.nin < kAdMaxIn) s.in[s.nin++] = AdSlot{p.id, i};
       else s.overflow = 1;
     }
   }
   PMHD_FI RecRef& operator=(const ARec<V>&) {
     AdState& s = *p.st;
-    if (!slot_has(s.out, s.nout, p.val, i)) {
-      if (s.nout < kAdMaxOut) s.out[s.nout++] = AdSlot{p.val, p.adj, i};
+    if (!slot_has(s.out, s.nout, p.id, i)) {
+      if (s.nout < kAdMaxOut) s.out[s.nout++] = AdSlot{p.id, i};
       else s.overflow = 1;
     }
     return *this;
@@ -238,7 +242,7 @@ struct Mem<ARec<V>> {
 PMHD_FI void ad_prune(AdState& s) {
   int n = 0;
   for (int k = 0; k < s.nin; ++k)
-    if (!slot_has(s.out, s.nout, s.in[k].base, s.in[k].i)) s.in[n++] = s.in[k];
+    if (!slot_has(s.out, s.nout, s.in[k].id, s.in[k].i)) s.in[n++] = s.in[k];
   s.nin = n;
 }
 

@@ -16,6 +16,10 @@
 // (target, place), a run-length encoding and a scan give the targets, the start of
 // each target's run and the places in ascending order.  Later launches check that
 // each element still makes as many contributions, to targets with the same hash.
+//
+// The generic element adjoints also keep their recording pass's slot lists (the
+// input slots to seed and the outputs whose cotangents they consume) with the
+// launch, by array id and index, so later evaluations skip the recording pass.
 #pragma once
 #include <hipcub/hipcub.hpp>
 
@@ -56,8 +60,20 @@ struct DevDetAdd {
   }
 };
 
+// the slot lists of a launch: written by its first evaluation (rows of E elements,
+// outputs from row rin), read by the later ones
+struct SlotBufs {
+  uint64_t* slots;
+  int32_t* nin;
+  int32_t* nout;
+  int rin;
+  bool write;
+  bool use;
+};
+
 template <class Ph, class T>
-__global__ void __launch_bounds__(64) k_adjoint_det(const Ctx<T> c, const Ctx<double> a, int n1, int n2, DetSink s) {
+__global__ void __launch_bounds__(64) k_adjoint_det(const Ctx<T> c, const Ctx<double> a, int n1, int n2, DetSink s,
+                                                    SlotBufs sb) {
   const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
   const int i = int(blockIdx.y * blockDim.y + threadIdx.y);
   const int j = int(blockIdx.z);
@@ -66,7 +82,8 @@ __global__ void __launch_bounds__(64) k_adjoint_det(const Ctx<T> c, const Ctx<do
     int n = 0;
     uint32_t h = 0x811c9dc5u;
     AdState st;
-    const bool ok = adjoint_element<Ph>(c, a, I3{j, i, k}, st, DevDetAdd{s, e, &n, &h});
+    const SlotView sv{sb.slots, sb.nin, sb.nout, s.E, e, sb.rin, sb.write};
+    const bool ok = adjoint_element<Ph>(c, a, I3{j, i, k}, st, DevDetAdd{s, e, &n, &h}, sb.use ? &sv : nullptr);
     if (!ok || n > kAdjMaxContrib) atomicOr(s.flag, 1);
     if (s.keys) {
       s.cnt[e] = n;
@@ -84,6 +101,16 @@ __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* p, uint32_t v, int64
 __global__ void __launch_bounds__(256) k_iota_u32(uint32_t* p, int64_t n) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     p[i] = uint32_t(i);
+}
+
+__global__ void __launch_bounds__(256) k_max_pair(const int32_t* a, const int32_t* b, int64_t n, int* out) {
+  int ma = 0, mb = 0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    ma = a[i] > ma ? a[i] : ma;
+    mb = b[i] > mb ? b[i] : mb;
+  }
+  atomicMax(out, ma);
+  atomicMax(out + 1, mb);
 }
 
 // one thread per target: its contributions summed in the order of their places
@@ -112,6 +139,18 @@ struct HipAdjBackend : HipBackend {
     int32_t* cnt = nullptr;
     uint32_t* hash = nullptr;
   };
+  struct SlotPlan {
+    int id = -1;
+    int64_t E = 0;
+    uint64_t* slots = nullptr;
+    int32_t* nin = nullptr;
+    int32_t* nout = nullptr;
+    int rin = 0;
+  };
+  std::vector<SlotPlan> splans;   // by tape position (the same for both parities)
+  uint64_t* slot_scratch = nullptr;
+  int64_t slot_cap = 0;
+  int* pair = nullptr;
   bool det = true;
   double* arena = nullptr;
   int64_t arena_len = 0;
@@ -168,6 +207,33 @@ struct HipAdjBackend : HipBackend {
     temp = nullptr;
     temp_cap = 0;
     sk_cap = 0;
+    drop(slot_scratch);
+    slot_scratch = nullptr;
+    slot_cap = 0;
+  }
+
+  // the kept slot lists of a launch whose first evaluation wrote them into the
+  // scratch rows (inputs from row 0, outputs from row kAdMaxIn): trimmed to the
+  // largest counts
+  void finish_slots(SlotPlan& sp, int64_t E) {
+    if (!pair) pair = static_cast<int*>(grab(2 * sizeof(int)));
+    if (!pair) return;
+    ex.note(hipMemsetAsync(pair, 0, 2 * sizeof(int), ex.s));
+    hipLaunchKernelGGL(k_max_pair, dim3(stride_blocks(E)), dim3(256), 0, ex.s, sp.nin, sp.nout, E, pair);
+    ex.note(hipGetLastError());
+    int m[2] = {0, 0};
+    ex.note(hipMemcpyAsync(m, pair, sizeof(m), hipMemcpyDeviceToHost, ex.s));
+    ex.note(hipStreamSynchronize(ex.s));
+    if (ex.last != hipSuccess) return;
+    sp.slots = static_cast<uint64_t*>(grab(sizeof(uint64_t) * size_t(E) * size_t(m[0] + m[1] > 0 ? m[0] + m[1] : 1)));
+    if (!sp.slots) return;
+    if (m[0] > 0)
+      ex.note(hipMemcpyAsync(sp.slots, slot_scratch, sizeof(uint64_t) * size_t(E) * size_t(m[0]),
+                             hipMemcpyDeviceToDevice, ex.s));
+    if (m[1] > 0)
+      ex.note(hipMemcpyAsync(sp.slots + int64_t(m[0]) * E, slot_scratch + int64_t(kAdMaxIn) * E,
+                             sizeof(uint64_t) * size_t(E) * size_t(m[1]), hipMemcpyDeviceToDevice, ex.s));
+    sp.rin = m[0];
   }
   std::string det_err;
   const char* error() {
@@ -286,6 +352,33 @@ struct HipAdjBackend : HipBackend {
       vals_cap = places;
     }
     if (ex.last != hipSuccess || !vals || !pl.cnt || !pl.hash) return;
+    // the launch's slot lists: kept from its first evaluation, else written now
+    SlotBufs sb{nullptr, nullptr, nullptr, 0, false, false};
+    SlotPlan* sp = nullptr;
+    bool sfresh = false;
+    if constexpr (uses_slots<Ph>()) {
+      if (splans.size() <= sel_t) splans.resize(sel_t + 1);
+      sp = &splans[sel_t];
+      sfresh = sp->id != int(Ph::id) || sp->E != E;
+      if (sfresh) {
+        drop(sp->slots);
+        drop(sp->nin);
+        drop(sp->nout);
+        *sp = SlotPlan();
+        sp->nin = static_cast<int32_t*>(grab(sizeof(int32_t) * size_t(E)));
+        sp->nout = static_cast<int32_t*>(grab(sizeof(int32_t) * size_t(E)));
+        const int64_t need = E * (kAdMaxIn + kAdMaxOut);
+        if (need > slot_cap) {
+          drop(slot_scratch);
+          slot_scratch = static_cast<uint64_t*>(grab(sizeof(uint64_t) * size_t(need)));
+          slot_cap = need;
+        }
+        if (ex.last != hipSuccess || !sp->nin || !sp->nout || !slot_scratch) return;
+        sb = SlotBufs{slot_scratch, sp->nin, sp->nout, kAdMaxIn, true, true};
+      } else {
+        sb = SlotBufs{sp->slots, sp->nin, sp->nout, sp->rin, false, true};
+      }
+    }
     DetSink s{vals, nullptr, pl.cnt, pl.hash, arena, arena_len, E, adj_flag};
     if (fresh) {
       s.keys = sk[0];
@@ -298,9 +391,19 @@ struct HipAdjBackend : HipBackend {
     const unsigned by = 64 / bx;
     dim3 grid(unsigned((n[2] + bx - 1) / bx), unsigned((n[1] + by - 1) / by), unsigned(n[0]));
     ex.timed(K_ADJOINT, E, [&] {
-      hipLaunchKernelGGL((k_adjoint_det<Ph, T>), grid, dim3(bx, by, 1), 0, ex.s, c, a, int(n[1]), int(n[2]), s);
+      hipLaunchKernelGGL((k_adjoint_det<Ph, T>), grid, dim3(bx, by, 1), 0, ex.s, c, a, int(n[1]), int(n[2]), s, sb);
     });
     ex.note(hipGetLastError());
+    if (sfresh) {
+      finish_slots(*sp, E);
+      if (ex.last != hipSuccess) return;
+      if (!sp->slots) {
+        det_err = "could not keep the adjoint's slot lists";
+        return;
+      }
+      sp->id = int(Ph::id);
+      sp->E = E;
+    }
     if (fresh) build_plan(pl, places);
     if (ex.last != hipSuccess) return;
     if (!pl.pos) {
