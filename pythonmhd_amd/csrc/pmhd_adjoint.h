@@ -493,6 +493,35 @@ PMHD_FI bool adjoint_generic(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdStat
   return true;
 }
 
+// the generic element adjoint from kept slot lists, split over `lanes` threads: the
+// thread of lane L seeds the element's input slots L, L + lanes, ... and writes the
+// share of slot k to vals[k * E + e].  The threads of one element must be lanes of
+// one wave: after the passes of all of them, lane 0 clears the cotangents of the
+// element's outputs.
+template <class Ph, class V>
+PMHD_FI void adjoint_generic_lanes(const Ctx<V>& c, const Ctx<double>& a, I3 p, const SlotView& sv, int lane,
+                                   int lanes, double* vals) {
+  const int nin = sv.nin[sv.e];
+  AdHot hot{-1, -1, 0.0};
+  Ctx<ADual<V>> d;
+  make_ctx_of<ADual<V>>(c, a, [&](const V* val, double* adj, int id) { return AdPtr<V>{val, adj, &hot, 0, id}; }, d);
+  for (int k = lane; k < nin; k += lanes) {
+    const AdSlot s = slot_of(sv.slots[k * sv.E + sv.e]);
+    hot.seed_id = s.id;
+    hot.seed_i = s.i;
+    hot.acc = 0.0;
+    call_phase<Ph>(d, p);
+    vals[k * sv.E + sv.e] = hot.acc;
+  }
+  if (lane == 0) {
+    const int nout = sv.nout[sv.e];
+    for (int k = 0; k < nout; ++k) {
+      const AdSlot s = slot_of(sv.slots[(sv.rin + k) * sv.E + sv.e]);
+      adj_array(a, s.id)[s.i] = 0.0;
+    }
+  }
+}
+
 // whether the element adjoint of Ph records slot lists (the generic path)
 template <class Ph>
 constexpr bool uses_slots() {

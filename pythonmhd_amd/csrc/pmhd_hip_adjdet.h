@@ -94,6 +94,22 @@ __global__ void __launch_bounds__(64) k_adjoint_det(const Ctx<T> c, const Ctx<do
   }
 }
 
+// kept slot lists: two adjacent threads (lanes of one wave) per element
+template <class Ph, class T>
+__global__ void __launch_bounds__(64) k_adjoint_kept2(const Ctx<T> c, const Ctx<double> a, int n1, int n2, int64_t E,
+                                                      double* vals, SlotBufs sb) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t e = t >> 1;
+  if (e < E) {
+    const int k = int(e % n2);
+    const int64_t r = e / n2;
+    const int i = int(r % n1);
+    const int j = int(r / n1);
+    const SlotView sv{sb.slots, sb.nin, sb.nout, E, e, sb.rin, false};
+    adjoint_generic_lanes<Ph>(c, a, I3{j, i, k}, sv, int(t & 1), 2, vals);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     p[i] = v;
@@ -390,9 +406,16 @@ struct HipAdjBackend : HipBackend {
     while (bx < 64 && int64_t(bx) < n[2]) bx <<= 1;
     const unsigned by = 64 / bx;
     dim3 grid(unsigned((n[2] + bx - 1) / bx), unsigned((n[1] + by - 1) / by), unsigned(n[0]));
-    ex.timed(K_ADJOINT, E, [&] {
-      hipLaunchKernelGGL((k_adjoint_det<Ph, T>), grid, dim3(bx, by, 1), 0, ex.s, c, a, int(n[1]), int(n[2]), s, sb);
-    });
+    if (uses_slots<Ph>() && !fresh && !sfresh) {
+      ex.timed(K_ADJOINT, E, [&] {
+        hipLaunchKernelGGL((k_adjoint_kept2<Ph, T>), dim3(unsigned((2 * E + 63) / 64)), dim3(64), 0, ex.s, c, a,
+                           int(n[1]), int(n[2]), E, vals, sb);
+      });
+    } else {
+      ex.timed(K_ADJOINT, E, [&] {
+        hipLaunchKernelGGL((k_adjoint_det<Ph, T>), grid, dim3(bx, by, 1), 0, ex.s, c, a, int(n[1]), int(n[2]), s, sb);
+      });
+    }
     ex.note(hipGetLastError());
     if (sfresh) {
       finish_slots(*sp, E);
