@@ -220,7 +220,8 @@ def run_adjoint(n=256, steps=200, every=20, ncheck=32, check_steps=20):
     s.set_state(prim, cons, faces)
     dt = s.compute_dt()
     out = {"grid": "%d^2 periodic, PLM + Roe + CTU + CT" % n, "steps": steps, "checkpoint_every": every,
-           "dt": dt, "loss": "sum 0.5 rho v^2 dx dy after the last step"}
+           "dt": dt, "loss": "sum 0.5 rho v^2 dx dy after the last step",
+           "accumulation": "deterministic (each cotangent's contributions summed in a fixed order)"}
     for prec in (32, 64):   # the plain step, same fixed dt
         sp = Solver((n, n), (h, h), (1, 1), GAMMA, 0.8, order=1, precision=prec)
         sp.set_state(prim, cons, faces)
@@ -241,9 +242,16 @@ def run_adjoint(n=256, steps=200, every=20, ncheck=32, check_steps=20):
         t0 = time.perf_counter()
         gp, gc, _ = run.backward(None, _ke_cotangent(co, area), None)
         tb = time.perf_counter() - t0
+        # a second backward of the same forward (its accumulation plans already built)
+        t0 = time.perf_counter()
+        gp2, gc2, _ = run.backward(None, _ke_cotangent(co, area), None)
+        tb2 = time.perf_counter() - t0
         run.close()
+        same = bool(np.array_equal(gp.view(np.uint64), gp2.view(np.uint64))
+                    and np.array_equal(gc.view(np.uint64), gc2.view(np.uint64)))
         grads[prec] = _rho_gradient(gp, gc, prim)
-        out["fp%d" % prec] = {"forward_seconds": tf, "backward_seconds": tb,
+        out["fp%d" % prec] = {"forward_seconds": tf, "backward_seconds": tb, "second_backward_seconds": tb2,
+                              "repeat_bitwise_equal": same,
                               "loss": float(np.sum(0.5 * (co[1] ** 2 + co[2] ** 2 + co[3] ** 2) / co[0]) * area),
                               "max_abs_grad": float(np.max(np.abs(grads[prec])))}
         print('bench: adjoint fp%d forward %.2f s backward %.2f s' % (prec, tf, tb), file=sys.stderr, flush=True)
@@ -254,6 +262,11 @@ def run_adjoint(n=256, steps=200, every=20, ncheck=32, check_steps=20):
     # flow with velocity (the configured state is a stationary entropy mode whose
     # gradient is zero up to rounding), over check_steps steps
     prim2, cons2, faces2 = adjoint_state(n, velocity=0.2)
+    run32 = AdjointRun((n, n), (h, h), (1, 1), GAMMA, order=1, precision=32, checkpoint_every=every)
+    _, co32, _ = run32.forward(prim2, cons2, faces2, dt, check_steps)
+    gp, gc, _ = run32.backward(None, _ke_cotangent(co32, area), None)
+    run32.close()
+    g32 = _rho_gradient(gp, gc, prim2)
     run = AdjointRun((n, n), (h, h), (1, 1), GAMMA, order=1, precision=64, checkpoint_every=every)
     _, co, _ = run.forward(prim2, cons2, faces2, dt, check_steps)
     gp, gc, _ = run.backward(None, _ke_cotangent(co, area), None)
@@ -275,13 +288,18 @@ def run_adjoint(n=256, steps=200, every=20, ncheck=32, check_steps=20):
             _, cq, _ = run.forward(p3, c3, faces2, dt, check_steps)
             vals.append(float(np.sum(0.5 * (cq[1] ** 2 + cq[2] ** 2 + cq[3] ** 2) / cq[0]) * area))
         fd = (vals[0] - vals[1]) / 2e-4
-        errs.append((g[j, i], fd))
+        errs.append((g[j, i], fd, g32[j, i]))
     run.close()
     gmax = float(np.max(np.abs(g)))
     out["fd_check"] = {"state": "same rho, plus a smooth velocity field of rms 0.2", "precision": 64,
                        "steps": check_steps, "cells": ncheck, "eps": 1e-4, "max_abs_grad": gmax,
-                       "max_err_over_max_grad": max(abs(a - b) for a, b in errs) / gmax,
-                       "median_rel_err": float(np.median([abs(a - b) / max(abs(b), 1e-300) for a, b in errs]))}
+                       "max_err_over_max_grad": max(abs(a - b) for a, b, _ in errs) / gmax,
+                       "median_rel_err": float(np.median([abs(a - b) / max(abs(b), 1e-300) for a, b, _ in errs])),
+                       "fp32_forward": {
+                           "max_abs_diff_vs_fp64_over_max_grad": float(np.max(np.abs(g32 - g))) / gmax,
+                           "max_err_vs_fd_over_max_grad": max(abs(c - b) for _, b, c in errs) / gmax,
+                           "median_rel_err_vs_fd": float(np.median([abs(c - b) / max(abs(b), 1e-300)
+                                                                   for _, b, c in errs]))}}
     return out
 
 
