@@ -87,7 +87,8 @@ def build_gpu(only=None):
 def build_host():
     """The host test library, and beside it the same library with every adjoint
     phase through the per-slot dual-number passes (libpmhd_host_dualc.so, the
-    reference the corrector's hand-written reverse is tested against)."""
+    reference the hand-written reverses of the corrector and the update are tested
+    against)."""
     os.makedirs(HOST_OBJ, exist_ok=True)
     inc = ['-I' + INC, '-I' + CSRC]
     cmds, objs = [], []

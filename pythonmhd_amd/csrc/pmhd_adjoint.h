@@ -370,9 +370,15 @@ struct is_correct<Correct<D, S>> : std::true_type {};
 // the hand-written reverse of the CTU correction (ctu_vjp) distributes g to the
 // inputs.  15 passes through the Roe solver and one reverse sweep, instead of one
 // pass through the whole body per input slot (about 55).
-template <class Ph, class V, class Add>
-PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, AdState& st, Add add) {
-  (void)st;
+// One face may be split over `lanes` threads (lanes of one wave): lane L forms g_q
+// for q = L, L + lanes, ..., share(g) sums the lanes' g (zero where a lane formed
+// none, so the sums are exact), and lane 0 runs the reverse of the CTU correction.
+struct NoShare {
+  PMHD_FI void operator()(double*) const {}
+};
+template <class Ph, class V, class Add, class Share = NoShare>
+PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, Add add, int lane = 0, int lanes = 1,
+                             Share share = Share()) {
   constexpr int Dir = Ph::dir;
   using D1 = ADual<V>;
   AdHot hot{-1, -1, 0.0};
@@ -389,7 +395,9 @@ PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, AdStat
   for (int v = 0; v < 7; ++v) gF[v] = a.Fc[Dir][v * NF + idx];
   // stage 1: the flux's sensitivities to the 15 intermediate values
   double g[15];
-  for (int q = 0; q < 15; ++q) {
+#pragma unroll
+  for (int q = 0; q < 15; ++q) g[q] = 0.0;
+  for (int q = lane; q < 15; q += lanes) {
     D1 y[15];
 #pragma unroll
     for (int k = 0; k < 15; ++k) y[k] = D1(x[k].v, k == q ? V(1) : V(0), 0);
@@ -400,6 +408,8 @@ PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, AdStat
     for (int v = 0; v < 7; ++v) acc += gF[v] * double(F[v].t);
     g[q] = acc;
   }
+  share(g);
+  if (lane != 0) return true;
   // stage 2: the reverse of the CTU correction, and the normal field's share
   Ctx<V> cv = c;
   cv.fcnt = nullptr;
@@ -407,6 +417,125 @@ PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, AdStat
   add(a.nfb[Dir] + idx, g[14]);
 #pragma unroll
   for (int v = 0; v < 7; ++v) a.Fc[Dir][v * NF + idx] = 0.0;
+  return true;
+}
+
+template <class Ph>
+struct is_update : std::false_type {};
+template <>
+struct is_update<Update> : std::true_type {};
+
+// Adjoint of one cell of the MHD update (k_update, 2D and 3D; the adjoint's steps
+// store no cell E in it): the new conserved state is linear in the old one, the
+// corrector fluxes and the face field, and the new primitive state is cons->prim
+// of it.  The reverse is written out: the cotangents of U2 and W2 go back through
+// cons->prim to the new state, then through the flux differences and face averages.
+template <class V, class Add>
+PMHD_FI bool adjoint_update(const Ctx<V>& c, const Ctx<double>& a, I3 p, Add add) {
+  auto put = [&](const double* arr, int64_t i, double v) { add(const_cast<double*>(arr) + i, v); };
+  const int64_t N = c.ncell();
+  const int64_t idx = c.cid(p.j, p.i, p.k);
+  const bool d3 = c.ndim == 3;
+  const int dv[7] = {0, 1, 2, 3, 5, 6, 7};
+  const int64_t NX = c.nface(0), NY = c.nface(1), NZ = d3 ? c.nface(2) : 0;
+  const int64_t fx0 = c.fid(0, p.j, p.i, p.k), fx1 = c.fid(0, p.j, p.i + 1, p.k);
+  const int64_t fy0 = c.fid(1, p.j, p.i, p.k), fy1 = c.fid(1, p.j + 1, p.i, p.k);
+  const int64_t fz0 = d3 ? c.fid(2, p.j, p.i, p.k) : 0, fz1 = d3 ? c.fid(2, p.j, p.i, p.k + 1) : 0;
+  // the forward values (k_update's expressions)
+  V u[7];
+#pragma unroll
+  for (int v = 0; v < 7; ++v) u[v] = ld(c.U + dv[v] * N, idx);
+  V nbz = V(0);
+  if (!d3)
+    nbz = u[5] - c.dtd[0] * (ld(c.Fc[0] + 5 * NX, fx1) - ld(c.Fc[0] + 5 * NX, fx0))
+          - c.dtd[1] * (ld(c.Fc[1] + 4 * NY, fy1) - ld(c.Fc[1] + 4 * NY, fy0));
+#pragma unroll
+  for (int v = 0; v < 7; ++v) {
+    u[v] = u[v] - c.dtd[0] * (ld(c.Fc[0] + v * NX, fx1) - ld(c.Fc[0] + v * NX, fx0));
+    u[v] = u[v] - c.dtd[1] * (ld(c.Fc[1] + v * NY, fy1) - ld(c.Fc[1] + v * NY, fy0));
+    if (d3) u[v] = u[v] - c.dtd[2] * (ld(c.Fc[2] + v * NZ, fz1) - ld(c.Fc[2] + v * NZ, fz0));
+  }
+  V bx;
+  if (!d3) {
+    bx = (V(ld(c.fb2[0], fx0)) + V(ld(c.fb2[0], fx1))) / V(2);
+    u[4] = (V(ld(c.fb2[1], fy0)) + V(ld(c.fb2[1], fy1))) / V(2);
+    u[5] = nbz;
+  } else {
+    bx = V(0.5) * (V(ld(c.fb2[0], fx0)) + V(ld(c.fb2[0], fx1)));
+    u[4] = V(0.5) * (V(ld(c.fb2[1], fy0)) + V(ld(c.fb2[1], fy1)));
+    u[5] = V(0.5) * (V(ld(c.fb2[2], fz0)) + V(ld(c.fb2[2], fz1)));
+  }
+  const V invv = V(1) / u[0];
+  const double inv = double(invv);
+  const double ke = double(ke_sum(u, 0));
+  const double gm1 = double(c.gamma - V(1));
+  double x[7];
+#pragma unroll
+  for (int v = 0; v < 7; ++v) x[v] = double(u[v]);
+  const double b = double(bx);
+  // cotangents of the outputs U2 = (u0..u3, bx, u4, u5, u6), W2 = (w0..w3, bx, w4, w5, w6)
+  double gU[8], gW[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    gU[q] = a.U2[q * N + idx];
+    gW[q] = a.W2[q * N + idx];
+  }
+  double gu[7] = {gU[0] + gW[0], gU[1], gU[2], gU[3], gU[5] + gW[5], gU[6] + gW[6], gU[7]};
+  double gb = gU[4] + gW[4];
+  double ginv = 0.0;
+#pragma unroll
+  for (int k = 1; k <= 3; ++k) {
+    gu[k] += gW[k] * inv;
+    ginv += gW[k] * x[k];
+  }
+  // w6 = (gamma - 1) (u6 - ke inv / 2 - (bx^2 + u4^2 + u5^2) / 2), ke = u1^2 + u2^2 + u3^2
+  const double gp = gm1 * gW[7];
+  gu[6] += gp;
+  const double gke = -0.5 * inv * gp;
+  ginv += -0.5 * ke * gp;
+  gb += -b * gp;
+  gu[4] += -x[4] * gp;
+  gu[5] += -x[5] * gp;
+#pragma unroll
+  for (int k = 1; k <= 3; ++k) gu[k] += 2.0 * x[k] * gke;
+  gu[0] += -inv * inv * ginv;
+  // back through the update
+  const double hx = double(c.dtd[0]), hy = double(c.dtd[1]), hz = d3 ? double(c.dtd[2]) : 0.0;
+  const int lin[5] = {0, 1, 2, 3, 6};
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const int v = lin[q];
+    put(a.U, dv[v] * N + idx, gu[v]);
+    put(a.Fc[0], v * NX + fx1, -hx * gu[v]);
+    put(a.Fc[0], v * NX + fx0, hx * gu[v]);
+    put(a.Fc[1], v * NY + fy1, -hy * gu[v]);
+    put(a.Fc[1], v * NY + fy0, hy * gu[v]);
+    if (d3) {
+      put(a.Fc[2], v * NZ + fz1, -hz * gu[v]);
+      put(a.Fc[2], v * NZ + fz0, hz * gu[v]);
+    }
+  }
+  put(a.fb2[0], fx0, 0.5 * gb);
+  put(a.fb2[0], fx1, 0.5 * gb);
+  put(a.fb2[1], fy0, 0.5 * gu[4]);
+  put(a.fb2[1], fy1, 0.5 * gu[4]);
+  if (d3) {
+    put(a.fb2[2], fz0, 0.5 * gu[5]);
+    put(a.fb2[2], fz1, 0.5 * gu[5]);
+  } else {
+    // u5 = the reference's new B_z: old B_z minus the x flux of its slot 5 and the y
+    // flux of its slot 4
+    put(a.U, 6 * N + idx, gu[5]);
+    put(a.Fc[0], 5 * NX + fx1, -hx * gu[5]);
+    put(a.Fc[0], 5 * NX + fx0, hx * gu[5]);
+    put(a.Fc[1], 4 * NY + fy1, -hy * gu[5]);
+    put(a.Fc[1], 4 * NY + fy0, hy * gu[5]);
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    a.U2[q * N + idx] = 0.0;
+    a.W2[q * N + idx] = 0.0;
+  }
   return true;
 }
 
@@ -523,12 +652,14 @@ PMHD_FI void adjoint_generic_lanes(const Ctx<V>& c, const Ctx<double>& a, I3 p, 
 }
 
 // whether the element adjoint of Ph records slot lists (the generic path)
+// (PMHD_ADJ_DUAL_CORRECTOR: every phase through the per-slot dual passes, the
+// reference the hand-written reverses are tested against)
 template <class Ph>
 constexpr bool uses_slots() {
 #ifdef PMHD_ADJ_DUAL_CORRECTOR
   return true;
 #else
-  return !is_correct<Ph>::value;
+  return !is_correct<Ph>::value && !is_update<Ph>::value;
 #endif
 }
 
@@ -544,15 +675,12 @@ constexpr int kAdjMaxContrib = 160;
 template <class Ph, class V, class Add>
 PMHD_FI bool adjoint_element(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdState& st, Add add,
                              const SlotView* sv = nullptr) {
-#ifdef PMHD_ADJ_DUAL_CORRECTOR
-  // every phase through the per-slot dual passes (the reference the corrector's
-  // hand-written reverse is tested against)
-  constexpr bool reverse = false;
-#else
-  constexpr bool reverse = is_correct<Ph>::value;
-#endif
-  if constexpr (reverse) {
-    return adjoint_correct<Ph>(c, a, p, st, add);
+  if constexpr (!uses_slots<Ph>() && is_correct<Ph>::value) {
+    (void)st;
+    return adjoint_correct<Ph>(c, a, p, add);
+  } else if constexpr (!uses_slots<Ph>() && is_update<Ph>::value) {
+    (void)st;
+    return adjoint_update(c, a, p, add);
   } else {
     return adjoint_generic<Ph>(c, a, p, st, add, sv);
   }

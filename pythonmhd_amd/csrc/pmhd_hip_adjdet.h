@@ -110,6 +110,30 @@ __global__ void __launch_bounds__(64) k_adjoint_kept2(const Ctx<T> c, const Ctx<
   }
 }
 
+// corrector faces: two adjacent threads per face split the 15 flux passes, then
+// the face's first thread adds every contribution (in the one-thread order)
+struct PairShare {
+  __device__ void operator()(double* g) const {
+#pragma unroll
+    for (int q = 0; q < 15; ++q) g[q] += __shfl_xor(g[q], 1);
+  }
+};
+template <class Ph, class T>
+__global__ void __launch_bounds__(64) k_adjoint_corr2(const Ctx<T> c, const Ctx<double> a, int n1, int n2, int64_t E,
+                                                      DetSink s) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t e = t >> 1;
+  if (e < E) {
+    const int k = int(e % n2);
+    const int64_t r = e / n2;
+    const int i = int(r % n1);
+    const int j = int(r / n1);
+    int n = 0;
+    uint32_t h = 0x811c9dc5u;
+    adjoint_correct<Ph>(c, a, I3{j, i, k}, DevDetAdd{s, e, &n, &h}, int(t & 1), 2, PairShare());
+  }
+}
+
 __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     p[i] = v;
@@ -406,12 +430,26 @@ struct HipAdjBackend : HipBackend {
     while (bx < 64 && int64_t(bx) < n[2]) bx <<= 1;
     const unsigned by = 64 / bx;
     dim3 grid(unsigned((n[2] + bx - 1) / bx), unsigned((n[1] + by - 1) / by), unsigned(n[0]));
-    if (uses_slots<Ph>() && !fresh && !sfresh) {
-      ex.timed(K_ADJOINT, E, [&] {
-        hipLaunchKernelGGL((k_adjoint_kept2<Ph, T>), dim3(unsigned((2 * E + 63) / 64)), dim3(64), 0, ex.s, c, a,
-                           int(n[1]), int(n[2]), E, vals, sb);
-      });
-    } else {
+    bool launched = false;
+    if constexpr (is_correct<Ph>::value && !uses_slots<Ph>()) {
+      if (!fresh) {
+        ex.timed(K_ADJOINT, E, [&] {
+          hipLaunchKernelGGL((k_adjoint_corr2<Ph, T>), dim3(unsigned((2 * E + 63) / 64)), dim3(64), 0, ex.s, c, a,
+                             int(n[1]), int(n[2]), E, s);
+        });
+        launched = true;
+      }
+    }
+    if constexpr (uses_slots<Ph>()) {
+      if (!fresh && !sfresh) {
+        ex.timed(K_ADJOINT, E, [&] {
+          hipLaunchKernelGGL((k_adjoint_kept2<Ph, T>), dim3(unsigned((2 * E + 63) / 64)), dim3(64), 0, ex.s, c, a,
+                             int(n[1]), int(n[2]), E, vals, sb);
+        });
+        launched = true;
+      }
+    }
+    if (!launched) {
       ex.timed(K_ADJOINT, E, [&] {
         hipLaunchKernelGGL((k_adjoint_det<Ph, T>), grid, dim3(bx, by, 1), 0, ex.s, c, a, int(n[1]), int(n[2]), s, sb);
       });

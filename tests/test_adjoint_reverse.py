@@ -1,9 +1,10 @@
-"""The corrector adjoint's hand-written reverse of the CTU correction against the
-per-slot dual-number evaluation of the same face (CPU, host builds).
+"""The adjoint's hand-written reverses against the per-slot dual-number evaluation
+of the same elements (CPU, host builds).
 
 The host library evaluates each corrector face's adjoint as 15 dual passes through
 the flux followed by the hand-written reverse of the transverse correction
-(ctu_vjp in pythonmhd_amd/csrc/pmhd_adjoint.h); libpmhd_host_dualc.so is the same
+(ctu_vjp in pythonmhd_amd/csrc/pmhd_adjoint.h), and each cell of the update through
+its hand-written reverse (adjoint_update); libpmhd_host_dualc.so is the same
 library built with every phase through one dual pass per input slot (no hand-
 written derivative anywhere).  The two give the same gradient of a whole
 checkpointed run to FP64 rounding, on 2D and 3D grids, periodic, outflow and mixed
