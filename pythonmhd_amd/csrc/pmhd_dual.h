@@ -62,6 +62,36 @@ PMHD_FI ADual<V> fabs_(const ADual<V>& a) {
   return a.v < V(0) ? ADual<V>(-a.v, -a.t, 0) : ADual<V>(fabs_(a.v), a.t, 0);
 }
 
+// The eigen-system of a dual body.  When none of its inputs carries a tangent (a
+// seeded pass whose slot is not one of the cell's own values), the dual evaluation
+// would give the plain-value eigen-system with zero tangents: computed so, at the
+// plain cost.  Otherwise the generic dual evaluation.  (Not in the all-dual test
+// build, PMHD_ADJ_DUAL_CORRECTOR, which checks this against the dual evaluation.)
+#ifndef PMHD_ADJ_DUAL_CORRECTOR
+template <class V>
+PMHD_FI void eigen_prim(const ADual<V> w[7], ADual<V> bn, ADual<V> gamma, PrimEig<ADual<V>>& E) {
+  bool seeded = bn.t != V(0) || gamma.t != V(0);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) seeded = seeded || w[k].t != V(0);
+  if (seeded) {
+    eigen_prim<ADual<V>>(w, bn, gamma, E);
+    return;
+  }
+  V wv[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) wv[k] = w[k].v;
+  PrimEig<V> P;
+  eigen_prim<V>(wv, bn.v, gamma.v, P);
+  constexpr int kFields = sizeof(PrimEig<V>) / sizeof(V);
+  static_assert(sizeof(PrimEig<V>) == kFields * sizeof(V), "PrimEig holds only numbers");
+  static_assert(sizeof(PrimEig<ADual<V>>) == kFields * sizeof(ADual<V>), "PrimEig holds only numbers");
+  const V* src = reinterpret_cast<const V*>(&P);
+  ADual<V>* dst = reinterpret_cast<ADual<V>*>(&E);
+#pragma unroll
+  for (int k = 0; k < kFields; ++k) dst[k] = ADual<V>(src[k], V(0), 0);
+}
+#endif
+
 // the wave-speed islands of a dual body run on dual doubles
 template <class V>
 struct Island<ADual<V>> {

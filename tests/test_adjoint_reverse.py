@@ -4,9 +4,10 @@ of the same elements (CPU, host builds).
 The host library evaluates each corrector face's adjoint as 15 dual passes through
 the flux followed by the hand-written reverse of the transverse correction
 (ctu_vjp in pythonmhd_amd/csrc/pmhd_adjoint.h), and each cell of the update through
-its hand-written reverse (adjoint_update); libpmhd_host_dualc.so is the same
-library built with every phase through one dual pass per input slot (no hand-
-written derivative anywhere).  The two give the same gradient of a whole
+its hand-written reverse (adjoint_update), and forms the eigen-system of passes
+that seed none of its inputs on plain values (pythonmhd_amd/csrc/pmhd_dual.h);
+libpmhd_host_dualc.so is the same library built with every phase through one
+full dual pass per input slot (no hand-written derivative or shortcut anywhere).  The two give the same gradient of a whole
 checkpointed run to FP64 rounding, on 2D and 3D grids, periodic, outflow and mixed
 boundaries, PLM and first order, with and without the reference's boundary quirks,
 and with the density and energy floors acting on part of the grid (the reverse
