@@ -492,12 +492,13 @@ PMHD_FI void cell_edge_states(const Ctx<T>& c, int d, int bc, I3 p, bool from_s,
 }
 
 // edges and characteristic tracing of each cell; writes the two faces it owns
+// (from_s false: PLM with the cell's own limited slope formed here)
 template <class T>
-PMHD_FI void k_edges(const Ctx<T>& c, int d, int bc, I3 p) {
+PMHD_FI void k_edges(const Ctx<T>& c, int d, int bc, I3 p, bool from_s = true) {
   const int64_t n = c.n_along(d);
   const int64_t e = co(p, d);
   T dla[7], dra[7];
-  cell_edge_states(c, d, bc, p, true, dla, dra);
+  cell_edge_states(c, d, bc, p, from_s, dla, dra);
   const int64_t NF = c.nface(d);
   I3 f0 = with(p, d, e);
   I3 f1 = with(p, d, e + 1);
