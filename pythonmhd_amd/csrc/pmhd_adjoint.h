@@ -420,6 +420,13 @@ PMHD_FI bool adjoint_correct(const Ctx<V>& c, const Ctx<double>& a, I3 f, Add ad
   return true;
 }
 
+// the one-launch PLM slope and edge body: its seeded passes need every VGPR a wave
+// has, and the two-lane form spills, so it keeps one thread per cell
+template <class Ph>
+struct is_edges_own : std::false_type {};
+template <int D>
+struct is_edges_own<EdgesOwn<D>> : std::true_type {};
+
 template <class Ph>
 struct is_update : std::false_type {};
 template <>

@@ -440,7 +440,7 @@ struct HipAdjBackend : HipBackend {
         launched = true;
       }
     }
-    if constexpr (uses_slots<Ph>()) {
+    if constexpr (uses_slots<Ph>() && !is_edges_own<Ph>::value) {
       if (!fresh && !sfresh) {
         ex.timed(K_ADJOINT, E, [&] {
           hipLaunchKernelGGL((k_adjoint_kept2<Ph, T>), dim3(unsigned((2 * E + 63) / 64)), dim3(64), 0, ex.s, c, a,
