@@ -328,13 +328,14 @@ int pmhd_adjoint_backward_device(pmhd_adjoint* a, const double* gprim, const dou
 int pmhd_adjoint_set_stream(pmhd_adjoint* a, void* stream);
 /* Deterministic accumulation (enable != 0, the default): every backward() of the
  * same forward gives the same bits.  Each launch's contributions are written to
- * their own places and summed per target in a fixed order; the first backward
+ * their own places and summed per target in a fixed order.  The first backward
  * builds that order once per launch of a step (a sort of the targets, with about
- * 4 KB of scratch per element of the largest launch, released when it ends) and
- * keeps it with the handle, beside a buffer of 1.3 KB per element for the
- * contributions.  Up to 13.4M elements per launch.  enable = 0 adds with FP64 atomics instead (summation order, and so the
- * last bits, vary between runs).  No counterpart in the reference, which has no
- * adjoint. */
+ * 5.5 KB of scratch per element of the largest launch, released when it ends) and
+ * keeps it with the handle together with the launch's slot lists (about 0.5 KB per
+ * element and launch of a step), beside a buffer of 1.3 KB per element for the
+ * contributions.  Up to 13.4M elements per launch.  enable = 0 adds with FP64
+ * atomics instead (summation order, and so the last bits, vary between runs).  No
+ * counterpart in the reference, which has no adjoint. */
 int pmhd_adjoint_set_deterministic(pmhd_adjoint* a, int32_t enable);
 int pmhd_adjoint_set_profiling(pmhd_adjoint* a, int32_t enable);
 int pmhd_adjoint_get_profile(pmhd_adjoint* a, int32_t max_entries, char* names, double* total_ms,

@@ -8,14 +8,17 @@
 //   of the segment, last first, recomputes that step's intermediates into buffers
 //   that are never reused within a step, and runs the adjoint of every phase in
 //   reverse order.  A phase's adjoint evaluates the phase's own kernel body on dual
-//   numbers (pmhd_dual.h): per output element, one recording pass (plain values)
-//   finds the input slots, one seeded pass per slot adds that slot's share of the
-//   vector-Jacobian product to the slot's cotangent (FP64 atomics), and the
-//   element's output cotangents are then cleared (the values they belong to are
-//   dead before the phase).  The corrector faces are differentiated in two stages
-//   (adjoint_correct): through the Riemann flux per intermediate value, then through
-//   the CTU correction per input slot.  Returns the cotangent of the initial state
-//   (prim, cons, faces).
+//   numbers (pmhd_dual.h): per output element, a recording pass (plain values)
+//   finds the input slots (the HIP backend keeps these lists after a launch's first
+//   evaluation), one seeded pass per slot forms that slot's share of the
+//   vector-Jacobian product, and the element's output cotangents are then cleared
+//   (the values they belong to are dead before the phase).  The corrector faces are
+//   differentiated in two stages (adjoint_correct): through the Riemann flux per
+//   intermediate value, then through the hand-written reverse of the CTU correction;
+//   the update cells through a hand-written reverse (adjoint_update).  The shares
+//   reach the cotangent arrays through the backend's accumulation (a fixed-order
+//   sum per target on the GPU, pmhd_hip_adjdet.h, or FP64 atomics).  Returns the
+//   cotangent of the initial state (prim, cons, faces).
 //
 // The loss's dependence on dt is not differentiated: dt is a fixed input.
 #pragma once
