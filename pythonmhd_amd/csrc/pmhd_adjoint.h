@@ -430,6 +430,211 @@ struct is_edges_own : std::false_type {};
 template <int D>
 struct is_edges_own<EdgesOwn<D>> : std::true_type {};
 
+// ---------------------------------------------------------------------------
+// Predictor faces in two stages (as the corrector): the face's outputs (U_L, U_R
+// and the flux) depend on its input slots only through the prepared primitive
+// states w_L, w_R (floors and reconstruction source terms applied) and the normal
+// field (15 values).  15 dual passes through prim->cons and the Roe flux give the
+// outputs' sensitivities g; the hand-written reverse of the preparation
+// (predict_prep_vjp) distributes them to the inputs.
+// ---------------------------------------------------------------------------
+template <class Ph>
+struct is_predict : std::false_type {};
+template <int D, int S>
+struct is_predict<Predict<D, S>> : std::true_type {};
+
+// minmod_recon(dn, dt, zero_fix)'s choice: 1 dn, -1 minus dt, 0 zero
+template <class V>
+PMHD_FI int minmod_recon_pick(V dn, V dt_, bool zero_fix) {
+  bool took_t;
+  V t;
+  if (dn >= V(0)) {
+    took_t = !(dn < -dt_);
+    t = took_t ? -dt_ : dn;
+  } else {
+    took_t = !(dn > -dt_);
+    t = took_t ? -dt_ : dn;
+  }
+  if (zero_fix && took_t && dt_ == V(0)) return 0;
+  if (dn >= V(0) && t < V(0)) return 0;
+  if (!(dn >= V(0)) && t > V(0)) return 0;
+  return took_t ? -1 : 1;
+}
+
+// reverse of the source term of the reconstruction on cell m of the line of faces
+// through f along d (cell_recon_src): go[0], go[1] the cotangents of its terms
+template <class V, class Put>
+PMHD_FI void recon_src_vjp(const Ctx<V>& c, const Ctx<double>& a, int d, I3 f, int64_t m, const double go[2],
+                           Put put) {
+  const int64_t N = c.ncell();
+  const double hdt = double(c.hdt);
+  if (c.ndim == 2) {
+    const int64_t n = c.n_along(d);
+    const int bc = c.bc[d];
+    const I3 pm = with(f, d, map1(m, n, bc));
+    const int vslot = d == 0 ? 2 : 1;
+    const int64_t cm = c.cid(pm.j, pm.i, pm.k);
+    const V v = ld(c.W + vslot * N, cm);
+    const int64_t flo = m < 0 ? (bc == OUTFLOW ? 0 : n - 1) : m;
+    const int64_t fhi = m + 1 > n ? (bc == OUTFLOW ? n : 1) : m + 1;
+    const double dd = double((face_val(c, c.fb[d], d, f, fhi) - face_val(c, c.fb[d], d, f, flo)) * c.invd[d]);
+    // out0 = hdt v ((fb[fhi] - fb[flo]) invd)
+    put(a.W, vslot * N + cm, hdt * dd * go[0]);
+    const double gd = hdt * double(v) * double(c.invd[d]) * go[0];
+    const I3 qh = with(f, d, fhi), ql = with(f, d, flo);
+    put(a.fb[d], c.fid(d, qh.j, qh.i, qh.k), gd);
+    put(a.fb[d], c.fid(d, ql.j, ql.i, ql.k), -gd);
+    return;
+  }
+  // 3D (recon_b_term_3d)
+  const int bc = d == 2 ? c.bc_zrec : c.bc[d];
+  const int64_t n = c.n_along(d);
+  const int t1 = (d + 1) % 3, t2 = (d + 2) % 3;
+  const I3 pm = with(f, d, map1(m, n, bc));
+  const int64_t flo = m < 0 ? (bc == OUTFLOW ? 0 : n - 1) : m;
+  const int64_t fhi = m + 1 > n ? (bc == OUTFLOW ? n : 1) : m + 1;
+  const V dbn = (face_val(c, c.fb[d], d, f, fhi) - face_val(c, c.fb[d], d, f, flo)) * c.invd[d];
+  const V db1 = (face_val(c, c.fb[t1], t1, pm, co(pm, t1) + 1) - face_val(c, c.fb[t1], t1, pm, co(pm, t1))) *
+                c.invd[t1];
+  const V db2 = (face_val(c, c.fb[t2], t2, pm, co(pm, t2) + 1) - face_val(c, c.fb[t2], t2, pm, co(pm, t2))) *
+                c.invd[t2];
+  const int p1 = minmod_recon_pick(dbn, db1, d == 1), p2 = minmod_recon_pick(dbn, db2, false);
+  const double term1 = p1 == 1 ? double(dbn) : (p1 == -1 ? -double(db1) : 0.0);
+  const double term2 = p2 == 1 ? double(dbn) : (p2 == -1 ? -double(db2) : 0.0);
+  const int64_t cm = c.cid(pm.j, pm.i, pm.k);
+  const V rho = ld(c.W, cm);
+  const V v1 = ld(c.U + (1 + t1) * N, cm) / rho;
+  const V v2 = ld(c.U + (1 + t2) * N, cm) / rho;
+  const double r = double(rho), w1 = double(v1), w2 = double(v2);
+  // out0 = hdt (v1 term1), out1 = hdt (v2 term2), v_k = M_k / rho
+  const double gv1 = hdt * term1 * go[0], gv2 = hdt * term2 * go[1];
+  const double gt1 = hdt * w1 * go[0], gt2 = hdt * w2 * go[1];
+  put(a.U, (1 + t1) * N + cm, gv1 / r);
+  put(a.U, (1 + t2) * N + cm, gv2 / r);
+  put(a.W, cm, -(gv1 * w1 + gv2 * w2) / r);
+  double gdbn = 0.0, gdb1 = 0.0, gdb2 = 0.0;
+  if (p1 == 1) gdbn += gt1; else if (p1 == -1) gdb1 -= gt1;
+  if (p2 == 1) gdbn += gt2; else if (p2 == -1) gdb2 -= gt2;
+  const double gn = gdbn * double(c.invd[d]), g1 = gdb1 * double(c.invd[t1]), g2 = gdb2 * double(c.invd[t2]);
+  const I3 qh = with(f, d, fhi), ql = with(f, d, flo);
+  put(a.fb[d], c.fid(d, qh.j, qh.i, qh.k), gn);
+  put(a.fb[d], c.fid(d, ql.j, ql.i, ql.k), -gn);
+  const I3 p1h = with(pm, t1, co(pm, t1) + 1), p2h = with(pm, t2, co(pm, t2) + 1);
+  put(a.fb[t1], c.fid(t1, p1h.j, p1h.i, p1h.k), g1);
+  put(a.fb[t1], c.fid(t1, pm.j, pm.i, pm.k), -g1);
+  put(a.fb[t2], c.fid(t2, p2h.j, p2h.i, p2h.k), g2);
+  put(a.fb[t2], c.fid(t2, pm.j, pm.i, pm.k), -g2);
+}
+
+// (MHD in 2D and 3D: the adjoint takes no 1D MHD run)
+template <class Ph, class V, class Add, class Share = NoShare>
+PMHD_FI bool adjoint_predict(const Ctx<V>& c, const Ctx<double>& a, I3 f, Add add, int lane = 0, int lanes = 1,
+                             Share share = Share()) {
+  constexpr int Dir = Ph::dir;
+  using D1 = ADual<V>;
+  auto put = [&](const double* arr, int64_t i, double v) { add(const_cast<double*>(arr) + i, v); };
+  const int64_t NF = c.nface(Dir);
+  const int64_t idx = c.fid(Dir, f.j, f.i, f.k);
+  const int64_t e = co(f, Dir);
+  // the prepared states (predict_face_src's values)
+  V wl[7], wr[7];
+  load7(c.WL, NF, idx, wl);
+  load7(c.WR, NF, idx, wr);
+  const bool fl0 = wl[0] < c.min_dens, fl6 = wl[6] < c.min_pres;
+  const bool fr0 = wr[0] < c.min_dens, fr6 = wr[6] < c.min_pres;
+  apply_floors(wl, c.min_dens, c.min_pres);
+  apply_floors(wr, c.min_dens, c.min_pres);
+  Ctx<V> cv = c;
+  cv.fcnt = nullptr;
+  V cl[2], cr[2];
+  cell_recon_src(cv, Dir, f, e - 1, cl);
+  cell_recon_src(cv, Dir, f, e, cr);
+  int ke = 0;
+  if (c.ndim == 3) {
+    wl[4] = wl[4] + cl[0];
+    wl[5] = wl[5] + cl[1];
+    wr[4] = wr[4] + cr[0];
+    wr[5] = wr[5] + cr[1];
+    ke = Dir;
+  } else {
+    const int slot = Dir == 0 ? 4 : 5;
+    wl[slot] = wl[slot] + cl[0];
+    wr[slot] = wr[slot] + cr[0];
+  }
+  const V bn = V(ld(c.fb[Dir], idx));
+  // cotangents of the outputs U_L, U_R, F
+  double gUL[7], gUR[7], gF[7];
+#pragma unroll
+  for (int v = 0; v < 7; ++v) {
+    gUL[v] = a.UL[Dir][v * NF + idx];
+    gUR[v] = a.UR[Dir][v * NF + idx];
+    gF[v] = a.F[Dir][v * NF + idx];
+  }
+  AdHot hot{-1, -1, 0.0};
+  Ctx<D1> d;
+  make_ctx_of<D1>(c, a, [&](const V* val, double* adj, int id) { return AdPtr<V>{val, adj, &hot, 0, id}; }, d);
+  // stage 1: the outputs' sensitivities to the 15 prepared values
+  double g[15];
+#pragma unroll
+  for (int q = 0; q < 15; ++q) g[q] = 0.0;
+  for (int q = lane; q < 15; q += lanes) {
+    D1 y[15];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      y[k] = D1(wl[k], k == q ? V(1) : V(0), 0);
+      y[7 + k] = D1(wr[k], 7 + k == q ? V(1) : V(0), 0);
+    }
+    y[14] = D1(bn, q == 14 ? V(1) : V(0), 0);
+    D1 ul[7], ur[7], F[7];
+    predict_flux<Ph::solver>(d, Dir, ke, y, y + 7, y[14], ul, ur, F);
+    double acc = 0.0;
+#pragma unroll
+    for (int v = 0; v < 7; ++v) acc += gUL[v] * double(ul[v].t) + gUR[v] * double(ur[v].t) + gF[v] * double(F[v].t);
+    g[q] = acc;
+  }
+  share(g);
+  if (lane != 0) return true;
+  // stage 2: the reverse of the preparation
+#pragma unroll
+  for (int v = 0; v < 7; ++v) {
+    put(a.WL, v * NF + idx, ((v == 0 && fl0) || (v == 6 && fl6)) ? 0.0 : g[v]);
+    put(a.WR, v * NF + idx, ((v == 0 && fr0) || (v == 6 && fr6)) ? 0.0 : g[7 + v]);
+  }
+  put(a.fb[Dir], idx, g[14]);
+  double gl[2], gr[2];
+  if (c.ndim == 3) {
+    gl[0] = g[4];
+    gl[1] = g[5];
+    gr[0] = g[11];
+    gr[1] = g[12];
+  } else {
+    const int slot = Dir == 0 ? 4 : 5;
+    gl[0] = g[slot];
+    gr[0] = g[7 + slot];
+    gl[1] = gr[1] = 0.0;
+  }
+  recon_src_vjp(cv, a, Dir, f, e - 1, gl, put);
+  recon_src_vjp(cv, a, Dir, f, e, gr, put);
+#pragma unroll
+  for (int v = 0; v < 7; ++v) {
+    a.UL[Dir][v * NF + idx] = 0.0;
+    a.UR[Dir][v * NF + idx] = 0.0;
+    a.F[Dir][v * NF + idx] = 0.0;
+  }
+  return true;
+}
+
+// the two-stage element adjoints (corrector and predictor faces), for a kernel that
+// splits their first stage over lanes
+template <class Ph, class V, class Add, class Share>
+PMHD_FI bool adjoint_staged(const Ctx<V>& c, const Ctx<double>& a, I3 f, Add add, int lane, int lanes, Share share) {
+  if constexpr (is_predict<Ph>::value) {
+    return adjoint_predict<Ph>(c, a, f, add, lane, lanes, share);
+  } else {
+    return adjoint_correct<Ph>(c, a, f, add, lane, lanes, share);
+  }
+}
+
 template <class Ph>
 struct is_update : std::false_type {};
 template <>
@@ -669,7 +874,7 @@ constexpr bool uses_slots() {
 #ifdef PMHD_ADJ_DUAL_CORRECTOR
   return true;
 #else
-  return !is_correct<Ph>::value && !is_update<Ph>::value;
+  return !is_correct<Ph>::value && !is_update<Ph>::value && !is_predict<Ph>::value;
 #endif
 }
 
@@ -688,6 +893,9 @@ PMHD_FI bool adjoint_element(const Ctx<V>& c, const Ctx<double>& a, I3 p, AdStat
   if constexpr (!uses_slots<Ph>() && is_correct<Ph>::value) {
     (void)st;
     return adjoint_correct<Ph>(c, a, p, add);
+  } else if constexpr (!uses_slots<Ph>() && is_predict<Ph>::value) {
+    (void)st;
+    return adjoint_predict<Ph>(c, a, p, add);
   } else if constexpr (!uses_slots<Ph>() && is_update<Ph>::value) {
     (void)st;
     return adjoint_update(c, a, p, add);

@@ -110,8 +110,9 @@ __global__ void __launch_bounds__(64) k_adjoint_kept2(const Ctx<T> c, const Ctx<
   }
 }
 
-// corrector faces: two adjacent threads per face split the 15 flux passes, then
-// the face's first thread adds every contribution (in the one-thread order)
+// corrector and predictor faces: two adjacent threads per face split the 15 flux
+// passes, then the face's first thread adds every contribution (in the one-thread
+// order)
 struct PairShare {
   __device__ void operator()(double* g) const {
 #pragma unroll
@@ -130,7 +131,7 @@ __global__ void __launch_bounds__(64) k_adjoint_corr2(const Ctx<T> c, const Ctx<
     const int j = int(r / n1);
     int n = 0;
     uint32_t h = 0x811c9dc5u;
-    adjoint_correct<Ph>(c, a, I3{j, i, k}, DevDetAdd{s, e, &n, &h}, int(t & 1), 2, PairShare());
+    adjoint_staged<Ph>(c, a, I3{j, i, k}, DevDetAdd{s, e, &n, &h}, int(t & 1), 2, PairShare());
   }
 }
 
@@ -431,7 +432,7 @@ struct HipAdjBackend : HipBackend {
     const unsigned by = 64 / bx;
     dim3 grid(unsigned((n[2] + bx - 1) / bx), unsigned((n[1] + by - 1) / by), unsigned(n[0]));
     bool launched = false;
-    if constexpr (is_correct<Ph>::value && !uses_slots<Ph>()) {
+    if constexpr ((is_correct<Ph>::value || is_predict<Ph>::value) && !uses_slots<Ph>()) {
       if (!fresh) {
         ex.timed(K_ADJOINT, E, [&] {
           hipLaunchKernelGGL((k_adjoint_corr2<Ph, T>), dim3(unsigned((2 * E + 63) / 64)), dim3(64), 0, ex.s, c, a,

@@ -603,6 +603,23 @@ PMHD_FI void cell_recon_src(const Ctx<T>& c, int d, I3 f, int64_t m, T out[2]) {
   }
 }
 
+// the predictor's conserved states and flux of a face of direction d from its
+// prepared primitive states (floors and source terms applied) and normal field
+// (the context supplies gamma and the entropy fix; its number type may differ)
+template <int S, class T, class U>
+PMHD_FI void predict_flux(const Ctx<U>& c, int d, int ke, const T wl[7], const T wr[7], T bn, T ul[7], T ur[7],
+                          T df[7]) {
+  prim_to_cons(wl, bn, T(c.gamma), ke, ul);
+  prim_to_cons(wr, bn, T(c.gamma), ke, ur);
+  T rwl[7], rwr[7], rul[7], rur[7], rf[7];
+  to_rot(wl, d, rwl);
+  to_rot(wr, d, rwr);
+  to_rot(ul, d, rul);
+  to_rot(ur, d, rur);
+  roe_flux<S>(rwl, rwr, rul, rur, bn, T(c.gamma), c.efix != 0, T(c.eps), rf);
+  from_rot(rf, d, df);
+}
+
 // predictor of face f from its reconstructed states wl, wr (directional order;
 // modified in place) and the source terms of its left and right cells
 // (cell_recon_src of cells e-1 and e)
@@ -629,18 +646,10 @@ PMHD_FI void predict_face_src(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7], co
   } else {
     bn = c.bx1d;
   }
-  T ul[7], ur[7];
-  prim_to_cons(wl, bn, c.gamma, ke, ul);
-  prim_to_cons(wr, bn, c.gamma, ke, ur);
+  T ul[7], ur[7], df[7];
+  predict_flux<S>(c, d, ke, wl, wr, bn, ul, ur, df);
   store7(c, c.UL[d], NF, idx, ul);
   store7(c, c.UR[d], NF, idx, ur);
-  T rwl[7], rwr[7], rul[7], rur[7], rf[7], df[7];
-  to_rot(wl, d, rwl);
-  to_rot(wr, d, rwr);
-  to_rot(ul, d, rul);
-  to_rot(ur, d, rur);
-  roe_flux<S>(rwl, rwr, rul, rur, bn, c.gamma, c.efix != 0, c.eps, rf);
-  from_rot(rf, d, df);
   store7(c, c.F[d], NF, idx, df);
 }
 
