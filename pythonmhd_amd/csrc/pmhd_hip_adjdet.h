@@ -14,12 +14,18 @@
 // the first time, the launch also writes each place's target index (the cotangent
 // arrays are one block, so a target is an index into it); a stable radix sort of
 // (target, place), a run-length encoding and a scan give the targets, the start of
-// each target's run and the places in ascending order.  Later launches check that
-// each element still makes as many contributions, to targets with the same hash.
+// each target's run and the places in ascending order.  Later one-thread launches
+// check that each element still makes as many contributions, to targets with the
+// same hash.
 //
 // The generic element adjoints also keep their recording pass's slot lists (the
 // input slots to seed and the outputs whose cotangents they consume) with the
 // launch, by array id and index, so later evaluations skip the recording pass.
+// Later evaluations then split an element's seeded passes over two lanes of one
+// wave (k_adjoint_kept2: the share of slot k goes to place k, so the places are the
+// one-thread ones by construction), and the predictor and corrector faces split
+// their 15 flux passes likewise (k_adjoint_corr2: the face's first lane then adds
+// every contribution in the one-thread order).
 #pragma once
 #include <hipcub/hipcub.hpp>
 
