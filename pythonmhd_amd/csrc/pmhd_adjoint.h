@@ -4,10 +4,11 @@
 // forward(state0, dt, nsteps): runs the steps, keeping the state at every
 //   checkpoint_every-th step.
 // backward(cotangent of the final state): for each checkpoint segment, last
-//   first, recomputes the segment's states from its checkpoint; then for each step
-//   of the segment, last first, recomputes that step's intermediates into buffers
-//   that are never reused within a step, and runs the adjoint of every phase in
-//   reverse order.  A phase's adjoint evaluates the phase's own kernel body on dual
+//   first, runs the segment's steps forward from its checkpoint, recording each
+//   step's intermediates into value buffers of its own (or, beyond a memory budget,
+//   replays the states and recomputes each step's intermediates when its turn
+//   comes); then for each step of the segment, last first, runs the adjoint of
+//   every phase in reverse order.  A phase's adjoint evaluates the phase's own kernel body on dual
 //   numbers (pmhd_dual.h): per output element, a recording pass (plain values)
 //   finds the input slots (the HIP backend keeps these lists after a launch's first
 //   evaluation), one seeded pass per slot forms that slot's share of the
@@ -1045,6 +1046,11 @@ struct AdjointHandle : public pmhd_adjoint {
   std::vector<Snap> ckpt;
   std::vector<Snap> seg;
   StepBufs<T> vals;
+  // value buffers of the further steps of a segment, when they fit the budget below
+  // (then one forward pass over a segment records every step's tape)
+  std::vector<StepBufs<T>> vals_more;
+  bool vals_more_failed = false;
+  static constexpr int64_t kKeepBytes = int64_t(4) << 30;
   StepBufs<double> adjs;
   double* adj_state[2][5] = {};   // cotangents of (U, W, fb0, fb1, fb2), two sets
   double dt = 0.0;
@@ -1190,6 +1196,55 @@ struct AdjointHandle : public pmhd_adjoint {
     return e ? fail(PMHD_EHIP, e) : PMHD_OK;
   }
 
+  // the cotangent set a step's inputs use: the last step (nsteps - 1) takes set 1,
+  // and the sets alternate from there
+  int in_parity(int64_t s) const { return 1 - int((nsteps - 1 - s) % 2); }
+
+  // runs step cs forward into the value buffers vb while recording its adjoint tape
+  // (the tape's cotangent pointers: the state set `in`)
+  void record_step(TapeExec<T, B>& tx, const Ctx<T>& cs, const StepBufs<T>& vb, int in) {
+    tx.be = &be;
+    tx.vals = &vb;
+    tx.adjs = &adjs;
+    std::memset(&tx.abase, 0, sizeof(tx.abase));
+    tx.abase.U = adj_state[in][0];
+    tx.abase.W = adj_state[in][1];
+    for (int d = 0; d < 3; ++d) tx.abase.fb[d] = adj_state[in][2 + d];
+    run_cycle(tx, cs);
+  }
+
+  // value buffers of the i-th step of a segment
+  const StepBufs<T>& step_vals(int64_t i) const { return i == 0 ? vals : vals_more[size_t(i - 1)]; }
+
+  // whether every step of a segment of len steps can have its own value buffers
+  // (within kKeepBytes; allocated on first use)
+  bool keep_values(int64_t len) {
+    if (len <= 1) return true;
+    if (vals_more_failed) return false;
+    if (int64_t(vals_more.size()) >= len - 1) return true;
+    int64_t words = 0;
+    StepBufs<T> probe;
+    alloc_step(c, probe, [&](int64_t n) {
+      words += n > 0 ? n : 1;
+      return static_cast<T*>(nullptr);
+    });
+    if (words * int64_t(sizeof(T)) * len > kKeepBytes) {
+      vals_more_failed = true;
+      return false;
+    }
+    try {
+      while (int64_t(vals_more.size()) < len - 1) {
+        StepBufs<T> b;
+        alloc_step(c, b, [&](int64_t n) { return dalloc(n); });
+        vals_more.push_back(b);
+      }
+    } catch (const std::bad_alloc&) {
+      vals_more_failed = true;
+      return false;
+    }
+    return true;
+  }
+
   int backward(const double* gprim, const double* gcons, const double* const gf[3], double* gprim0,
                double* gcons0, double* const gf0[3], bool dev) override {
     if (!have_forward) return fail(PMHD_EINVAL, "backward needs a preceding forward");
@@ -1203,53 +1258,74 @@ struct AdjointHandle : public pmhd_adjoint {
     for (int64_t sgi = nck - 1; sgi >= 0; --sgi) {
       const int64_t s0 = sgi * every;
       const int64_t s1 = std::min<int64_t>(s0 + every, nsteps);
-      // states of the segment, from its checkpoint
       const Snap& k = ckpt[size_t(sgi)];
-      cur = 0;
-      copy_snap(sb.U[0], sb.W[0], sb.FB[0], k.U, k.W, k.fb);
-      for (int64_t s = s0; s < s1; ++s) {
-        const Snap& q = seg[size_t(s - s0)];
-        copy_snap(q.U, q.W, q.fb, sb.U[cur], sb.W[cur], sb.FB[cur]);
-        if (s + 1 < s1) step_forward();
+      const bool keep = keep_values(s1 - s0);
+      std::vector<TapeExec<T, B>> txs(size_t(keep ? s1 - s0 : 0));
+      if (keep) {
+        // one forward pass over the segment records every step's tape, each step with
+        // its own value buffers; step s reads the outputs of step s - 1 where they are
+        const Snap& q = seg[0];
+        copy_snap(q.U, q.W, q.fb, k.U, k.W, k.fb);
+        for (int64_t s = s0; s < s1; ++s) {
+          const int64_t i = s - s0;
+          Ctx<T> cs = c;
+          if (i == 0) {
+            cs.U = q.U;
+            cs.W = q.W;
+            for (int d = 0; d < 3; ++d) cs.fb[d] = q.fb[d];
+          } else {
+            const StepBufs<T>& pv = step_vals(i - 1);
+            cs.U = pv.U2;
+            cs.W = pv.W2;
+            for (int d = 0; d < 3; ++d) cs.fb[d] = pv.fb2[d];
+          }
+          record_step(txs[size_t(i)], cs, step_vals(i), in_parity(s));
+          if (txs[size_t(i)].fused) return fail(PMHD_EINVAL, "the adjoint needs separate predictor launches (seg = 0)");
+        }
+      } else {
+        // the states of the segment, from its checkpoint
+        cur = 0;
+        copy_snap(sb.U[0], sb.W[0], sb.FB[0], k.U, k.W, k.fb);
+        for (int64_t s = s0; s < s1; ++s) {
+          const Snap& q = seg[size_t(s - s0)];
+          copy_snap(q.U, q.W, q.fb, sb.U[cur], sb.W[cur], sb.FB[cur]);
+          if (s + 1 < s1) step_forward();
+        }
       }
       for (int64_t s = s1 - 1; s >= s0; --s) {
-        const Snap& q = seg[size_t(s - s0)];
         const int in = 1 - out;
         for (int a = 0; a < 5; ++a) {
           const int64_t len = a < 2 ? nv() * N : nface(a - 2);
           if (len > 0) be.zero(adj_state[in][a], sizeof(double) * size_t(len));
         }
-        // the step's context: inputs from the stored state
-        Ctx<T> cs = c;
-        cs.U = q.U;
-        cs.W = q.W;
-        for (int d = 0; d < 3; ++d) cs.fb[d] = q.fb[d];
-        TapeExec<T, B> tx;
-        tx.be = &be;
-        tx.vals = &vals;
-        tx.adjs = &adjs;
-        std::memset(&tx.abase, 0, sizeof(tx.abase));
-        tx.abase.U = adj_state[in][0];
-        tx.abase.W = adj_state[in][1];
-        for (int d = 0; d < 3; ++d) tx.abase.fb[d] = adj_state[in][2 + d];
         // the output state's cotangents live where this step writes U2, W2, fb2
         be.d2d(adjs.U2, adj_state[out][0], sizeof(double) * size_t(nv() * N));
         be.d2d(adjs.W2, adj_state[out][1], sizeof(double) * size_t(nv() * N));
         for (int d = 0; d < 3; ++d)
           if (nface(d) > 0) be.d2d(adjs.fb2[d], adj_state[out][2 + d], sizeof(double) * size_t(nface(d)));
-        run_cycle(tx, cs);
-        if (tx.fused) return fail(PMHD_EINVAL, "the adjoint needs separate predictor launches (seg = 0)");
+        TapeExec<T, B> one;
+        if (!keep) {
+          // the step's context: inputs from the stored state
+          const Snap& q = seg[size_t(s - s0)];
+          Ctx<T> cs = c;
+          cs.U = q.U;
+          cs.W = q.W;
+          for (int d = 0; d < 3; ++d) cs.fb[d] = q.fb[d];
+          record_step(one, cs, vals, in);
+          if (one.fused) return fail(PMHD_EINVAL, "the adjoint needs separate predictor launches (seg = 0)");
+        }
+        TapeExec<T, B>& tx = keep ? txs[size_t(s - s0)] : one;
         for (size_t t = tx.tape.size(); t-- > 0;) {
           be.adjoint_select(in, t);   // the launch's accumulation plan (its targets depend on in)
           tx.tape[t]();
         }
-        if (be.adjoint_overflow())
-          return fail(PMHD_EINVAL, "adjoint element with more contributions than supported, or whose "
-                                   "contributions changed between steps");
-        const char* ae = be.error();
-        if (ae) return fail(PMHD_EHIP, ae);
         out = in;
       }
+      if (be.adjoint_overflow())
+        return fail(PMHD_EINVAL, "adjoint element with more contributions than supported, or whose "
+                                 "contributions changed between steps");
+      const char* ae = be.error();
+      if (ae) return fail(PMHD_EHIP, ae);
     }
     be.adjoint_end();
     // cotangent of the initial state
