@@ -23,6 +23,8 @@
 //
 // The loss's dependence on dt is not differentiated: dt is a fixed input.
 #pragma once
+#include <cstdlib>
+
 #include "pmhd_api.h"
 #include "pmhd_dual.h"
 
@@ -1050,7 +1052,8 @@ struct AdjointHandle : public pmhd_adjoint {
   // (then one forward pass over a segment records every step's tape)
   std::vector<StepBufs<T>> vals_more;
   bool vals_more_failed = false;
-  static constexpr int64_t kKeepBytes = int64_t(4) << 30;
+  // the budget (bytes; environment PMHD_ADJ_KEEP_BYTES at creation overrides it)
+  int64_t keep_bytes = int64_t(4) << 30;
   StepBufs<double> adjs;
   double* adj_state[2][5] = {};   // cotangents of (U, W, fb0, fb1, fb2), two sets
   double dt = 0.0;
@@ -1093,6 +1096,7 @@ struct AdjointHandle : public pmhd_adjoint {
     g = g_;
     p = p_;
     every = every_ > 0 ? every_ : 1;
+    if (const char* kb = std::getenv("PMHD_ADJ_KEEP_BYTES")) keep_bytes = std::atoll(kb);
     init_ctx(c, g, p);
     try {
       alloc_grid(c, sb, p.reconstruct_order, [&](int64_t n) { return dalloc(n); });
@@ -1217,7 +1221,7 @@ struct AdjointHandle : public pmhd_adjoint {
   const StepBufs<T>& step_vals(int64_t i) const { return i == 0 ? vals : vals_more[size_t(i - 1)]; }
 
   // whether every step of a segment of len steps can have its own value buffers
-  // (within kKeepBytes; allocated on first use)
+  // (within keep_bytes; allocated on first use)
   bool keep_values(int64_t len) {
     if (len <= 1) return true;
     if (vals_more_failed) return false;
@@ -1228,7 +1232,7 @@ struct AdjointHandle : public pmhd_adjoint {
       words += n > 0 ? n : 1;
       return static_cast<T*>(nullptr);
     });
-    if (words * int64_t(sizeof(T)) * len > kKeepBytes) {
+    if (words * int64_t(sizeof(T)) * len > keep_bytes) {
       vals_more_failed = true;
       return false;
     }

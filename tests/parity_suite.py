@@ -642,6 +642,34 @@ def test_adjoint_fp32_3d_against_fp64(lib):
         assert np.max(np.abs(a - b)) <= 1e-3 * np.max(np.abs(b)) + 1e-12
 
 
+def test_adjoint_segment_replay_matches_kept_values(lib, monkeypatch):
+    """A backward pass that keeps every step's values of a segment and one that
+    replays the segment from its checkpoint (value budget 0, environment
+    PMHD_ADJ_KEEP_BYTES) give the same bits."""
+    from pythonmhd_amd.adjoint import AdjointRun
+    shape = (12, 10)
+    rng = np.random.default_rng(41)
+    prim, faces = _generic_state(rng, shape)
+    cons = M.prim_to_cons_all(prim, GAMMA)
+    widths = widths_of(shape)
+    print('case: adjoint segment replay', flush=True)
+    grads = []
+    for budget in (None, '0'):
+        if budget is None:
+            monkeypatch.delenv('PMHD_ADJ_KEEP_BYTES', raising=False)
+        else:
+            monkeypatch.setenv('PMHD_ADJ_KEEP_BYTES', budget)
+        for prec in (64, 32):
+            run = AdjointRun(shape, widths, (1, 0), GAMMA, order=1, precision=prec, checkpoint_every=4, lib=lib)
+            _, co, _ = run.forward(prim, cons, faces, 0.004, 7)
+            gp, gc, gf = run.backward(None, _kinetic_cotangent(co), None)
+            run.close()
+            grads.append([gp, gc] + list(gf))
+    for k in range(2):
+        for x, y in zip(grads[k], grads[2 + k]):
+            assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+
+
 def test_adjoint_hydro_1d(lib):
     from pythonmhd_amd._lib import HYDRO
     from pythonmhd_amd.adjoint import AdjointRun
