@@ -606,11 +606,14 @@ PMHD_FI void cell_recon_src(const Ctx<T>& c, int d, I3 f, int64_t m, T out[2]) {
 // the predictor's conserved states and flux of a face of direction d from its
 // prepared primitive states (floors and source terms applied) and normal field
 // (the context supplies gamma and the entropy fix; its number type may differ)
-template <int S, class T, class U>
-PMHD_FI void predict_flux(const Ctx<U>& c, int d, int ke, const T wl[7], const T wr[7], T bn, T ul[7], T ur[7],
-                          T df[7]) {
+template <class T, class U>
+PMHD_FI void predict_states(const Ctx<U>& c, int ke, const T wl[7], const T wr[7], T bn, T ul[7], T ur[7]) {
   prim_to_cons(wl, bn, T(c.gamma), ke, ul);
   prim_to_cons(wr, bn, T(c.gamma), ke, ur);
+}
+template <int S, class T, class U>
+PMHD_FI void predict_roe(const Ctx<U>& c, int d, const T wl[7], const T wr[7], const T ul[7], const T ur[7], T bn,
+                         T df[7]) {
   T rwl[7], rwr[7], rul[7], rur[7], rf[7];
   to_rot(wl, d, rwl);
   to_rot(wr, d, rwr);
@@ -618,6 +621,12 @@ PMHD_FI void predict_flux(const Ctx<U>& c, int d, int ke, const T wl[7], const T
   to_rot(ur, d, rur);
   roe_flux<S>(rwl, rwr, rul, rur, bn, T(c.gamma), c.efix != 0, T(c.eps), rf);
   from_rot(rf, d, df);
+}
+template <int S, class T, class U>
+PMHD_FI void predict_flux(const Ctx<U>& c, int d, int ke, const T wl[7], const T wr[7], T bn, T ul[7], T ur[7],
+                          T df[7]) {
+  predict_states(c, ke, wl, wr, bn, ul, ur);
+  predict_roe<S>(c, d, wl, wr, ul, ur, bn, df);
 }
 
 // predictor of face f from its reconstructed states wl, wr (directional order;
@@ -646,10 +655,13 @@ PMHD_FI void predict_face_src(const Ctx<T>& c, int d, I3 f, T wl[7], T wr[7], co
   } else {
     bn = c.bx1d;
   }
+  // the states are stored before the flux is formed (they need not stay live
+  // through the Roe solver)
   T ul[7], ur[7], df[7];
-  predict_flux<S>(c, d, ke, wl, wr, bn, ul, ur, df);
+  predict_states(c, ke, wl, wr, bn, ul, ur);
   store7(c, c.UL[d], NF, idx, ul);
   store7(c, c.UR[d], NF, idx, ur);
+  predict_roe<S>(c, d, wl, wr, ul, ur, bn, df);
   store7(c, c.F[d], NF, idx, df);
 }
 
